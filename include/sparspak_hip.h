@@ -1,0 +1,212 @@
+/*
+ * sparspak_hip.h — C ABI of libsparspak_hip.so, the MI355X (gfx950) numeric
+ * phase of the Sparspak.jl sparse direct solver plus its host symbolic phase.
+ *
+ * Conventions (the reference's own): every array is caller-owned and is not
+ * retained after the call returns; integer arrays are int64 and hold 1-based
+ * index VALUES exactly as Sparspak.jl stores them; numeric arrays are updated
+ * in place. Functions return 0 on success and a negative code on failure:
+ *   -1     zero pivot in a diagonal block (SpkLUFactor.jl:230-233 meaning),
+ *          or a structural error named by the function's comment
+ *   -2     workspace / size limit
+ *   -3     sequence error (a phase called before the one it depends on)
+ *   -4     bad argument
+ *   <= -1000  HIP runtime error (-1000 - hipError_t)
+ * No C++ exception crosses this boundary. One handle serialises its calls on
+ * its own HIP stream; different handles may be used from different threads.
+ */
+#ifndef SPARSPAK_HIP_H
+#define SPARSPAK_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================================
+ * Host symbolic phase (bit-exact restatements; run on the CPU).
+ * ====================================================================== */
+
+/* SpkGraph.jl:298-316 isstructuresymmetric -> 1 or 0. */
+int64_t spk_isstructuresymmetric(int64_t nv, const int64_t* xadj,
+                                 const int64_t* adj);
+/* SpkGraph.jl:94-268 (counting pass): edge count after symmetrisation. */
+int64_t spk_symmetrized_nedges(int64_t nv, const int64_t* xadj,
+                               const int64_t* adj);
+/* SpkGraph.jl:94-289 makestructuresymmetric + sortgraph!. Returns 1 if the
+ * graph was already symmetric (copied unchanged), 0 if symmetrised. */
+int64_t spk_makestructuresymmetric(int64_t nv, const int64_t* xadj,
+                                   const int64_t* adj, int64_t* hxadj,
+                                   int64_t* hadj);
+/* SpkMMD.jl:38-209 mmd!/_generalmmd (delta = 0): perm (new->old), invp. */
+int64_t spk_mmd(int64_t n, const int64_t* xadj, const int64_t* adj,
+                int64_t* perm, int64_t* invp);
+/* SpkETree.jl:62-88 _findetree!. */
+void spk_etree(int64_t n, const int64_t* xadj, const int64_t* adj,
+               const int64_t* perm, const int64_t* invp, int64_t* parent);
+/* SpkETree.jl:106-139 _getpostorder! (weight == NULL: plain postorder). */
+void spk_postorder(int64_t n, int64_t* parent, int64_t* perm, int64_t* invp,
+                   int64_t* weight);
+/* SpkSymFct.jl:37-59 _findcolumncounts! -> nnz(L) including the diagonal. */
+int64_t spk_colcounts(int64_t n, const int64_t* xadj, const int64_t* adj,
+                      const int64_t* perm, const int64_t* invp,
+                      const int64_t* parent, int64_t* colcnt);
+/* SpkSymFct.jl:415-464 _findsupernodes! -> nsuper (xsuper needs n+1). */
+int64_t spk_supernodes(int64_t n, const int64_t* parent, const int64_t* colcnt,
+                       int64_t maxsize, int64_t* xsuper, int64_t* snode,
+                       int64_t* nofsub);
+/* SpkSparseBase.jl:253-284 findnonzeroindexs (LU panel pointers). */
+void spk_nonzero_indexs(int64_t n, const int64_t* colcnt, int64_t nsuper,
+                        const int64_t* xsuper, int64_t* xlnz, int64_t* xunz);
+/* SpkSparseSpdBase.jl:359-375 _findnonzeroindexs! (SPD panel pointers). */
+void spk_nonzero_indexs_spd(int64_t n, const int64_t* colcnt, int64_t nsuper,
+                            const int64_t* xsuper, int64_t* xlnz);
+/* SpkSymFct.jl:105-246 _symbolicfact! (-1: "Inconsistency in data
+ * structure"). */
+int64_t spk_symbfact(int64_t n, const int64_t* xadj, const int64_t* adj,
+                     const int64_t* perm, const int64_t* invp,
+                     const int64_t* colcnt, int64_t nsuper,
+                     const int64_t* xsuper, const int64_t* snode,
+                     int64_t nofsub, int64_t* xlindx, int64_t* lindx);
+/* SpkSymFct.jl:372-380 _findsupernodetree! (0 marks a root). */
+void spk_supernode_tree(int64_t nsuper, const int64_t* xsuper,
+                        const int64_t* snode, const int64_t* xlindx,
+                        const int64_t* lindx, int64_t* sparent);
+
+/* ======================================================================
+ * Device numeric phase.
+ * ====================================================================== */
+
+typedef struct spkhip_handle* spkhip_t;
+
+/* Factor kinds. */
+#define SPKHIP_KIND_LU 0  /* SparseSolver: LU, pivoting inside diagonal blocks */
+#define SPKHIP_KIND_SPD 1 /* SparseSpdSolver: LDL^T, no pivoting */
+
+/* Factor precisions. */
+#define SPKHIP_FP64 0     /* factor and solve in double */
+#define SPKHIP_FP32 1     /* factor and solve in float (refine in double) */
+
+typedef struct spkhip_stats {
+  int64_t n, nsuper, nfront, nlevel;
+  int64_t nnz_lnz, nnz_unz;       /* |lnz|, |unz| of the reference layout */
+  int64_t first_fail_supernode;   /* 1-based; 0 = no zero pivot */
+  int64_t refine_iters;
+  double factor_flops;            /* F_LU or F_Chol from the column counts */
+  double solve_flops;             /* per right-hand side */
+  double front_bytes;             /* device front/update arena, bytes */
+  double t_upload_ms, t_inmatrix_ms, t_factor_ms, t_solve_ms, t_refine_ms;
+  double backward_error;          /* last normwise backward error (refine) */
+} spkhip_stats_t;
+
+/* Handle lifetime. device = HIP device ordinal. */
+int64_t spkhip_create(spkhip_t* h, int64_t device);
+int64_t spkhip_destroy(spkhip_t h);
+
+/* Uploads the symbolic factor produced by the host phase and builds the
+ * device schedule (front tree, levels, relative-index maps, A -> factor
+ * offset map). Replaces the storage set-up at the end of
+ * SpkSparseBase.jl:193-251 (kind LU) / SpkSparseSpdBase.jl:178-232 (SPD).
+ * colptr/rowval: the CSC pattern of A in the ORIGINAL numbering
+ * (SparseCSCInterface.jl:101-169 reads the same arrays); nnz = colptr[n]-1.
+ * Returns -1 ("No space for matrix element") if an entry of A falls outside
+ * the symbolic structure. */
+int64_t spkhip_symbolic_upload(spkhip_t h, int64_t kind, int64_t n,
+                               int64_t nsuper, const int64_t* xsuper,
+                               const int64_t* snode, const int64_t* xlindx,
+                               const int64_t* lindx, const int64_t* xlnz,
+                               const int64_t* xunz, const int64_t* perm,
+                               const int64_t* invp, const int64_t* colptr,
+                               const int64_t* rowval);
+
+/* _inmatrix! (SpkSparseBase.jl:302-372, SparseCSCInterface.jl:101-169):
+ * zero lnz/unz/ipiv on the device and add every value of A (CSC order,
+ * original numbering, nnz values) into its slot. nzval is a host pointer;
+ * the _device variant takes a device pointer (resident values). */
+int64_t spkhip_inmatrix(spkhip_t h, const double* nzval);
+int64_t spkhip_inmatrix_device(spkhip_t h, const double* d_nzval);
+
+/* _factor! -> _lufactor! (SpkSparseBase.jl:378-391, SpkLUFactor.jl:60-255)
+ * or _ldltfactor! (SpkSparseSpdBase.jl:313-332). precision: SPKHIP_FP64 or
+ * SPKHIP_FP32. Returns -1 if any diagonal block met a zero pivot (the first
+ * failing supernode is in stats). */
+int64_t spkhip_factor(spkhip_t h, int64_t precision);
+
+/* _triangularsolve! (SpkSparseBase.jl:400-416) for nrhs right-hand sides
+ * stored column-major in B (host, leading dimension ldb, original
+ * numbering); overwritten by the solution. Uses the last factor. */
+int64_t spkhip_solve(spkhip_t h, int64_t nrhs, double* b, int64_t ldb);
+/* Same with device-resident B (double, column-major). */
+int64_t spkhip_solve_device(spkhip_t h, int64_t nrhs, double* d_b,
+                            int64_t ldb);
+
+/* Solve with iterative refinement: the residual r = b - A x and the update
+ * x += d run in double on the device, the correction solve uses the last
+ * factor (double or float). Stops when the normwise backward error
+ * ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) <= tol for every
+ * column, or after max_iter corrections. Host B (as spkhip_solve). */
+int64_t spkhip_solve_refine(spkhip_t h, int64_t nrhs, double* b, int64_t ldb,
+                            double tol, int64_t max_iter);
+int64_t spkhip_solve_refine_device(spkhip_t h, int64_t nrhs, double* d_b,
+                                   int64_t ldb, double tol, int64_t max_iter);
+
+/* Copies the factor back in the reference layout: lnz (|lnz|), unz (|unz|),
+ * ipvt (n, local 1..nj pivots). Double variant converts a float factor. */
+int64_t spkhip_download(spkhip_t h, double* lnz, double* unz, int64_t* ipvt);
+int64_t spkhip_download_f32(spkhip_t h, float* lnz, float* unz, int64_t* ipvt);
+
+int64_t spkhip_stats(spkhip_t h, spkhip_stats_t* st);
+
+/* Stateless drop-ins with the reference's argument lists (upload, compute on
+ * the device, download). lnz/unz hold the _inmatrix! output on entry. */
+/* SpkLUFactor.jl:60 _lufactor! */
+int64_t spkhip_lufactor_f64(int64_t n, int64_t nsuper, const int64_t* xsuper,
+                            const int64_t* snode, const int64_t* xlindx,
+                            const int64_t* lindx, const int64_t* xlnz,
+                            double* lnz, const int64_t* xunz, double* unz,
+                            int64_t* ipvt);
+int64_t spkhip_lufactor_f32(int64_t n, int64_t nsuper, const int64_t* xsuper,
+                            const int64_t* snode, const int64_t* xlindx,
+                            const int64_t* lindx, const int64_t* xlnz,
+                            float* lnz, const int64_t* xunz, float* unz,
+                            int64_t* ipvt);
+/* SpkLUFactor.jl:269 _lulsolve! */
+int64_t spkhip_lulsolve_f64(int64_t nsuper, const int64_t* xsuper,
+                            const int64_t* xlindx, const int64_t* lindx,
+                            const int64_t* xlnz, const double* lnz,
+                            const int64_t* ipiv, double* rhs);
+int64_t spkhip_lulsolve_f32(int64_t nsuper, const int64_t* xsuper,
+                            const int64_t* xlindx, const int64_t* lindx,
+                            const int64_t* xlnz, const float* lnz,
+                            const int64_t* ipiv, float* rhs);
+/* SpkLUFactor.jl:325 _luusolve! */
+int64_t spkhip_luusolve_f64(int64_t n, int64_t nsuper, const int64_t* xsuper,
+                            const int64_t* xlindx, const int64_t* lindx,
+                            const int64_t* xlnz, const double* lnz,
+                            const int64_t* xunz, const double* unz,
+                            double* rhs);
+int64_t spkhip_luusolve_f32(int64_t n, int64_t nsuper, const int64_t* xsuper,
+                            const int64_t* xlindx, const int64_t* lindx,
+                            const int64_t* xlnz, const float* lnz,
+                            const int64_t* xunz, const float* unz, float* rhs);
+/* SpkLDLtFactor.jl:58 _ldltfactor! (corrected numerics, same layout) */
+int64_t spkhip_ldltfactor_f64(int64_t n, int64_t nsuper, const int64_t* xsuper,
+                              const int64_t* snode, const int64_t* xlindx,
+                              const int64_t* lindx, const int64_t* xlnz,
+                              double* lnz);
+/* SpkLDLtFactor.jl:266 _ldltsolve! */
+int64_t spkhip_ldltsolve_f64(int64_t nsuper, const int64_t* xsuper,
+                             const int64_t* xlindx, const int64_t* lindx,
+                             const int64_t* xlnz, const double* lnz,
+                             double* rhs);
+
+/* Library/device information. */
+int64_t spkhip_device_count(void);
+const char* spkhip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPARSPAK_HIP_H */

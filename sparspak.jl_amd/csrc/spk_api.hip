@@ -1,0 +1,1312 @@
+// C ABI of the device numeric phase: handle, schedule construction (host),
+// launches and the stateless drop-ins. Declarations: include/sparspak_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "sparspak_hip.h"
+#include "spk_kernels.h"
+#include "spk_kernels.hip"
+
+namespace {
+
+using i64 = int64_t;
+using i32 = int32_t;
+using spk::DevSym;
+using spk::StepDesc;
+using spk::kWG;
+
+constexpr int kSmallLmax = 64;  // fronts with L <= this factor inside LDS
+
+struct HipFail {
+  hipError_t e;
+};
+inline void hipck(hipError_t e) {
+  if (e != hipSuccess) throw HipFail{e};
+}
+struct ApiError {
+  int64_t code;
+};
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    reset();
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipck(hipMalloc(&p, bytes));
+    n = count;
+  }
+  void ensure(size_t count) {
+    if (count > n || !p) alloc(count);
+  }
+  void upload(const T* h, size_t count, hipStream_t st) {
+    alloc(count);
+    if (count) hipck(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+  }
+  void upload(const std::vector<T>& v, hipStream_t st) { upload(v.data(), v.size(), st); }
+};
+
+struct LevelPlan {
+  int small_off = 0, small_cnt = 0, small_maxL = 0;
+  int items_off = 0, items_cnt = 0;
+  struct Step {
+    int pd_off, pd_cnt, pwg;
+    int ud_off, ud_cnt, uwg;
+  };
+  std::vector<Step> steps;
+  int solve_off = 0, solve_cnt = 0;
+};
+
+}  // namespace
+
+struct spkhip_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // symbolic
+  bool have_sym = false;
+  int64_t kind = SPKHIP_KIND_LU;
+  i64 n = 0, nsuper = 0, nsub = 0, nlnz = 0, nunz = 0;
+  std::vector<i64> xsuper, snode, xlnz, xunz, xlindx, perm, invp;
+  std::vector<i32> lindx;
+  // fronts
+  i32 nfront = 0, nlevel = 0;
+  std::vector<i32> fL, fW, fc0, fsp0, fnp, fcbld, fchp, fch, fpar, flev, relmap;
+  std::vector<i64> foff, fcb, frel, fvec;
+  i64 arena_elems = 0, vec_elems = 0;
+  std::vector<LevelPlan> levels;
+  i64 max_step_fronts = 0;
+  double factor_flops = 0, solve_flops = 0;
+  // device symbolic
+  DevBuf<i64> d_xsuper, d_snode, d_xlnz, d_xunz, d_xlindx, d_perm, d_invp;
+  DevBuf<i32> d_lindx, d_fL, d_fW, d_fc0, d_fsp0, d_fnp, d_fcbld, d_fchp, d_fch,
+      d_relmap, d_small, d_it_front, d_it_col, d_ea_child, d_ea_t, d_solve;
+  DevBuf<i64> d_foff, d_fcb, d_frel, d_fvec, d_it_ea;
+  DevBuf<StepDesc> d_pdesc, d_udesc;
+  DevSym ds{};
+  // matrix
+  i64 nnz = 0;
+  bool have_csc = false, pattern_error = false;
+  DevBuf<i64> d_amap, d_rowptr;
+  DevBuf<i32> d_colind;
+  DevBuf<i64> d_csc2csr;
+  DevBuf<double> d_nzval, d_csrval, d_rowabs;
+  double anorm = 0;
+  bool have_values = false;
+  // numeric
+  int precision = -1;  // of the current factor
+  bool factored = false;
+  DevBuf<double> d_lnz64, d_unz64, d_arena64, d_stage64, d_vec64, d_rhs64;
+  DevBuf<float> d_lnz32, d_unz32, d_arena32, d_stage32, d_vec32, d_rhs32, d_tmp32;
+  DevBuf<i64> d_ipvt;
+  DevBuf<unsigned long long> d_fail;
+  i64 first_fail = 0;
+  // solve / refine work
+  DevBuf<double> d_b, d_x, d_r, d_tmp64, d_norms;
+  int refine_iters = 0;
+  double backward_error = 0;
+  // timing
+  double t_upload = 0, t_inmatrix = 0, t_factor = 0, t_solve = 0, t_refine = 0;
+};
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+inline double ms_since(Clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
+inline unsigned grid_for(i64 n, int bs = kWG) {
+  i64 g = (n + bs - 1) / bs;
+  if (g < 1) g = 1;
+  if (g > 65535 * 64) g = 65535 * 64;
+  return (unsigned)g;
+}
+
+// -------------------------------------------------------------------------
+// Front construction. Consecutive supernodes J-1, J belong to one front
+// (fundamental supernode) when J's index list is J-1's list minus J-1's own
+// columns: jlen(J-1) - nj(J-1) == jlen(J) and the first off-block index of
+// J-1 is J's first column. This is the adjacency rule of the first pass of
+// _findsupernodes! (SpkSymFct.jl:424-429) seen through the index lists.
+// -------------------------------------------------------------------------
+void build_fronts(spkhip_handle* h) {
+  const i64 ns = h->nsuper;
+  auto jlen = [&](i64 J) { return h->xlindx[J + 1] - h->xlindx[J]; };
+  auto nj = [&](i64 J) { return h->xsuper[J + 1] - h->xsuper[J]; };
+  std::vector<i32> piece_front(ns);
+  h->fsp0.clear();
+  h->fnp.clear();
+  for (i64 J = 0; J < ns; ++J) {
+    bool merge = false;
+    if (J > 0) {
+      i64 P = J - 1;
+      if (jlen(P) > nj(P) && jlen(P) - nj(P) == jlen(J) &&
+          h->lindx[h->xlindx[P] - 1 + nj(P)] == h->xsuper[J])
+        merge = true;
+    }
+    if (!merge) {
+      h->fsp0.push_back((i32)J);
+      h->fnp.push_back(0);
+    }
+    h->fnp.back() += 1;
+    piece_front[J] = (i32)h->fsp0.size() - 1;
+  }
+  const i32 nf = (i32)h->fsp0.size();
+  h->nfront = nf;
+  h->fL.assign(nf, 0);
+  h->fW.assign(nf, 0);
+  h->fc0.assign(nf, 0);
+  h->fpar.assign(nf, -1);
+  for (i32 f = 0; f < nf; ++f) {
+    i64 sp0 = h->fsp0[f], np = h->fnp[f];
+    h->fL[f] = (i32)jlen(sp0);
+    h->fW[f] = (i32)(h->xsuper[sp0 + np] - h->xsuper[sp0]);
+    h->fc0[f] = (i32)(h->xsuper[sp0] - 1);
+    if (h->fL[f] > h->fW[f]) {
+      i64 g = h->lindx[h->xlindx[sp0] - 1 + h->fW[f]] - 1;
+      h->fpar[f] = piece_front[h->snode[g] - 1];
+      if (h->fpar[f] <= f) throw ApiError{-4};
+    }
+  }
+  // children (ascending front order), levels
+  std::vector<i32> cnt(nf + 1, 0);
+  for (i32 f = 0; f < nf; ++f)
+    if (h->fpar[f] >= 0) cnt[h->fpar[f] + 1]++;
+  h->fchp.assign(nf + 1, 0);
+  for (i32 f = 0; f < nf; ++f) h->fchp[f + 1] = h->fchp[f] + cnt[f + 1];
+  h->fch.assign(h->fchp[nf], 0);
+  std::vector<i32> fillp(h->fchp.begin(), h->fchp.end() - 1);
+  for (i32 f = 0; f < nf; ++f)
+    if (h->fpar[f] >= 0) h->fch[fillp[h->fpar[f]]++] = f;
+  h->flev.assign(nf, 0);
+  for (i32 f = 0; f < nf; ++f)
+    if (h->fpar[f] >= 0)
+      h->flev[h->fpar[f]] = std::max(h->flev[h->fpar[f]], h->flev[f] + 1);
+  h->nlevel = 0;
+  for (i32 f = 0; f < nf; ++f) h->nlevel = std::max(h->nlevel, h->flev[f] + 1);
+  // relative-index maps: position of each update row of f in its parent's
+  // index list (the _ldindx!/_igathr! maps of SpkSpdMMOps.jl:70-102 taken
+  // per front instead of per supernode pair).
+  h->frel.assign(nf, 0);
+  i64 relsz = 0;
+  for (i32 f = 0; f < nf; ++f) {
+    h->frel[f] = relsz;
+    relsz += h->fL[f] - h->fW[f];
+  }
+  h->relmap.assign(std::max<i64>(relsz, 1), 0);
+  std::vector<i32> pos(h->n, -1), stamp(h->n, -1);
+  for (i32 p = 0; p < nf; ++p) {
+    if (h->fchp[p] == h->fchp[p + 1]) continue;
+    const i32* Ip = h->lindx.data() + (h->xlindx[h->fsp0[p]] - 1);
+    for (i32 k = 0; k < h->fL[p]; ++k) {
+      pos[Ip[k] - 1] = k;
+      stamp[Ip[k] - 1] = p;
+    }
+    for (i32 ci = h->fchp[p]; ci < h->fchp[p + 1]; ++ci) {
+      i32 c = h->fch[ci];
+      const i32* Ic = h->lindx.data() + (h->xlindx[h->fsp0[c]] - 1);
+      for (i32 t = 0; t < h->fL[c] - h->fW[c]; ++t) {
+        i32 g = Ic[h->fW[c] + t] - 1;
+        if (stamp[g] != p) throw ApiError{-4};  // structure not nested
+        h->relmap[h->frel[c] + t] = pos[g];
+      }
+    }
+  }
+  // storage: small fronts keep only their update matrix, large fronts the
+  // whole L x L work matrix (update matrix = its trailing block)
+  h->foff.assign(nf, 0);
+  h->fcb.assign(nf, 0);
+  h->fcbld.assign(nf, 0);
+  h->fvec.assign(nf, 0);
+  i64 off = 0, voff = 0;
+  for (i32 f = 0; f < nf; ++f) {
+    i64 L = h->fL[f], W = h->fW[f], R = L - W;
+    if (L <= kSmallLmax) {
+      h->foff[f] = off;
+      h->fcb[f] = off;
+      h->fcbld[f] = (i32)std::max<i64>(R, 1);
+      off += R * R;
+    } else {
+      h->foff[f] = off;
+      h->fcb[f] = off + W + W * L;
+      h->fcbld[f] = (i32)L;
+      off += L * L;
+    }
+    h->fvec[f] = voff;
+    voff += L;
+  }
+  h->arena_elems = off;
+  h->vec_elems = voff;
+}
+
+// Per-level launch plan: small-front list, large-front items (one per
+// front column, each with its list of (child, update column) pairs that
+// land in that column), panel/update step descriptors, solve lists.
+void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_front,
+                std::vector<i32>& it_col, std::vector<i64>& it_ea,
+                std::vector<i32>& ea_child, std::vector<i32>& ea_t,
+                std::vector<StepDesc>& pdesc, std::vector<StepDesc>& udesc,
+                std::vector<i32>& solve) {
+  const i32 nf = h->nfront;
+  std::vector<std::vector<i32>> bylev(h->nlevel);
+  for (i32 f = 0; f < nf; ++f) bylev[h->flev[f]].push_back(f);
+  h->levels.assign(h->nlevel, LevelPlan{});
+  it_ea.assign(1, 0);
+  h->max_step_fronts = 1;
+  std::vector<i32> colcnt;
+  for (i32 l = 0; l < h->nlevel; ++l) {
+    LevelPlan& lp = h->levels[l];
+    lp.small_off = (int)small.size();
+    lp.solve_off = (int)solve.size();
+    std::vector<i32> large;
+    for (i32 f : bylev[l]) {
+      solve.push_back(f);
+      if (h->fL[f] <= kSmallLmax) {
+        small.push_back(f);
+        lp.small_maxL = std::max(lp.small_maxL, h->fL[f]);
+      } else {
+        large.push_back(f);
+      }
+    }
+    lp.small_cnt = (int)small.size() - lp.small_off;
+    lp.solve_cnt = (int)solve.size() - lp.solve_off;
+    lp.items_off = (int)it_front.size();
+    int maxnp = 0;
+    for (i32 f : large) {
+      const i32 L = h->fL[f];
+      maxnp = std::max(maxnp, h->fnp[f]);
+      // extend-add entries per destination column
+      colcnt.assign(L + 1, 0);
+      for (i32 ci = h->fchp[f]; ci < h->fchp[f + 1]; ++ci) {
+        i32 c = h->fch[ci];
+        for (i32 t = 0; t < h->fL[c] - h->fW[c]; ++t) colcnt[h->relmap[h->frel[c] + t] + 1]++;
+      }
+      for (i32 j = 0; j < L; ++j) colcnt[j + 1] += colcnt[j];
+      i64 base = (i64)ea_child.size();
+      ea_child.resize(base + colcnt[L]);
+      ea_t.resize(base + colcnt[L]);
+      std::vector<i32> fillp(colcnt.begin(), colcnt.end() - 1);
+      for (i32 ci = h->fchp[f]; ci < h->fchp[f + 1]; ++ci) {
+        i32 c = h->fch[ci];
+        for (i32 t = 0; t < h->fL[c] - h->fW[c]; ++t) {
+          i32 j = h->relmap[h->frel[c] + t];
+          i64 e = base + fillp[j]++;
+          ea_child[e] = c;
+          ea_t[e] = t;
+        }
+      }
+      for (i32 j = 0; j < L; ++j) {
+        it_front.push_back(f);
+        it_col.push_back(j);
+        it_ea.push_back(base + colcnt[j + 1]);
+      }
+    }
+    lp.items_cnt = (int)it_front.size() - lp.items_off;
+    for (int st = 0; st < maxnp; ++st) {
+      LevelPlan::Step S{};
+      S.pd_off = (int)pdesc.size();
+      S.ud_off = (int)udesc.size();
+      int pwg = 0, uwg = 0, slot = 0;
+      for (i32 f : large) {
+        if (st >= h->fnp[f]) continue;
+        i32 p = h->fsp0[f] + st;
+        i64 cq = h->xsuper[p] - 1 - h->fc0[f];
+        i64 nq = h->xsuper[p + 1] - h->xsuper[p];
+        i64 m = h->fL[f] - cq - nq;
+        int nl = m > 0 ? (int)((m + kWG - 1) / kWG) : 0;
+        int nu = m > 0 ? (int)((m + spk::kUcols - 1) / spk::kUcols) : 0;
+        StepDesc pd{f, p, pwg, nl, slot};
+        pdesc.push_back(pd);
+        pwg += std::max(1, nl + nu);
+        int nt = m > 0 ? (int)((m + spk::kTile - 1) / spk::kTile) : 0;
+        StepDesc ud{f, p, uwg, nt, slot};
+        udesc.push_back(ud);
+        uwg += nt * nt + 1;
+        ++slot;
+      }
+      S.pd_cnt = (int)pdesc.size() - S.pd_off;
+      S.ud_cnt = (int)udesc.size() - S.ud_off;
+      S.pwg = pwg;
+      S.uwg = uwg;
+      h->max_step_fronts = std::max<i64>(h->max_step_fronts, slot);
+      lp.steps.push_back(S);
+    }
+  }
+}
+
+void sync(spkhip_handle* h) { hipck(hipStreamSynchronize(h->stream)); }
+
+// Map of each CSC entry of A (original numbering) to its lnz/unz slot, with
+// the reference's search semantics (SparseCSCInterface.jl:112-163): found ->
+// slot; a larger index met first -> "No space" error; list exhausted ->
+// entry ignored. The index lists are sorted, so a binary search replaces the
+// linear scan. SPD keeps only entries on or below the diagonal
+// (SpkSparseSpdBase.jl:274-282) and puts them in the lower triangle.
+void build_amap(spkhip_handle* h, const i64* colptr, const i64* rowval,
+                std::vector<i64>& amap) {
+  const i64 n = h->n;
+  amap.assign(h->nnz, -1);
+  h->pattern_error = false;
+  auto search = [&](i64 sup, i64 from, i64 target, i64& loc) -> int {
+    // returns 1 found, 0 exhausted, -1 error; loc = position counted from
+    // index `from` of the supernode's list
+    const i32* I = h->lindx.data() + (h->xlindx[sup] - 1);
+    i64 len = h->xlindx[sup + 1] - h->xlindx[sup];
+    const i32* b = I + from;
+    const i32* e = I + len;
+    if (b >= e) return 0;
+    const i32* it = std::lower_bound(b, e, (i32)target);
+    if (it == e) return 0;
+    if (*it != target) return -1;
+    loc = it - b;
+    return 1;
+  };
+  for (i64 i = 1; i <= n; ++i) {
+    for (i64 iptr = colptr[i - 1]; iptr <= colptr[i] - 1; ++iptr) {
+      i64 k = iptr - 1;
+      i64 orow = rowval[k];
+      if (h->kind == SPKHIP_KIND_SPD && orow < i) continue;
+      i64 inew = h->invp[orow - 1];
+      i64 jnew = h->invp[i - 1];
+      if (h->kind == SPKHIP_KIND_SPD && inew < jnew) std::swap(inew, jnew);
+      i64 loc = 0;
+      i64 js = h->snode[jnew - 1];
+      if (inew >= h->xsuper[js - 1]) {
+        int r = search(js - 1, 0, inew, loc);
+        if (r < 0) h->pattern_error = true;
+        if (r > 0) amap[k] = h->xlnz[jnew - 1] - 1 + loc;
+      } else {
+        i64 is = h->snode[inew - 1];
+        i64 width = h->xsuper[is] - h->xsuper[is - 1];
+        int r = search(is - 1, width, jnew, loc);
+        if (r < 0) h->pattern_error = true;
+        if (r > 0) amap[k] = -(h->xunz[inew - 1] - 1 + loc) - 2;
+      }
+    }
+  }
+}
+
+template <typename T>
+struct Bufs;
+template <>
+struct Bufs<double> {
+  static DevBuf<double>& lnz(spkhip_handle* h) { return h->d_lnz64; }
+  static DevBuf<double>& unz(spkhip_handle* h) { return h->d_unz64; }
+  static DevBuf<double>& arena(spkhip_handle* h) { return h->d_arena64; }
+  static DevBuf<double>& stage(spkhip_handle* h) { return h->d_stage64; }
+  static DevBuf<double>& vec(spkhip_handle* h) { return h->d_vec64; }
+  static DevBuf<double>& rhs(spkhip_handle* h) { return h->d_rhs64; }
+};
+template <>
+struct Bufs<float> {
+  static DevBuf<float>& lnz(spkhip_handle* h) { return h->d_lnz32; }
+  static DevBuf<float>& unz(spkhip_handle* h) { return h->d_unz32; }
+  static DevBuf<float>& arena(spkhip_handle* h) { return h->d_arena32; }
+  static DevBuf<float>& stage(spkhip_handle* h) { return h->d_stage32; }
+  static DevBuf<float>& vec(spkhip_handle* h) { return h->d_vec32; }
+  static DevBuf<float>& rhs(spkhip_handle* h) { return h->d_rhs32; }
+};
+
+template <typename T, int MAXW>
+void set_lds_attrs() {
+  static bool done = false;
+  if (done) return;
+  hipck(hipFuncSetAttribute((const void*)spk::small_front_kernel<T>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            kSmallLmax * kSmallLmax * (int)sizeof(T)));
+  done = true;
+}
+
+// The factorisation proper on already assembled lnz/unz (reference layout).
+template <typename T, int MAXW>
+void run_factor(spkhip_handle* h) {
+  hipStream_t st = h->stream;
+  DevBuf<T>& lnz = Bufs<T>::lnz(h);
+  DevBuf<T>& unz = Bufs<T>::unz(h);
+  DevBuf<T>& arena = Bufs<T>::arena(h);
+  DevBuf<T>& stage = Bufs<T>::stage(h);
+  arena.ensure((size_t)std::max<i64>(h->arena_elems, 1));
+  stage.ensure((size_t)h->max_step_fronts * MAXW * MAXW);
+  h->d_ipvt.ensure((size_t)h->n);
+  h->d_fail.ensure(1);
+  hipLaunchKernelGGL(spk::fill_kernel<unsigned long long>, dim3(1), dim3(kWG), 0, st,
+                     h->d_fail.p, (int64_t)1, ~0ull);
+  set_lds_attrs<T, MAXW>();
+  for (const LevelPlan& lp : h->levels) {
+    if (lp.small_cnt > 0) {
+      size_t lds = (size_t)lp.small_maxL * lp.small_maxL * sizeof(T);
+      hipLaunchKernelGGL(spk::small_front_kernel<T>, dim3(lp.small_cnt), dim3(kWG), lds, st,
+                         h->ds, h->d_small.p + lp.small_off, lnz.p, unz.p, h->d_ipvt.p,
+                         arena.p, h->d_fail.p);
+      hipck(hipGetLastError());
+    }
+    if (lp.items_cnt > 0) {
+      hipLaunchKernelGGL(spk::large_init_kernel<T>, dim3(lp.items_cnt), dim3(kWG), 0, st, h->ds,
+                         h->d_it_front.p + lp.items_off, h->d_it_col.p + lp.items_off,
+                         h->d_it_ea.p + lp.items_off, h->d_ea_child.p, h->d_ea_t.p, lnz.p,
+                         unz.p, arena.p);
+      hipck(hipGetLastError());
+      for (const LevelPlan::Step& S : lp.steps) {
+        hipLaunchKernelGGL((spk::large_panel_kernel<T, MAXW>), dim3(S.pwg), dim3(kWG), 0, st,
+                           h->ds, h->d_pdesc.p + S.pd_off, S.pd_cnt, arena.p, stage.p,
+                           h->d_ipvt.p, h->d_fail.p);
+        hipck(hipGetLastError());
+        hipLaunchKernelGGL((spk::large_update_kernel<T, MAXW>), dim3(S.uwg), dim3(kWG), 0, st,
+                           h->ds, h->d_udesc.p + S.ud_off, S.ud_cnt, arena.p, stage.p);
+        hipck(hipGetLastError());
+      }
+      hipLaunchKernelGGL(spk::large_writeback_kernel<T>, dim3(lp.items_cnt), dim3(kWG), 0, st,
+                         h->ds, h->d_it_front.p + lp.items_off, h->d_it_col.p + lp.items_off,
+                         lnz.p, unz.p, arena.p);
+      hipck(hipGetLastError());
+    }
+  }
+  unsigned long long ff = 0;
+  hipck(hipMemcpyAsync(&ff, h->d_fail.p, sizeof(ff), hipMemcpyDeviceToHost, st));
+  sync(h);
+  h->first_fail = (ff == ~0ull) ? 0 : (i64)ff;
+}
+
+// Scatter A's values (resident, double) into zeroed lnz/unz of type T.
+template <typename T>
+void run_assemble(spkhip_handle* h) {
+  hipStream_t st = h->stream;
+  DevBuf<T>& lnz = Bufs<T>::lnz(h);
+  DevBuf<T>& unz = Bufs<T>::unz(h);
+  lnz.ensure((size_t)std::max<i64>(h->nlnz, 1));
+  unz.ensure((size_t)std::max<i64>(h->nunz, 1));
+  hipLaunchKernelGGL(spk::fill_kernel<T>, dim3(grid_for(h->nlnz)), dim3(kWG), 0, st, lnz.p,
+                     h->nlnz, T(0));
+  hipLaunchKernelGGL(spk::fill_kernel<T>, dim3(grid_for(h->nunz)), dim3(kWG), 0, st, unz.p,
+                     h->nunz, T(0));
+  if (h->nnz > 0)
+    hipLaunchKernelGGL(spk::scatter_a_kernel<T>, dim3(grid_for(h->nnz)), dim3(kWG), 0, st,
+                       h->d_nzval.p, h->d_amap.p, h->nnz, lnz.p, unz.p);
+  hipck(hipGetLastError());
+}
+
+template <typename T>
+void solve_levels(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bwd) {
+  hipStream_t st = h->stream;
+  DevBuf<T>& vec = Bufs<T>::vec(h);
+  vec.ensure((size_t)std::max<i64>(h->vec_elems * nrhs, 1));
+  const T* lnz = Bufs<T>::lnz(h).p;
+  const T* unz = Bufs<T>::unz(h).p;
+  if (fwd)
+    for (const LevelPlan& lp : h->levels) {
+      if (lp.solve_cnt == 0) continue;
+      hipLaunchKernelGGL(spk::fwd_solve_kernel<T>, dim3(lp.solve_cnt), dim3(kWG), 0, st, h->ds,
+                         h->d_solve.p + lp.solve_off, lnz, h->d_ipvt.p, rhs, ldr, nrhs, vec.p);
+      hipck(hipGetLastError());
+    }
+  if (bwd)
+    for (int l = (int)h->levels.size() - 1; l >= 0; --l) {
+      const LevelPlan& lp = h->levels[l];
+      if (lp.solve_cnt == 0) continue;
+      hipLaunchKernelGGL(spk::bwd_solve_kernel<T>, dim3(lp.solve_cnt), dim3(kWG), 0, st, h->ds,
+                         h->d_solve.p + lp.solve_off, lnz, unz, rhs, ldr, nrhs, vec.p);
+      hipck(hipGetLastError());
+    }
+}
+
+// Full solve of device B (double, original numbering) with the current
+// factor: rhs = b[perm]; forward; backward; x = rhs[invp].
+template <typename T>
+void solve_device(spkhip_handle* h, double* d_b, i64 ldb, int nrhs) {
+  hipStream_t st = h->stream;
+  DevBuf<T>& rhs = Bufs<T>::rhs(h);
+  rhs.ensure((size_t)h->n * nrhs);
+  hipLaunchKernelGGL((spk::permute_kernel<double, T>), dim3(grid_for(h->n)), dim3(kWG), 0, st,
+                     d_b, ldb, rhs.p, h->n, h->d_perm.p, h->n, nrhs);
+  solve_levels<T>(h, rhs.p, h->n, nrhs, true, true);
+  hipLaunchKernelGGL((spk::permute_kernel<T, double>), dim3(grid_for(h->n)), dim3(kWG), 0, st,
+                     rhs.p, h->n, d_b, ldb, h->d_invp.p, h->n, nrhs);
+  hipck(hipGetLastError());
+}
+
+void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i64* xsuper,
+                     const i64* snode, const i64* xlindx, const i64* lindx, const i64* xlnz,
+                     const i64* xunz, const i64* perm, const i64* invp) {
+  auto t0 = Clock::now();
+  hipck(hipSetDevice(h->device));
+  h->kind = kind;
+  h->n = n;
+  h->nsuper = nsuper;
+  h->xsuper.assign(xsuper, xsuper + nsuper + 1);
+  if (snode) {
+    h->snode.assign(snode, snode + n);
+  } else {
+    h->snode.assign(n, 0);
+    for (i64 J = 0; J < nsuper; ++J)
+      for (i64 c = xsuper[J]; c < xsuper[J + 1]; ++c) h->snode[c - 1] = J + 1;
+  }
+  h->xlindx.assign(xlindx, xlindx + nsuper + 1);
+  h->nsub = h->xlindx[nsuper] - 1;
+  h->lindx.resize(std::max<i64>(h->nsub, 1));
+  for (i64 k = 0; k < h->nsub; ++k) h->lindx[k] = (i32)lindx[k];
+  h->xlnz.assign(xlnz, xlnz + n + 1);
+  h->xunz.resize(n + 1);
+  if (xunz && kind == SPKHIP_KIND_LU) {
+    h->xunz.assign(xunz, xunz + n + 1);
+  } else {  // U panels of the LU form (findnonzeroindexs rule)
+    i64 up = 1;
+    for (i64 J = 0; J < nsuper; ++J) {
+      i64 w = xsuper[J + 1] - xsuper[J];
+      i64 jl = h->xlindx[J + 1] - h->xlindx[J];
+      for (i64 c = xsuper[J]; c < xsuper[J + 1]; ++c) {
+        h->xunz[c - 1] = up;
+        up += jl - w;
+      }
+    }
+    h->xunz[n] = up;
+  }
+  h->nlnz = h->xlnz[n] - 1;
+  h->nunz = h->xunz[n] - 1;
+  h->perm.resize(n);
+  h->invp.resize(n);
+  for (i64 i = 0; i < n; ++i) {
+    h->perm[i] = perm ? perm[i] : i + 1;
+    h->invp[i] = invp ? invp[i] : i + 1;
+  }
+  // flops from the index lists (colcnt[j] = jlen of j's supernode minus the
+  // columns of the supernode before j)
+  double fl = 0, sl = 0;
+  for (i64 J = 0; J < nsuper; ++J) {
+    i64 jl = h->xlindx[J + 1] - h->xlindx[J];
+    for (i64 c = 0; c < xsuper[J + 1] - xsuper[J]; ++c) {
+      double cc = (double)(jl - c), m = cc - 1.0;
+      fl += (kind == SPKHIP_KIND_LU) ? (m + 2.0 * m * m) : cc * cc;
+      sl += cc;
+    }
+  }
+  h->factor_flops = fl;
+  h->solve_flops = (kind == SPKHIP_KIND_LU) ? 2.0 * (2.0 * sl - (double)n) - 3.0 * n
+                                            : 4.0 * sl - 3.0 * n;
+  const i64 wmax = (kind == SPKHIP_KIND_LU) ? 64 : 96;
+  for (i64 J = 0; J < nsuper; ++J)
+    if (xsuper[J + 1] - xsuper[J] > wmax) throw ApiError{-2};
+  build_fronts(h);
+  std::vector<i32> small, it_front, it_col, ea_child, ea_t, solve;
+  std::vector<i64> it_ea;
+  std::vector<StepDesc> pdesc, udesc;
+  build_plan(h, small, it_front, it_col, it_ea, ea_child, ea_t, pdesc, udesc, solve);
+  hipStream_t st = h->stream;
+  h->d_xsuper.upload(h->xsuper, st);
+  h->d_snode.upload(h->snode, st);
+  h->d_xlnz.upload(h->xlnz, st);
+  h->d_xunz.upload(h->xunz, st);
+  h->d_xlindx.upload(h->xlindx, st);
+  h->d_lindx.upload(h->lindx, st);
+  h->d_perm.upload(h->perm, st);
+  h->d_invp.upload(h->invp, st);
+  h->d_fL.upload(h->fL, st);
+  h->d_fW.upload(h->fW, st);
+  h->d_fc0.upload(h->fc0, st);
+  h->d_fsp0.upload(h->fsp0, st);
+  h->d_fnp.upload(h->fnp, st);
+  h->d_foff.upload(h->foff, st);
+  h->d_fcb.upload(h->fcb, st);
+  h->d_fcbld.upload(h->fcbld, st);
+  h->d_frel.upload(h->frel, st);
+  h->d_relmap.upload(h->relmap, st);
+  h->d_fchp.upload(h->fchp, st);
+  if (h->fch.empty()) h->fch.push_back(0);
+  h->d_fch.upload(h->fch, st);
+  h->d_fvec.upload(h->fvec, st);
+  if (small.empty()) small.push_back(0);
+  h->d_small.upload(small, st);
+  if (it_front.empty()) {
+    it_front.push_back(0);
+    it_col.push_back(0);
+  }
+  h->d_it_front.upload(it_front, st);
+  h->d_it_col.upload(it_col, st);
+  h->d_it_ea.upload(it_ea, st);
+  if (ea_child.empty()) {
+    ea_child.push_back(0);
+    ea_t.push_back(0);
+  }
+  h->d_ea_child.upload(ea_child, st);
+  h->d_ea_t.upload(ea_t, st);
+  if (pdesc.empty()) pdesc.push_back(StepDesc{});
+  if (udesc.empty()) udesc.push_back(StepDesc{});
+  h->d_pdesc.upload(pdesc, st);
+  h->d_udesc.upload(udesc, st);
+  if (solve.empty()) solve.push_back(0);
+  h->d_solve.upload(solve, st);
+  DevSym& d = h->ds;
+  d.n = n;
+  d.nsuper = nsuper;
+  d.nfront = h->nfront;
+  d.xsuper = h->d_xsuper.p;
+  d.snode = h->d_snode.p;
+  d.xlnz = h->d_xlnz.p;
+  d.xunz = h->d_xunz.p;
+  d.xlindx = h->d_xlindx.p;
+  d.lindx = h->d_lindx.p;
+  d.fL = h->d_fL.p;
+  d.fW = h->d_fW.p;
+  d.fc0 = h->d_fc0.p;
+  d.fsp0 = h->d_fsp0.p;
+  d.fnp = h->d_fnp.p;
+  d.foff = h->d_foff.p;
+  d.fcb = h->d_fcb.p;
+  d.fcbld = h->d_fcbld.p;
+  d.frel = h->d_frel.p;
+  d.relmap = h->d_relmap.p;
+  d.fchp = h->d_fchp.p;
+  d.fch = h->d_fch.p;
+  d.fvec = h->d_fvec.p;
+  d.pivot = (kind == SPKHIP_KIND_LU) ? 1 : 0;
+  d.sym = (kind == SPKHIP_KIND_SPD) ? 1 : 0;
+  sync(h);
+  h->have_sym = true;
+  h->have_csc = false;
+  h->have_values = false;
+  h->factored = false;
+  h->t_upload = ms_since(t0);
+}
+
+template <typename T>
+int64_t download_impl(spkhip_handle* h, T* lnz, T* unz, int64_t* ipvt) {
+  hipStream_t st = h->stream;
+  const i64 nl = h->nlnz, nu = h->nunz;
+  if (h->precision == SPKHIP_FP64) {
+    if (std::is_same<T, double>::value) {
+      if (lnz) hipck(hipMemcpyAsync(lnz, h->d_lnz64.p, nl * sizeof(T), hipMemcpyDeviceToHost, st));
+      if (unz && h->kind == SPKHIP_KIND_LU)
+        hipck(hipMemcpyAsync(unz, h->d_unz64.p, nu * sizeof(T), hipMemcpyDeviceToHost, st));
+    } else {
+      DevBuf<float> tmp;
+      tmp.alloc(std::max(nl, nu));
+      if (lnz) {
+        hipLaunchKernelGGL((spk::convert_kernel<double, float>), dim3(grid_for(nl)), dim3(kWG), 0,
+                           st, h->d_lnz64.p, tmp.p, nl);
+        hipck(hipMemcpyAsync(lnz, tmp.p, nl * sizeof(float), hipMemcpyDeviceToHost, st));
+        sync(h);
+      }
+      if (unz && h->kind == SPKHIP_KIND_LU) {
+        hipLaunchKernelGGL((spk::convert_kernel<double, float>), dim3(grid_for(nu)), dim3(kWG), 0,
+                           st, h->d_unz64.p, tmp.p, nu);
+        hipck(hipMemcpyAsync(unz, tmp.p, nu * sizeof(float), hipMemcpyDeviceToHost, st));
+      }
+      sync(h);
+    }
+  } else {
+    if (std::is_same<T, float>::value) {
+      if (lnz) hipck(hipMemcpyAsync(lnz, h->d_lnz32.p, nl * sizeof(T), hipMemcpyDeviceToHost, st));
+      if (unz && h->kind == SPKHIP_KIND_LU)
+        hipck(hipMemcpyAsync(unz, h->d_unz32.p, nu * sizeof(T), hipMemcpyDeviceToHost, st));
+    } else {
+      DevBuf<double> tmp;
+      tmp.alloc(std::max(nl, nu));
+      if (lnz) {
+        hipLaunchKernelGGL((spk::convert_kernel<float, double>), dim3(grid_for(nl)), dim3(kWG), 0,
+                           st, h->d_lnz32.p, tmp.p, nl);
+        hipck(hipMemcpyAsync(lnz, tmp.p, nl * sizeof(double), hipMemcpyDeviceToHost, st));
+        sync(h);
+      }
+      if (unz && h->kind == SPKHIP_KIND_LU) {
+        hipLaunchKernelGGL((spk::convert_kernel<float, double>), dim3(grid_for(nu)), dim3(kWG), 0,
+                           st, h->d_unz32.p, tmp.p, nu);
+        hipck(hipMemcpyAsync(unz, tmp.p, nu * sizeof(double), hipMemcpyDeviceToHost, st));
+      }
+      sync(h);
+    }
+  }
+  if (ipvt) hipck(hipMemcpyAsync(ipvt, h->d_ipvt.p, h->n * sizeof(i64), hipMemcpyDeviceToHost, st));
+  sync(h);
+  if (h->kind == SPKHIP_KIND_SPD && lnz) {
+    // LDL^T layout: D on the diagonal, unit L below, zeros above
+    // (SpkLDLtFactor.jl, rectangular panels of SpkSparseSpdBase.jl:359-375).
+    for (i64 J = 0; J < h->nsuper; ++J) {
+      i64 fj = h->xsuper[J] - 1, nj = h->xsuper[J + 1] - h->xsuper[J];
+      i64 jl = h->xlindx[J + 1] - h->xlindx[J];
+      T* p = lnz + (h->xlnz[fj] - 1);
+      for (i64 c = 0; c < nj; ++c)
+        for (i64 r = 0; r < c; ++r) p[r + c * jl] = T(0);
+    }
+  }
+  return 0;
+}
+
+int64_t guard(spkhip_t h) { return h ? 0 : -4; }
+
+template <typename F>
+int64_t wrap(F&& f) {
+  try {
+    return f();
+  } catch (const HipFail& e) {
+    return -1000 - (int64_t)e.e;
+  } catch (const ApiError& e) {
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    return -2;
+  } catch (...) {
+    return -4;
+  }
+}
+
+// Stateless helpers: build a handle from the reference's arrays.
+int64_t stateless_setup(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i64* xsuper,
+                        const i64* snode, const i64* xlindx, const i64* lindx, const i64* xlnz,
+                        const i64* xunz) {
+  upload_symbolic(h, kind, n, nsuper, xsuper, snode, xlindx, lindx, xlnz, xunz, nullptr,
+                  nullptr);
+  return 0;
+}
+
+template <typename T>
+int64_t lufactor_stateless(i64 n, i64 nsuper, const i64* xsuper, const i64* snode,
+                           const i64* xlindx, const i64* lindx, const i64* xlnz, T* lnz,
+                           const i64* xunz, T* unz, i64* ipvt, int64_t kind) {
+  spkhip_t h = nullptr;
+  int64_t rc = spkhip_create(&h, 0);
+  if (rc) return rc;
+  rc = wrap([&]() -> int64_t {
+    stateless_setup(h, kind, n, nsuper, xsuper, snode, xlindx, lindx, xlnz, xunz);
+    hipStream_t st = h->stream;
+    DevBuf<T>& dl = Bufs<T>::lnz(h);
+    DevBuf<T>& du = Bufs<T>::unz(h);
+    dl.upload(lnz, (size_t)h->nlnz, st);
+    du.alloc((size_t)std::max<i64>(h->nunz, 1));
+    if (kind == SPKHIP_KIND_LU && h->nunz > 0)
+      hipck(hipMemcpyAsync(du.p, unz, h->nunz * sizeof(T), hipMemcpyHostToDevice, st));
+    const int prec = std::is_same<T, double>::value ? SPKHIP_FP64 : SPKHIP_FP32;
+    if (kind == SPKHIP_KIND_LU)
+      run_factor<T, 64>(h);
+    else
+      run_factor<T, 96>(h);
+    h->precision = prec;
+    h->factored = true;
+    download_impl<T>(h, lnz, kind == SPKHIP_KIND_LU ? unz : nullptr, ipvt);
+    return h->first_fail ? -1 : 0;
+  });
+  spkhip_destroy(h);
+  return rc;
+}
+
+template <typename T>
+int64_t trisolve_stateless(i64 nsuper, const i64* xsuper, const i64* xlindx, const i64* lindx,
+                           const i64* xlnz, const T* lnz, const i64* xunz, const T* unz,
+                           const i64* ipiv, T* rhs, bool fwd, bool bwd, int64_t kind) {
+  spkhip_t h = nullptr;
+  int64_t rc = spkhip_create(&h, 0);
+  if (rc) return rc;
+  rc = wrap([&]() -> int64_t {
+    if (nsuper <= 0) return -1;
+    const i64 n = xsuper[nsuper] - 1;
+    stateless_setup(h, kind, n, nsuper, xsuper, nullptr, xlindx, lindx, xlnz, xunz);
+    hipStream_t st = h->stream;
+    Bufs<T>::lnz(h).upload(lnz, (size_t)h->nlnz, st);
+    DevBuf<T>& du = Bufs<T>::unz(h);
+    du.alloc((size_t)std::max<i64>(h->nunz, 1));
+    if (unz && h->nunz > 0)
+      hipck(hipMemcpyAsync(du.p, unz, h->nunz * sizeof(T), hipMemcpyHostToDevice, st));
+    std::vector<i64> piv(n);
+    for (i64 i = 0; i < n; ++i) piv[i] = ipiv ? ipiv[i] : 1;
+    if (ipiv)  // local pivots must stay inside their diagonal block
+      for (i64 J = 0; J < nsuper; ++J)
+        for (i64 c = xsuper[J]; c < xsuper[J + 1]; ++c)
+          if (piv[c - 1] < 1 || piv[c - 1] > xsuper[J + 1] - xsuper[J]) return -4;
+    if (!ipiv)  // identity pivots: position within the own block
+      for (i64 J = 0; J < nsuper; ++J)
+        for (i64 c = xsuper[J]; c < xsuper[J + 1]; ++c) piv[c - 1] = c - xsuper[J] + 1;
+    h->d_ipvt.upload(piv, st);
+    DevBuf<T>& r = Bufs<T>::rhs(h);
+    r.upload(rhs, (size_t)n, st);
+    solve_levels<T>(h, r.p, n, 1, fwd, bwd);
+    hipck(hipMemcpyAsync(rhs, r.p, n * sizeof(T), hipMemcpyDeviceToHost, st));
+    sync(h);
+    return 0;
+  });
+  spkhip_destroy(h);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t spkhip_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+const char* spkhip_version(void) { return "sparspak_hip 0.1 (gfx950)"; }
+
+int64_t spkhip_create(spkhip_t* out, int64_t device) {
+  if (!out) return -4;
+  *out = nullptr;
+  return wrap([&]() -> int64_t {
+    int cnt = 0;
+    hipck(hipGetDeviceCount(&cnt));
+    if (device < 0 || device >= cnt) return -4;
+    spkhip_handle* h = new spkhip_handle();
+    h->device = (int)device;
+    try {
+      hipck(hipSetDevice(h->device));
+      hipck(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+      hipck(hipEventCreate(&h->ev0));
+      hipck(hipEventCreate(&h->ev1));
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+    return 0;
+  });
+}
+
+int64_t spkhip_destroy(spkhip_t h) {
+  if (!h) return 0;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  int dev = h->device;
+  hipStream_t st = h->stream;
+  hipEvent_t e0 = h->ev0, e1 = h->ev1;
+  delete h;  // frees device buffers
+  (void)dev;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  return 0;
+}
+
+int64_t spkhip_symbolic_upload(spkhip_t h, int64_t kind, int64_t n, int64_t nsuper,
+                               const int64_t* xsuper, const int64_t* snode,
+                               const int64_t* xlindx, const int64_t* lindx, const int64_t* xlnz,
+                               const int64_t* xunz, const int64_t* perm, const int64_t* invp,
+                               const int64_t* colptr, const int64_t* rowval) {
+  if (guard(h)) return -4;
+  if (n <= 0 || nsuper <= 0 || !xsuper || !xlindx || !lindx || !xlnz || !perm || !invp)
+    return -4;
+  if (kind != SPKHIP_KIND_LU && kind != SPKHIP_KIND_SPD) return -4;
+  return wrap([&]() -> int64_t {
+    upload_symbolic(h, kind, n, nsuper, xsuper, snode, xlindx, lindx, xlnz, xunz, perm, invp);
+    if (colptr && rowval) {
+      auto t0 = Clock::now();
+      h->nnz = colptr[n] - 1;
+      std::vector<i64> amap;
+      build_amap(h, colptr, rowval, amap);
+      hipStream_t st = h->stream;
+      h->d_amap.upload(amap, st);
+      // CSR of A (original numbering) for the residual: rowptr, colind and
+      // the CSC -> CSR value permutation
+      std::vector<i64> rowptr(n + 1, 0), c2r(h->nnz);
+      std::vector<i32> colind(h->nnz);
+      for (i64 k = 0; k < h->nnz; ++k) rowptr[rowval[k]]++;
+      for (i64 i = 0; i < n; ++i) rowptr[i + 1] += rowptr[i];
+      std::vector<i64> fillp(rowptr.begin(), rowptr.end() - 1);
+      for (i64 j = 1; j <= n; ++j)
+        for (i64 k = colptr[j - 1] - 1; k < colptr[j] - 1; ++k) {
+          i64 r = rowval[k] - 1;
+          i64 d = fillp[r]++;
+          colind[d] = (i32)(j - 1);
+          c2r[k] = d;
+        }
+      h->d_rowptr.upload(rowptr, st);
+      h->d_colind.upload(colind, st);
+      h->d_csc2csr.upload(c2r, st);
+      sync(h);
+      h->have_csc = true;
+      h->t_upload += ms_since(t0);
+    }
+    return h->pattern_error ? -1 : 0;
+  });
+}
+
+}  // extern "C"
+
+namespace {
+__global__ void csr_values_kernel(const double* __restrict__ nz, const int64_t* __restrict__ c2r,
+                                  double* __restrict__ csr, int64_t nnz) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nnz) csr[c2r[k]] = nz[k];
+}
+
+int64_t inmatrix_common(spkhip_t h) {
+  hipStream_t st = h->stream;
+  h->d_csrval.ensure((size_t)std::max<i64>(h->nnz, 1));
+  h->d_rowabs.ensure((size_t)h->n);
+  h->d_norms.ensure(1);
+  if (h->nnz > 0)
+    hipLaunchKernelGGL(csr_values_kernel, dim3(grid_for(h->nnz)), dim3(kWG), 0, st,
+                       h->d_nzval.p, h->d_csc2csr.p, h->d_csrval.p, h->nnz);
+  hipLaunchKernelGGL(spk::rowabs_kernel, dim3(grid_for(h->n)), dim3(kWG), 0, st, h->d_rowptr.p,
+                     h->d_csrval.p, h->d_rowabs.p, h->n);
+  hipLaunchKernelGGL(spk::colmaxabs_kernel, dim3(1), dim3(kWG), 0, st, h->d_rowabs.p, h->n, h->n,
+                     h->d_norms.p);
+  hipck(hipGetLastError());
+  hipck(hipMemcpyAsync(&h->anorm, h->d_norms.p, sizeof(double), hipMemcpyDeviceToHost, st));
+  sync(h);
+  h->have_values = true;
+  h->factored = false;
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int64_t spkhip_inmatrix(spkhip_t h, const double* nzval) {
+  if (guard(h)) return -4;
+  if (!h->have_sym || !h->have_csc) return -3;
+  if (h->pattern_error) return -1;
+  return wrap([&]() -> int64_t {
+    auto t0 = Clock::now();
+    h->d_nzval.upload(nzval, (size_t)h->nnz, h->stream);
+    int64_t rc = inmatrix_common(h);
+    h->t_inmatrix = ms_since(t0);
+    return rc;
+  });
+}
+
+int64_t spkhip_inmatrix_device(spkhip_t h, const double* d_nzval) {
+  if (guard(h)) return -4;
+  if (!h->have_sym || !h->have_csc) return -3;
+  if (h->pattern_error) return -1;
+  return wrap([&]() -> int64_t {
+    auto t0 = Clock::now();
+    h->d_nzval.ensure((size_t)std::max<i64>(h->nnz, 1));
+    if (h->nnz)
+      hipck(hipMemcpyAsync(h->d_nzval.p, d_nzval, h->nnz * sizeof(double),
+                           hipMemcpyDeviceToDevice, h->stream));
+    int64_t rc = inmatrix_common(h);
+    h->t_inmatrix = ms_since(t0);
+    return rc;
+  });
+}
+
+int64_t spkhip_factor(spkhip_t h, int64_t precision) {
+  if (guard(h)) return -4;
+  if (!h->have_values) return -3;
+  if (precision != SPKHIP_FP64 && precision != SPKHIP_FP32) return -4;
+  return wrap([&]() -> int64_t {
+    hipck(hipSetDevice(h->device));
+    hipck(hipEventRecord(h->ev0, h->stream));
+    const bool lu = h->kind == SPKHIP_KIND_LU;
+    if (precision == SPKHIP_FP64) {
+      run_assemble<double>(h);
+      if (lu) run_factor<double, 64>(h); else run_factor<double, 96>(h);
+    } else {
+      run_assemble<float>(h);
+      if (lu) run_factor<float, 64>(h); else run_factor<float, 96>(h);
+    }
+    hipck(hipEventRecord(h->ev1, h->stream));
+    hipck(hipEventSynchronize(h->ev1));
+    float ms = 0;
+    hipck(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->t_factor = ms;
+    h->precision = (int)precision;
+    h->factored = true;
+    return h->first_fail ? -1 : 0;
+  });
+}
+
+int64_t spkhip_solve_device(spkhip_t h, int64_t nrhs, double* d_b, int64_t ldb) {
+  if (guard(h)) return -4;
+  if (!h->factored) return -3;
+  if (nrhs < 0 || ldb < h->n) return -4;
+  if (nrhs == 0) return 0;
+  return wrap([&]() -> int64_t {
+    hipck(hipEventRecord(h->ev0, h->stream));
+    if (h->precision == SPKHIP_FP64)
+      solve_device<double>(h, d_b, ldb, (int)nrhs);
+    else
+      solve_device<float>(h, d_b, ldb, (int)nrhs);
+    hipck(hipEventRecord(h->ev1, h->stream));
+    hipck(hipEventSynchronize(h->ev1));
+    float ms = 0;
+    hipck(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->t_solve = ms;
+    return 0;
+  });
+}
+
+int64_t spkhip_solve(spkhip_t h, int64_t nrhs, double* b, int64_t ldb) {
+  if (guard(h)) return -4;
+  if (!h->factored) return -3;
+  if (nrhs < 0 || ldb < h->n) return -4;
+  if (nrhs == 0) return 0;
+  return wrap([&]() -> int64_t {
+    const i64 n = h->n;
+    h->d_b.ensure((size_t)n * nrhs);
+    for (i64 q = 0; q < nrhs; ++q)
+      hipck(hipMemcpyAsync(h->d_b.p + q * n, b + q * ldb, n * sizeof(double),
+                           hipMemcpyHostToDevice, h->stream));
+    int64_t rc = spkhip_solve_device(h, nrhs, h->d_b.p, n);
+    if (rc) return rc;
+    for (i64 q = 0; q < nrhs; ++q)
+      hipck(hipMemcpyAsync(b + q * ldb, h->d_b.p + q * n, n * sizeof(double),
+                           hipMemcpyDeviceToHost, h->stream));
+    sync(h);
+    return 0;
+  });
+}
+
+}  // extern "C"
+
+namespace {
+// Iterative refinement on device data: X (n x nrhs, ld n) receives the
+// solution of A X = Bm (both double, original numbering).
+template <typename T>
+int64_t refine_impl(spkhip_t h, const double* d_bm, double* d_xo, int nrhs, double tol,
+                    int64_t max_iter) {
+  hipStream_t st = h->stream;
+  const i64 n = h->n;
+  h->d_r.ensure((size_t)n * nrhs);
+  h->d_norms.ensure((size_t)3 * nrhs + 1);
+  DevBuf<T>& rhs = Bufs<T>::rhs(h);
+  rhs.ensure((size_t)n * nrhs);
+  DevBuf<T>* tmp;
+  DevBuf<float> tmpf;
+  DevBuf<double>& tmpd = h->d_tmp64;
+  if (std::is_same<T, float>::value) {
+    h->d_tmp32.ensure((size_t)n * nrhs);
+  } else {
+    tmpd.ensure((size_t)n * nrhs);
+  }
+  (void)tmp;
+  (void)tmpf;
+  T* tbuf = std::is_same<T, float>::value ? (T*)h->d_tmp32.p : (T*)tmpd.p;
+  // x0 = solve(b)
+  hipLaunchKernelGGL(spk::fill_kernel<double>, dim3(grid_for(n * nrhs)), dim3(kWG), 0, st, d_xo,
+                     n * nrhs, 0.0);
+  std::vector<double> nrm(3 * nrhs);
+  hipLaunchKernelGGL(spk::colmaxabs_kernel, dim3(nrhs), dim3(kWG), 0, st, d_bm, n, n,
+                     h->d_norms.p + 2 * nrhs);
+  const double* src = d_bm;
+  int iters = 0;
+  double eta_max = 0;
+  for (;;) {
+    // correction d = A^-1 src in precision T; x += d
+    hipLaunchKernelGGL((spk::permute_kernel<double, T>), dim3(grid_for(n)), dim3(kWG), 0, st, src,
+                       n, rhs.p, n, h->d_perm.p, n, nrhs);
+    solve_levels<T>(h, rhs.p, n, nrhs, true, true);
+    hipLaunchKernelGGL((spk::permute_kernel<T, T>), dim3(grid_for(n)), dim3(kWG), 0, st, rhs.p, n,
+                       tbuf, n, h->d_invp.p, n, nrhs);
+    hipLaunchKernelGGL(spk::axpy_kernel<T>, dim3(grid_for(n)), dim3(kWG), 0, st, d_xo, n, tbuf, n,
+                       n, nrhs);
+    // r = b - A x (double), eta per column
+    hipLaunchKernelGGL(spk::residual_kernel, dim3(grid_for(n)), dim3(kWG), 0, st, h->d_rowptr.p,
+                       h->d_colind.p, h->d_csrval.p, d_xo, n, d_bm, n, h->d_r.p, n, n, nrhs);
+    hipLaunchKernelGGL(spk::colmaxabs_kernel, dim3(nrhs), dim3(kWG), 0, st, h->d_r.p, n, n,
+                       h->d_norms.p);
+    hipLaunchKernelGGL(spk::colmaxabs_kernel, dim3(nrhs), dim3(kWG), 0, st, d_xo, n, n,
+                       h->d_norms.p + nrhs);
+    hipck(hipGetLastError());
+    hipck(hipMemcpyAsync(nrm.data(), h->d_norms.p, 3 * nrhs * sizeof(double),
+                         hipMemcpyDeviceToHost, st));
+    sync(h);
+    eta_max = 0;
+    for (int q = 0; q < nrhs; ++q) {
+      double den = h->anorm * nrm[nrhs + q] + nrm[2 * nrhs + q];
+      double eta = den > 0 ? nrm[q] / den : 0.0;
+      eta_max = std::max(eta_max, eta);
+    }
+    if (eta_max <= tol || iters >= max_iter || !std::isfinite(eta_max)) break;
+    src = h->d_r.p;
+    ++iters;
+  }
+  h->refine_iters = iters;
+  h->backward_error = eta_max;
+  return (eta_max <= tol) ? 0 : -5;
+}
+}  // namespace
+
+extern "C" {
+
+int64_t spkhip_solve_refine_device(spkhip_t h, int64_t nrhs, double* d_b, int64_t ldb, double tol,
+                                   int64_t max_iter) {
+  if (guard(h)) return -4;
+  if (!h->factored) return -3;
+  if (nrhs < 0 || ldb < h->n || max_iter < 0) return -4;
+  if (nrhs == 0) return 0;
+  return wrap([&]() -> int64_t {
+    const i64 n = h->n;
+    hipck(hipEventRecord(h->ev0, h->stream));
+    DevBuf<double> bm;
+    bm.alloc((size_t)n * nrhs);
+    h->d_x.ensure((size_t)n * nrhs);
+    for (i64 q = 0; q < nrhs; ++q)
+      hipck(hipMemcpyAsync(bm.p + q * n, d_b + q * ldb, n * sizeof(double),
+                           hipMemcpyDeviceToDevice, h->stream));
+    int64_t rc = (h->precision == SPKHIP_FP64)
+                     ? refine_impl<double>(h, bm.p, h->d_x.p, (int)nrhs, tol, max_iter)
+                     : refine_impl<float>(h, bm.p, h->d_x.p, (int)nrhs, tol, max_iter);
+    for (i64 q = 0; q < nrhs; ++q)
+      hipck(hipMemcpyAsync(d_b + q * ldb, h->d_x.p + q * n, n * sizeof(double),
+                           hipMemcpyDeviceToDevice, h->stream));
+    hipck(hipEventRecord(h->ev1, h->stream));
+    hipck(hipEventSynchronize(h->ev1));
+    float ms = 0;
+    hipck(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->t_refine = ms;
+    return rc;
+  });
+}
+
+int64_t spkhip_solve_refine(spkhip_t h, int64_t nrhs, double* b, int64_t ldb, double tol,
+                            int64_t max_iter) {
+  if (guard(h)) return -4;
+  if (!h->factored) return -3;
+  if (nrhs < 0 || ldb < h->n || max_iter < 0) return -4;
+  if (nrhs == 0) return 0;
+  return wrap([&]() -> int64_t {
+    const i64 n = h->n;
+    h->d_b.ensure((size_t)n * nrhs);
+    for (i64 q = 0; q < nrhs; ++q)
+      hipck(hipMemcpyAsync(h->d_b.p + q * n, b + q * ldb, n * sizeof(double),
+                           hipMemcpyHostToDevice, h->stream));
+    int64_t rc = spkhip_solve_refine_device(h, nrhs, h->d_b.p, n, tol, max_iter);
+    if (rc != 0 && rc != -5) return rc;
+    for (i64 q = 0; q < nrhs; ++q)
+      hipck(hipMemcpyAsync(b + q * ldb, h->d_b.p + q * n, n * sizeof(double),
+                           hipMemcpyDeviceToHost, h->stream));
+    sync(h);
+    return rc;
+  });
+}
+
+int64_t spkhip_download(spkhip_t h, double* lnz, double* unz, int64_t* ipvt) {
+  if (guard(h)) return -4;
+  if (!h->factored) return -3;
+  return wrap([&]() -> int64_t { return download_impl<double>(h, lnz, unz, ipvt); });
+}
+
+int64_t spkhip_download_f32(spkhip_t h, float* lnz, float* unz, int64_t* ipvt) {
+  if (guard(h)) return -4;
+  if (!h->factored) return -3;
+  return wrap([&]() -> int64_t { return download_impl<float>(h, lnz, unz, ipvt); });
+}
+
+int64_t spkhip_stats(spkhip_t h, spkhip_stats_t* s) {
+  if (guard(h) || !s) return -4;
+  std::memset(s, 0, sizeof(*s));
+  s->n = h->n;
+  s->nsuper = h->nsuper;
+  s->nfront = h->nfront;
+  s->nlevel = h->nlevel;
+  s->nnz_lnz = h->nlnz;
+  s->nnz_unz = h->nunz;
+  s->first_fail_supernode = h->first_fail;
+  s->refine_iters = h->refine_iters;
+  s->factor_flops = h->factor_flops;
+  s->solve_flops = h->solve_flops;
+  s->front_bytes = (double)h->arena_elems *
+                   (h->precision == SPKHIP_FP32 ? sizeof(float) : sizeof(double));
+  s->t_upload_ms = h->t_upload;
+  s->t_inmatrix_ms = h->t_inmatrix;
+  s->t_factor_ms = h->t_factor;
+  s->t_solve_ms = h->t_solve;
+  s->t_refine_ms = h->t_refine;
+  s->backward_error = h->backward_error;
+  return 0;
+}
+
+int64_t spkhip_lufactor_f64(int64_t n, int64_t nsuper, const int64_t* xsuper,
+                            const int64_t* snode, const int64_t* xlindx, const int64_t* lindx,
+                            const int64_t* xlnz, double* lnz, const int64_t* xunz, double* unz,
+                            int64_t* ipvt) {
+  return lufactor_stateless<double>(n, nsuper, xsuper, snode, xlindx, lindx, xlnz, lnz, xunz, unz,
+                                    ipvt, SPKHIP_KIND_LU);
+}
+int64_t spkhip_lufactor_f32(int64_t n, int64_t nsuper, const int64_t* xsuper,
+                            const int64_t* snode, const int64_t* xlindx, const int64_t* lindx,
+                            const int64_t* xlnz, float* lnz, const int64_t* xunz, float* unz,
+                            int64_t* ipvt) {
+  return lufactor_stateless<float>(n, nsuper, xsuper, snode, xlindx, lindx, xlnz, lnz, xunz, unz,
+                                   ipvt, SPKHIP_KIND_LU);
+}
+int64_t spkhip_lulsolve_f64(int64_t nsuper, const int64_t* xsuper, const int64_t* xlindx,
+                            const int64_t* lindx, const int64_t* xlnz, const double* lnz,
+                            const int64_t* ipiv, double* rhs) {
+  return trisolve_stateless<double>(nsuper, xsuper, xlindx, lindx, xlnz, lnz, nullptr, nullptr,
+                                    ipiv, rhs, true, false, SPKHIP_KIND_LU);
+}
+int64_t spkhip_lulsolve_f32(int64_t nsuper, const int64_t* xsuper, const int64_t* xlindx,
+                            const int64_t* lindx, const int64_t* xlnz, const float* lnz,
+                            const int64_t* ipiv, float* rhs) {
+  return trisolve_stateless<float>(nsuper, xsuper, xlindx, lindx, xlnz, lnz, nullptr, nullptr,
+                                   ipiv, rhs, true, false, SPKHIP_KIND_LU);
+}
+int64_t spkhip_luusolve_f64(int64_t n, int64_t nsuper, const int64_t* xsuper,
+                            const int64_t* xlindx, const int64_t* lindx, const int64_t* xlnz,
+                            const double* lnz, const int64_t* xunz, const double* unz,
+                            double* rhs) {
+  (void)n;
+  return trisolve_stateless<double>(nsuper, xsuper, xlindx, lindx, xlnz, lnz, xunz, unz, nullptr,
+                                    rhs, false, true, SPKHIP_KIND_LU);
+}
+int64_t spkhip_luusolve_f32(int64_t n, int64_t nsuper, const int64_t* xsuper,
+                            const int64_t* xlindx, const int64_t* lindx, const int64_t* xlnz,
+                            const float* lnz, const int64_t* xunz, const float* unz, float* rhs) {
+  (void)n;
+  return trisolve_stateless<float>(nsuper, xsuper, xlindx, lindx, xlnz, lnz, xunz, unz, nullptr,
+                                   rhs, false, true, SPKHIP_KIND_LU);
+}
+int64_t spkhip_ldltsolve_f64(int64_t nsuper, const int64_t* xsuper, const int64_t* xlindx,
+                             const int64_t* lindx, const int64_t* xlnz, const double* lnz,
+                             double* rhs) {
+  if (nsuper <= 0 || !xsuper || !xlindx || !lindx || !xlnz || !lnz || !rhs) return -4;
+  // LDL^T panels (D on the diagonal, unit L below) -> the LU form the solve
+  // kernels read: U11 = D L11^T in the upper block, U12^T = L21 D in unz.
+  const i64 n = xsuper[nsuper] - 1;
+  std::vector<i64> xunz(n + 1);
+  i64 up = 1;
+  for (i64 J = 0; J < nsuper; ++J) {
+    i64 w = xsuper[J + 1] - xsuper[J], jl = xlindx[J + 1] - xlindx[J];
+    for (i64 c = xsuper[J]; c < xsuper[J + 1]; ++c) {
+      xunz[c - 1] = up;
+      up += jl - w;
+    }
+  }
+  xunz[n] = up;
+  std::vector<double> l2(lnz, lnz + (xlnz[n] - 1)), u2(std::max<i64>(up - 1, 1));
+  for (i64 J = 0; J < nsuper; ++J) {
+    i64 fj = xsuper[J] - 1, w = xsuper[J + 1] - xsuper[J], jl = xlindx[J + 1] - xlindx[J];
+    double* p = l2.data() + (xlnz[fj] - 1);
+    double* u = u2.data() + (xunz[fj] - 1);
+    for (i64 c = 0; c < w; ++c) {
+      const double d = p[c + c * jl];
+      for (i64 k = c + 1; k < w; ++k) p[c + k * jl] = d * p[k + c * jl];
+      for (i64 r = w; r < jl; ++r) u[(r - w) + c * (jl - w)] = d * p[r + c * jl];
+    }
+  }
+  std::vector<i64> piv(n);
+  for (i64 J = 0; J < nsuper; ++J)
+    for (i64 c = xsuper[J]; c < xsuper[J + 1]; ++c) piv[c - 1] = c - xsuper[J] + 1;
+  return trisolve_stateless<double>(nsuper, xsuper, xlindx, lindx, xlnz, l2.data(), xunz.data(),
+                                    u2.data(), piv.data(), rhs, true, true, SPKHIP_KIND_LU);
+}
+
+int64_t spkhip_ldltfactor_f64(int64_t n, int64_t nsuper, const int64_t* xsuper,
+                              const int64_t* snode, const int64_t* xlindx, const int64_t* lindx,
+                              const int64_t* xlnz, double* lnz) {
+  return lufactor_stateless<double>(n, nsuper, xsuper, snode, xlindx, lindx, xlnz, lnz, nullptr,
+                                    nullptr, nullptr, SPKHIP_KIND_SPD);
+}
+
+}  // extern "C"

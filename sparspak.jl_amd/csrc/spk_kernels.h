@@ -1,0 +1,73 @@
+// Device kernels of the numeric phase (gfx950, wave64).
+//
+// Storage model. The reference keeps the factor as per-supernode panels
+// (SpkSparseBase.jl:253-284): lnz holds a column-major jlen x nj panel per
+// supernode (rows = the supernode's index list, the top nj x nj block is the
+// LAPACK-packed L11\U11), unz holds U12^T as a (jlen-nj) x nj panel. The
+// reference splits every fundamental supernode into pieces of <= 30 (LU) or
+// <= 60 (SPD) columns; pivoting is confined to each piece's diagonal block.
+//
+// On the device the unit of work is a FRONT: one fundamental supernode,
+// i.e. a maximal chain of pieces whose index lists nest (piece k+1's list is
+// piece k's list without its own columns). A front with index list I (length
+// L) and W own columns is factored as a dense L x L column-major matrix: its
+// first W columns/rows are the pieces' panels, the trailing R x R block
+// (R = L - W) receives the Schur complement (update matrix) that the
+// front's parent adds into itself through a relative-index map. Fronts are
+// processed one elimination-tree level at a time, all fronts of a level in
+// the same launches. Results are written back into the reference's lnz /
+// unz / ipvt layout, which the solves read.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spk {
+
+constexpr int kWG = 256;      // threads per workgroup (4 waves)
+constexpr int kTile = 64;     // update-kernel tile edge
+constexpr int kMaxPiece = 64; // widest piece of a small (LDS) front; the
+                              // large-front kernels take the width bound as
+                              // a template argument: 64 for LU, 96 for SPD
+                              // (the reference's split leaves a last piece
+                              // of up to 1.5x its 30 / 60 limit)
+constexpr int kUcols = 32;    // U12 columns per panel workgroup
+
+// Device view of the symbolic factor and the front schedule.
+struct DevSym {
+  int64_t n, nsuper, nfront;
+  const int64_t* xsuper;  // nsuper+1, 1-based values
+  const int64_t* snode;   // n, 1-based values
+  const int64_t* xlnz;    // n+1, 1-based values
+  const int64_t* xunz;    // n+1, 1-based values
+  const int64_t* xlindx;  // nsuper+1, 1-based values
+  const int32_t* lindx;   // nsub, 1-based values
+  // per front
+  const int32_t* fL;      // index-list length
+  const int32_t* fW;      // own columns
+  const int32_t* fc0;     // first column (0-based)
+  const int32_t* fsp0;    // first piece (0-based supernode)
+  const int32_t* fnp;     // number of pieces
+  const int64_t* foff;    // arena offset of the L x L work matrix (large
+                          // fronts) — unused for small fronts
+  const int64_t* fcb;     // arena offset of the update matrix
+  const int32_t* fcbld;   // its leading dimension
+  const int64_t* frel;    // offset of the front's relative-index map
+  const int32_t* relmap;  // position in the parent's index list
+  const int32_t* fchp;    // children CSR pointer (nfront+1)
+  const int32_t* fch;     // children
+  const int64_t* fvec;    // offset in the solve vector arena (L per rhs)
+  int pivot;              // 1: pivot inside diagonal blocks (LU); 0: none
+  int sym;                // 1: U entries are read from the transposed lnz slot
+                          //    (SPD input: lower triangle only)
+};
+
+// Per-step descriptor of the large-front kernels.
+struct StepDesc {
+  int32_t front;
+  int32_t piece;     // 0-based supernode
+  int32_t wg_begin;  // prefix sum of workgroups (panel) / tiles (update)
+  int32_t nl;        // panel: L21 row blocks; update: tiles per dimension
+  int32_t slot;      // diagonal-block staging slot
+};
+
+}  // namespace spk

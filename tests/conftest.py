@@ -1,0 +1,20 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (ROOT, os.path.join(ROOT, "sparspak.jl_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libsparspak_hip.so)")
+
+
+@pytest.fixture(scope="session")
+def fixtures():
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "fixtures.json")) as f:
+        return json.load(f)

@@ -1,0 +1,256 @@
+"""Parity of the GPU numeric phase (through libsparspak_hip.so) with the CPU
+oracle, on the reference's fixtures and on synthetic matrices. Tolerances:
+factor entries within 1e-12 (fp64) / 2e-5 (fp32) of the oracle relative to
+the largest entry, identical pivots, normwise backward error of solves
+<= 1e-15 (fp64) and <= 1e-14 after refinement (fp32)."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle.oracle as orc
+from helpers import backward_error, csc1, random_pattern
+from sparspak_hip import (Problem, SparseSolver, SparseSpdSolver, factor, findorder,
+                          inmatrix, solve, sparspaklu, sparspaklu_refactor, symbolicfactor,
+                          triangularsolve, backslash)
+from sparspak_hip import _lib
+from sparspak_hip.gallery import convdiff3d, laplace2d, uniform
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _prepared(A, cls=SparseSolver, precision="fp64"):
+    s = cls(A)
+    findorder(s)
+    symbolicfactor(s)
+    inmatrix(s)
+    factor(s, precision)
+    return s
+
+
+def _oracle_factor(s, A, dtype=np.float64):
+    n, cp, rv, nz = csc1(A)
+    return orc.factor_csc(s.slvr.sym, cp, rv, nz, dtype)
+
+
+def _relmax(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+CASES = {
+    "rand_sym_weakdiag": lambda: random_pattern(60, 0.1, 1, True, False) + sp.diags(np.full(60, 0.3)),
+    "rand_unsym_weakdiag": lambda: random_pattern(60, 0.1, 2, False, False) + sp.diags(np.full(60, 0.3)),
+    "laplace2d_20": lambda: laplace2d(20),
+    "convdiff3d_10": lambda: convdiff3d(10),
+    "dense_70": lambda: sp.csc_matrix(np.random.default_rng(5).uniform(-1, 1, (70, 70)) + 0.5 * np.eye(70)),
+    "dense_200": lambda: sp.csc_matrix(np.random.default_rng(6).uniform(-1, 1, (200, 200)) + 0.5 * np.eye(200)),
+    "convdiff3d_16": lambda: convdiff3d(16),
+}
+
+
+def test_gpu_tridiag11_golden(fixtures):
+    d = fixtures["tridiag11"]
+    _say("case tridiag11 golden")
+    p = Problem(11, 11)
+    for i in range(1, 11):
+        p.inaij(i + 1, i, -1.0)
+        p.inaij(i, i, 4.0)
+        p.inaij(i, i + 1, -1.0)
+        p.inbi(i, 2.0 * i)
+    p.inaij(11, 11, 4.0)
+    p.inbi(11, 34.0)
+    s = _prepared(p)
+    lnz, unz, ipiv = s.slvr.download()
+    gl = np.array(d["lnz_factor"])
+    assert np.linalg.norm(lnz - gl) / np.linalg.norm(gl) < d["factor_rtol"]
+    assert np.allclose(lnz, gl, rtol=1e-15, atol=0)
+    assert np.array_equal(unz, np.array(d["unz_factor"]))
+    triangularsolve(s)
+    A = p.outsparse()
+    assert backward_error(A, p.x[:11], p.rhs[:11]) < 1e-15
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_gpu_factor_matches_oracle(name):
+    _say(f"case factor fp64 {name}")
+    A = sp.csc_matrix(CASES[name]())
+    s = _prepared(A)
+    lnz, unz, ipiv = s.slvr.download()
+    ol, ou, op, rc, _ = _oracle_factor(s, A)
+    assert rc == 0
+    assert np.array_equal(ipiv, op)
+    assert _relmax(lnz, ol) < 1e-12
+    if ou.size:
+        assert _relmax(unz, ou) < 1e-12
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_gpu_solve_backward_error(name):
+    _say(f"case solve fp64 {name}")
+    A = sp.csc_matrix(CASES[name]())
+    s = _prepared(A)
+    n = A.shape[0]
+    B = np.asfortranarray(np.stack([uniform(10 + k, n, -1, 1) for k in range(4)], axis=1))
+    X = B.copy(order="F")
+    triangularsolve(s, X)
+    assert np.all(backward_error(A, X, B) < 1e-15)
+    # one rhs through the oracle's solve
+    ol, ou, op, _, _ = _oracle_factor(s, A)
+    xo = orc.solve(s.slvr.sym, ol, ou, op, B[:, 0].copy())
+    assert np.max(np.abs(X[:, 0] - xo)) <= 1e-10 * np.max(np.abs(xo))
+
+
+@pytest.mark.parametrize("name", ["convdiff3d_16", "dense_200", "laplace2d_20"])
+def test_gpu_fp32_factor_and_refine(name):
+    _say(f"case fp32 {name}")
+    A = sp.csc_matrix(CASES[name]())
+    s = _prepared(A, precision="fp32")
+    lnz, unz, ipiv = s.slvr.download()
+    ol, ou, op, _, _ = _oracle_factor(s, A, np.float32)
+    assert np.array_equal(ipiv, op)
+    assert _relmax(lnz, ol.astype(np.float64)) < 2e-5
+    n = A.shape[0]
+    b = uniform(3, n, -1, 1)
+    x = b.copy()
+    triangularsolve(s, x, refine=True, tol=1e-14)
+    st = s.slvr.stats()
+    _say(f"  refine iters {st.refine_iters} eta {st.backward_error:.3e}")
+    assert backward_error(A, x, b) <= 1e-14
+    assert st.refine_iters >= 1
+
+
+def test_gpu_fixture_solves(fixtures):
+    _say("case fixtures m31 + small4")
+    d = fixtures["m31"]
+    p = Problem(31, 31)
+    p.insparse(d["I"], d["J"], d["V"])
+    p.infullrhs(np.arange(1, 32, dtype=float))
+    s = SparseSolver(p)
+    solve(s)
+    A = p.outsparse().toarray()
+    xd = np.linalg.solve(A, p.rhs)
+    assert np.linalg.norm(p.x - xd) / np.linalg.norm(xd) < 1e-14
+    for case in fixtures["small4"]:
+        Ad = np.array(case["A"])
+        q = Problem(4, 4)
+        for i in range(4):
+            for j in range(4):
+                if Ad[i, j] != 0.0:
+                    q.inaij(i + 1, j + 1, Ad[i, j])
+        q.inbi(1, 1.0)
+        s = SparseSolver(q)
+        solve(s)
+        xd = np.linalg.solve(Ad, q.rhs)
+        assert np.linalg.norm(q.x - xd) / np.linalg.norm(xd) < 1e-14, case["source"]
+
+
+@pytest.mark.parametrize("A", [laplace2d(25), convdiff3d(8, velocity=(0, 0, 0))],
+                         ids=["laplace2d_25", "laplace3d_8"])
+def test_gpu_spd_solver(A):
+    _say("case spd")
+    s = _prepared(A, cls=SparseSpdSolver)
+    lnz, _, _ = s.slvr.download()
+    n, cp, rv, nz = csc1(A)
+    ol, _, _ = orc.inmatrix(s.slvr.sym, cp, rv, nz)
+    orc.ldltfactor(s.slvr.sym, ol)
+    assert _relmax(lnz, ol) < 1e-12
+    b = uniform(7, n, -1, 1)
+    x = b.copy()
+    triangularsolve(s, x)
+    assert backward_error(A, x, b) < 1e-15
+
+
+def test_gpu_stateless_dropins():
+    _say("case stateless drop-ins")
+    A = convdiff3d(9)
+    s = SparseSolver(A)
+    findorder(s)
+    symbolicfactor(s)
+    sym = s.slvr.sym
+    n, cp, rv, nz = csc1(A)
+    L = _lib.lib()
+    P = _lib.p64
+    lnz, unz, ipiv = orc.inmatrix(sym, cp, rv, nz)
+    g_l, g_u, g_p = lnz.copy(), unz.copy(), ipiv.copy()
+    rc = L.spkhip_lufactor_f64(n, sym.nsuper, P(sym.xsuper), P(sym.snode), P(sym.xlindx),
+                               P(sym.lindx), P(sym.xlnz), _lib.pd(g_l), P(sym.xunz),
+                               _lib.pd(g_u), P(g_p))
+    assert rc == 0
+    orc.lufactor(sym, lnz, unz, ipiv)
+    assert np.array_equal(g_p, ipiv)
+    assert _relmax(g_l, lnz) < 1e-12 and _relmax(g_u, unz) < 1e-12
+    b = uniform(9, n, -1, 1)
+    r1 = b[sym.order.rperm - 1].copy()
+    r2 = r1.copy()
+    L.spkhip_lulsolve_f64(sym.nsuper, P(sym.xsuper), P(sym.xlindx), P(sym.lindx), P(sym.xlnz),
+                          _lib.pd(g_l), P(g_p), _lib.pd(r1))
+    orc.lulsolve(sym, lnz, ipiv, r2)
+    assert np.max(np.abs(r1 - r2)) <= 1e-12 * np.max(np.abs(r2))
+    L.spkhip_luusolve_f64(n, sym.nsuper, P(sym.xsuper), P(sym.xlindx), P(sym.lindx),
+                          P(sym.xlnz), _lib.pd(g_l), P(sym.xunz), _lib.pd(g_u), _lib.pd(r1))
+    orc.luusolve(sym, lnz, unz, r2)
+    assert np.max(np.abs(r1 - r2)) <= 1e-12 * np.max(np.abs(r2))
+
+
+def test_gpu_stateless_ldlt():
+    _say("case stateless ldlt")
+    A = laplace2d(15)
+    s = SparseSpdSolver(A)
+    findorder(s)
+    symbolicfactor(s)
+    sym = s.slvr.sym
+    n, cp, rv, nz = csc1(A)
+    lnz, _, _ = orc.inmatrix(sym, cp, rv, nz)
+    g = lnz.copy()
+    L = _lib.lib()
+    P = _lib.p64
+    rc = L.spkhip_ldltfactor_f64(n, sym.nsuper, P(sym.xsuper), P(sym.snode), P(sym.xlindx),
+                                 P(sym.lindx), P(sym.xlnz), _lib.pd(g))
+    assert rc == 0
+    orc.ldltfactor(sym, lnz)
+    assert _relmax(g, lnz) < 1e-12
+    b = uniform(11, n, -1, 1)
+    r1 = b[sym.order.rperm - 1].copy()
+    r2 = r1.copy()
+    L.spkhip_ldltsolve_f64(sym.nsuper, P(sym.xsuper), P(sym.xlindx), P(sym.lindx), P(sym.xlnz),
+                           _lib.pd(g), _lib.pd(r1))
+    orc.ldltsolve_permuted(sym, lnz, r2)
+    assert np.max(np.abs(r1 - r2)) <= 1e-12 * np.max(np.abs(r2))
+
+
+def test_gpu_zero_pivot_raises():
+    _say("case zero pivot")
+    A = sp.csc_matrix(np.array([[1.0, 1.0, 0.0], [1.0, 1.0, 0.0], [0.0, 0.0, 2.0]]))
+    s = SparseSolver(A)
+    findorder(s)
+    symbolicfactor(s)
+    inmatrix(s)
+    with pytest.raises(RuntimeError, match="zero pivot"):
+        factor(s)
+
+
+def test_gpu_refactor_same_pattern():
+    _say("case sparspaklu!")
+    A = convdiff3d(8)
+    lu = sparspaklu(A)
+    n = A.shape[0]
+    b = uniform(13, n, -1, 1)
+    x = backslash(lu, b)
+    assert backward_error(A, x, b) < 1e-15
+    A2 = A.copy()
+    A2.data = A2.data * uniform(14, A2.nnz, 0.5, 1.5)
+    A2 = A2 + sp.diags(np.full(n, 3.0))
+    A2 = sp.csc_matrix(A2)
+    A2.sort_indices()
+    sparspaklu_refactor(lu, A2, allow_pattern_change=False)
+    x2 = backslash(lu, b)
+    assert backward_error(A2, x2, b) < 1e-15
+    with pytest.raises(RuntimeError, match="pattern"):
+        sparspaklu_refactor(lu, laplace2d(5), allow_pattern_change=False)
