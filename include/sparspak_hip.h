@@ -150,6 +150,12 @@ int64_t spkhip_solve_refine(spkhip_t h, int64_t nrhs, double* b, int64_t ldb,
                             double tol, int64_t max_iter);
 int64_t spkhip_solve_refine_device(spkhip_t h, int64_t nrhs, double* d_b,
                                    int64_t ldb, double tol, int64_t max_iter);
+/* Independent systems stacked in one factor (a block-diagonal batch): the
+ * refinement's backward error is then taken per system, over the rows
+ * [offsets[i], offsets[i+1]) (0-based, original numbering, offsets[0] = 0,
+ * offsets[nseg] = n). Default: one segment. No counterpart in the reference
+ * (it factors one system at a time). */
+int64_t spkhip_set_segments(spkhip_t h, int64_t nseg, const int64_t* offsets);
 
 /* Copies the factor back in the reference layout: lnz (|lnz|), unz (|unz|),
  * ipvt (n, local 1..nj pivots). Double variant converts a float factor. */
@@ -200,6 +206,31 @@ int64_t spkhip_ldltsolve_f64(int64_t nsuper, const int64_t* xsuper,
                              const int64_t* xlindx, const int64_t* lindx,
                              const int64_t* xlnz, const double* lnz,
                              double* rhs);
+
+/* Per-kernel-class profile: with profiling on, every launch of the numeric
+ * phase is bracketed by HIP events on the handle's stream and its
+ * algorithmic work (flops and bytes the algorithm must move) is added to its
+ * class. No counterpart in the reference (it has no device); used by the
+ * benchmark's roofline report. */
+#define SPKHIP_K_ASSEMBLE 0   /* zero lnz/unz + scatter A (inmatrix)     */
+#define SPKHIP_K_SMALL 1      /* whole small front in LDS                */
+#define SPKHIP_K_INIT 2       /* large front: load panels + extend-add   */
+#define SPKHIP_K_PANEL 3      /* large front: one piece inside its block */
+#define SPKHIP_K_UPDATE 4     /* large front: block trailing update      */
+#define SPKHIP_K_WRITEBACK 5  /* large front: panels back to lnz/unz     */
+#define SPKHIP_K_FWD 6        /* forward solve level                     */
+#define SPKHIP_K_BWD 7        /* backward solve level                    */
+#define SPKHIP_K_RESIDUAL 8   /* r = b - A x (refinement)                */
+#define SPKHIP_K_UROWS 9      /* large front: U rows of a block          */
+#define SPKHIP_K_FWD_GEMV 10  /* forward solve: rows below a block       */
+#define SPKHIP_K_BWD_GEMV 11  /* backward solve: U rows right of a block */
+#define SPKHIP_KCLASS_COUNT 12
+typedef struct spkhip_kstat {
+  int64_t launches;
+  double ms, flops, bytes;
+} spkhip_kstat_t;
+int64_t spkhip_profile_enable(spkhip_t h, int64_t on);
+int64_t spkhip_profile_get(spkhip_t h, int64_t kclass, spkhip_kstat_t* out);
 
 /* Library/device information. */
 int64_t spkhip_device_count(void);

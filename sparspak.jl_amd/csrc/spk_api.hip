@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -21,10 +22,12 @@ namespace {
 using i64 = int64_t;
 using i32 = int32_t;
 using spk::DevSym;
-using spk::StepDesc;
+using spk::BlkDesc;
 using spk::kWG;
 
-constexpr int kSmallLmax = 64;  // fronts with L <= this factor inside LDS
+using spk::kSmallLmax;
+constexpr int kGW = 128;    // widest block of pieces (LU)
+constexpr int kGMaxP = 8;   // most pieces in one block
 
 struct HipFail {
   hipError_t e;
@@ -59,9 +62,14 @@ struct DevBuf {
   void ensure(size_t count) {
     if (count > n || !p) alloc(count);
   }
+  // Host -> device copy that has completed on return (the host array may be
+  // a temporary).
   void upload(const T* h, size_t count, hipStream_t st) {
     alloc(count);
-    if (count) hipck(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+    if (count) {
+      hipck(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+      hipck(hipStreamSynchronize(st));
+    }
   }
   void upload(const std::vector<T>& v, hipStream_t st) { upload(v.data(), v.size(), st); }
 };
@@ -69,11 +77,24 @@ struct DevBuf {
 struct LevelPlan {
   int small_off = 0, small_cnt = 0, small_maxL = 0;
   int items_off = 0, items_cnt = 0;
-  struct Step {
-    int pd_off, pd_cnt, pwg;
-    int ud_off, ud_cnt, uwg;
+  struct Launch {
+    int off = 0, cnt = 0, wg = 0, lds_extra = 0;
+    int off256 = 0, wg256 = 0;  // panel: descriptors for 256 rows per workgroup
+    double flops = 0, elems = 0;  // algorithmic work of the launch
   };
-  std::vector<Step> steps;
+  struct Group {  // one block step of the level's large fronts
+    std::vector<Launch> panels;  // one launch per piece index in the block
+    Launch urows, gemm;
+  };
+  double small_flops = 0, small_elems = 0, init_elems = 0, wb_elems = 0, solve_elems = 0;
+  int max_w = 0;                    // widest front of the level
+  int large_off = 0, large_cnt = 0; // large fronts of the level
+  struct SolveStep {
+    int chain_off = 0, chain_cnt = 0, gemv_off = 0, gemv_cnt = 0;
+    double elems = 0;               // factor entries read by the step
+  };
+  std::vector<SolveStep> fsteps, bsteps;
+  std::vector<Group> groups;
   int solve_off = 0, solve_cnt = 0;
 };
 
@@ -101,8 +122,10 @@ struct spkhip_handle {
   DevBuf<i64> d_xsuper, d_snode, d_xlnz, d_xunz, d_xlindx, d_perm, d_invp;
   DevBuf<i32> d_lindx, d_fL, d_fW, d_fc0, d_fsp0, d_fnp, d_fcbld, d_fchp, d_fch,
       d_relmap, d_small, d_it_front, d_it_col, d_ea_child, d_ea_t, d_solve;
-  DevBuf<i64> d_foff, d_fcb, d_frel, d_fvec, d_it_ea;
-  DevBuf<StepDesc> d_pdesc, d_udesc;
+  DevBuf<i64> d_foff, d_fcb, d_frel, d_fvec, d_it_ea, d_lcol, d_ucol;
+  DevBuf<BlkDesc> d_bdesc;
+  DevBuf<i32> d_large;
+  DevBuf<spk::SolveDesc> d_sdesc;
   DevSym ds{};
   // matrix
   i64 nnz = 0;
@@ -123,10 +146,28 @@ struct spkhip_handle {
   i64 first_fail = 0;
   // solve / refine work
   DevBuf<double> d_b, d_x, d_r, d_tmp64, d_norms;
+  std::vector<i64> seg{0};            // row segments for the backward error
+  DevBuf<i64> d_seg;
   int refine_iters = 0;
   double backward_error = 0;
   // timing
   double t_upload = 0, t_inmatrix = 0, t_factor = 0, t_solve = 0, t_refine = 0;
+  // per-kernel-class profile (events around every launch when enabled)
+  bool prof = false;
+  spkhip_kstat_t kstat[SPKHIP_KCLASS_COUNT] = {};
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
+  struct Rec {
+    int cls;
+    size_t ea, eb;
+    double flops, bytes;
+    int level, grid;
+  };
+  int cur_level = -1, cur_grid = 0;   // tags of the next profiled launches
+  std::vector<Rec> recs;
+  ~spkhip_handle() {
+    for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+  }
 };
 
 namespace {
@@ -134,6 +175,57 @@ namespace {
 using Clock = std::chrono::steady_clock;
 inline double ms_since(Clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
+size_t prof_event(spkhip_handle* h) {
+  if (h->evused == h->evpool.size()) {
+    hipEvent_t e;
+    hipck(hipEventCreate(&e));
+    h->evpool.push_back(e);
+  }
+  return h->evused++;
+}
+
+// Records an event pair around the launches made during its lifetime.
+struct Prof {
+  spkhip_handle* h;
+  int cls;
+  double flops, bytes;
+  size_t ea = 0;
+  Prof(spkhip_handle* hh, int c, double fl, double by) : h(hh), cls(c), flops(fl), bytes(by) {
+    if (h->prof) {
+      ea = prof_event(h);
+      hipck(hipEventRecord(h->evpool[ea], h->stream));
+    }
+  }
+  ~Prof() {
+    if (h->prof) {
+      size_t eb = prof_event(h);
+      (void)hipEventRecord(h->evpool[eb], h->stream);
+      h->recs.push_back({cls, ea, eb, flops, bytes, h->cur_level, h->cur_grid});
+    }
+  }
+};
+
+void prof_collect(spkhip_handle* h) {
+  if (!h->prof || h->recs.empty()) return;
+  hipck(hipStreamSynchronize(h->stream));
+  // optional per-launch log (class, level, grid, ms) for tuning
+  const char* dump = std::getenv("SPKHIP_PROFILE_LAUNCHES");
+  FILE* fo = dump ? std::fopen(dump, "a") : nullptr;
+  for (const auto& r : h->recs) {
+    float ms = 0;
+    hipck(hipEventElapsedTime(&ms, h->evpool[r.ea], h->evpool[r.eb]));
+    if (fo) std::fprintf(fo, "%d,%d,%d,%.6f,%.6g\n", r.cls, r.level, r.grid, ms, r.flops);
+    spkhip_kstat_t& k = h->kstat[r.cls];
+    k.launches += 1;
+    k.ms += ms;
+    k.flops += r.flops;
+    k.bytes += r.bytes;
+  }
+  if (fo) std::fclose(fo);
+  h->recs.clear();
+  h->evused = 0;
 }
 
 inline unsigned grid_for(i64 n, int bs = kWG) {
@@ -266,8 +358,8 @@ void build_fronts(spkhip_handle* h) {
 void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_front,
                 std::vector<i32>& it_col, std::vector<i64>& it_ea,
                 std::vector<i32>& ea_child, std::vector<i32>& ea_t,
-                std::vector<StepDesc>& pdesc, std::vector<StepDesc>& udesc,
-                std::vector<i32>& solve) {
+                std::vector<BlkDesc>& bdesc, std::vector<i32>& solve, std::vector<i32>& largel,
+                std::vector<spk::SolveDesc>& sdesc) {
   const i32 nf = h->nfront;
   std::vector<std::vector<i32>> bylev(h->nlevel);
   for (i32 f = 0; f < nf; ++f) bylev[h->flev[f]].push_back(f);
@@ -291,11 +383,37 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
     }
     lp.small_cnt = (int)small.size() - lp.small_off;
     lp.solve_cnt = (int)solve.size() - lp.solve_off;
+    lp.large_off = (int)largel.size();
+    for (i32 f : bylev[l]) {
+      lp.max_w = std::max(lp.max_w, h->fW[f]);
+      if (h->fL[f] > kSmallLmax) largel.push_back(f);
+    }
+    lp.large_cnt = (int)largel.size() - lp.large_off;
+    for (i32 f : bylev[l]) {
+      const double L = h->fL[f], W = h->fW[f], R = L - W;
+      double fl = 0;
+      for (double k = 0; k < W; k += 1) {
+        double m = L - k - 1;
+        fl += m + 2.0 * m * m;
+      }
+      double panel = L * L - R * R;  // lnz + unz entries of the front
+      double cbin = 0;
+      for (i32 ci = h->fchp[f]; ci < h->fchp[f + 1]; ++ci) {
+        double rc = h->fL[h->fch[ci]] - h->fW[h->fch[ci]];
+        cbin += rc * rc;
+      }
+      lp.solve_elems += panel;
+      if (h->fL[f] <= kSmallLmax) {
+        lp.small_flops += fl;
+        lp.small_elems += 2 * panel + R * R + cbin;
+      } else {
+        lp.init_elems += L * L + panel + cbin;
+        lp.wb_elems += 2 * panel;
+      }
+    }
     lp.items_off = (int)it_front.size();
-    int maxnp = 0;
     for (i32 f : large) {
       const i32 L = h->fL[f];
-      maxnp = std::max(maxnp, h->fnp[f]);
       // extend-add entries per destination column
       colcnt.assign(L + 1, 0);
       for (i32 ci = h->fchp[f]; ci < h->fchp[f + 1]; ++ci) {
@@ -323,34 +441,142 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
       }
     }
     lp.items_cnt = (int)it_front.size() - lp.items_off;
-    for (int st = 0; st < maxnp; ++st) {
-      LevelPlan::Step S{};
-      S.pd_off = (int)pdesc.size();
-      S.ud_off = (int)udesc.size();
-      int pwg = 0, uwg = 0, slot = 0;
-      for (i32 f : large) {
-        if (st >= h->fnp[f]) continue;
-        i32 p = h->fsp0[f] + st;
-        i64 cq = h->xsuper[p] - 1 - h->fc0[f];
-        i64 nq = h->xsuper[p + 1] - h->xsuper[p];
-        i64 m = h->fL[f] - cq - nq;
-        int nl = m > 0 ? (int)((m + kWG - 1) / kWG) : 0;
-        int nu = m > 0 ? (int)((m + spk::kUcols - 1) / spk::kUcols) : 0;
-        StepDesc pd{f, p, pwg, nl, slot};
-        pdesc.push_back(pd);
-        pwg += std::max(1, nl + nu);
-        int nt = m > 0 ? (int)((m + spk::kTile - 1) / spk::kTile) : 0;
-        StepDesc ud{f, p, uwg, nt, slot};
-        udesc.push_back(ud);
-        uwg += nt * nt + 1;
-        ++slot;
+    // blocks of consecutive pieces, at most kGW columns each (a piece wider
+    // than that forms its own block); SPD keeps one piece per block
+    struct Blk {
+      i32 p0, np, c0, c1;
+    };
+    std::vector<std::vector<Blk>> fblocks;
+    size_t maxg = 0;
+    for (i32 f : large) {
+      std::vector<Blk> bl;
+      const i32 gw = (h->kind == SPKHIP_KIND_LU) ? kGW : 0;
+      for (i32 p = h->fsp0[f]; p < h->fsp0[f] + h->fnp[f]; ++p) {
+        const i32 cq = (i32)(h->xsuper[p] - 1 - h->fc0[f]);
+        const i32 nq = (i32)(h->xsuper[p + 1] - h->xsuper[p]);
+        if (!bl.empty() && cq + nq - bl.back().c0 <= gw && bl.back().np < kGMaxP) {
+          bl.back().np += 1;
+          bl.back().c1 = cq + nq;
+        } else {
+          bl.push_back(Blk{p, 1, cq, cq + nq});
+        }
       }
-      S.pd_cnt = (int)pdesc.size() - S.pd_off;
-      S.ud_cnt = (int)udesc.size() - S.ud_off;
-      S.pwg = pwg;
-      S.uwg = uwg;
-      h->max_step_fronts = std::max<i64>(h->max_step_fronts, slot);
-      lp.steps.push_back(S);
+      maxg = std::max(maxg, bl.size());
+      fblocks.push_back(bl);
+    }
+    for (size_t gi = 0; gi < maxg; ++gi) {
+      LevelPlan::Group G;
+      int maxp = 0;
+      std::vector<int> slot_base(large.size(), 0);
+      int slots = 0;
+      for (size_t fi = 0; fi < large.size(); ++fi)
+        if (gi < fblocks[fi].size()) {
+          slot_base[fi] = slots;
+          slots += fblocks[fi][gi].np;
+          maxp = std::max(maxp, fblocks[fi][gi].np);
+        }
+      h->max_step_fronts = std::max<i64>(h->max_step_fronts, slots);
+      for (int pi = 0; pi < maxp; ++pi) {
+        LevelPlan::Launch P;
+        P.off = (int)bdesc.size();
+        std::vector<BlkDesc> p256;
+        for (size_t fi = 0; fi < large.size(); ++fi) {
+          if (gi >= fblocks[fi].size() || pi >= fblocks[fi][gi].np) continue;
+          const Blk& B = fblocks[fi][gi];
+          const i32 f = large[fi];
+          const i32 p = B.p0 + pi;
+          const i32 cq = (i32)(h->xsuper[p] - 1 - h->fc0[f]);
+          const i32 nq = (i32)(h->xsuper[p + 1] - h->xsuper[p]);
+          const i32 m = h->fL[f] - cq - nq;
+          const i32 uc = B.c1 - cq - nq;
+          bdesc.push_back(BlkDesc{f, p, B.np, B.c0, B.c1, P.wg, 0, slot_base[fi] + pi});
+          P.wg += std::max(1, (m + 63) / 64);
+          p256.push_back(BlkDesc{f, p, B.np, B.c0, B.c1, P.wg256, 0, slot_base[fi] + pi});
+          P.wg256 += std::max(1, (m + 255) / 256);
+          P.lds_extra = std::max(P.lds_extra, uc);
+          const double md = m, qd = nq, ud = uc;
+          P.flops += 2.0 * qd * qd * qd / 3.0 + qd * qd * ud + md * qd * qd + 2.0 * md * qd * ud;
+          P.elems += 2.0 * (md + qd) * (qd + ud);
+        }
+        P.cnt = (int)bdesc.size() - P.off;
+        P.off256 = (int)bdesc.size();
+        bdesc.insert(bdesc.end(), p256.begin(), p256.end());
+        G.panels.push_back(P);
+      }
+      G.urows.off = (int)bdesc.size();
+      for (size_t fi = 0; fi < large.size(); ++fi) {
+        if (gi >= fblocks[fi].size()) continue;
+        const Blk& B = fblocks[fi][gi];
+        const i32 f = large[fi];
+        const i32 t = h->fL[f] - B.c1;
+        if (t <= 0) continue;
+        bdesc.push_back(BlkDesc{f, B.p0, B.np, B.c0, B.c1, G.urows.wg, 0, slot_base[fi]});
+        G.urows.wg += (t + 31) / 32;
+        const double bw = B.c1 - B.c0;
+        G.urows.flops += bw * bw * t;
+        G.urows.elems += 2.0 * bw * t;
+      }
+      G.urows.cnt = (int)bdesc.size() - G.urows.off;
+      G.gemm.off = (int)bdesc.size();
+      for (size_t fi = 0; fi < large.size(); ++fi) {
+        if (gi >= fblocks[fi].size()) continue;
+        const Blk& B = fblocks[fi][gi];
+        const i32 f = large[fi];
+        const i32 t = h->fL[f] - B.c1;
+        const i32 nt = t > 0 ? (t + 127) / 128 : 0;
+        bdesc.push_back(BlkDesc{f, B.p0, B.np, B.c0, B.c1, G.gemm.wg, nt, slot_base[fi]});
+        G.gemm.wg += nt * nt + B.np;
+        const double td = t, bw = B.c1 - B.c0;
+        G.gemm.flops += 2.0 * td * td * bw;
+        G.gemm.elems += 2.0 * td * td + 2.0 * td * bw;
+      }
+      G.gemm.cnt = (int)bdesc.size() - G.gemm.off;
+      lp.groups.push_back(G);
+    }
+    // blocked solve steps over the same blocks
+    for (size_t gi = 0; gi < maxg; ++gi) {
+      LevelPlan::SolveStep S;
+      S.chain_off = (int)sdesc.size();
+      for (size_t fi = 0; fi < large.size(); ++fi)
+        if (gi < fblocks[fi].size()) {
+          const Blk& B = fblocks[fi][gi];
+          sdesc.push_back(spk::SolveDesc{large[fi], B.p0, B.np, B.c0, B.c1, 0});
+        }
+      S.chain_cnt = (int)sdesc.size() - S.chain_off;
+      S.gemv_off = (int)sdesc.size();
+      for (size_t fi = 0; fi < large.size(); ++fi)
+        if (gi < fblocks[fi].size()) {
+          const Blk& B = fblocks[fi][gi];
+          const i32 t = h->fL[large[fi]] - B.c1;
+          for (i32 r0 = 0; r0 < t; r0 += 64)
+            sdesc.push_back(spk::SolveDesc{large[fi], B.p0, B.np, B.c0, B.c1, r0});
+          S.elems += (double)(B.c1 - B.c0) * (h->fL[large[fi]] - B.c0);
+        }
+      S.gemv_cnt = (int)sdesc.size() - S.gemv_off;
+      lp.fsteps.push_back(S);
+    }
+    for (size_t si = 0; si < maxg; ++si) {
+      LevelPlan::SolveStep S;
+      S.gemv_off = (int)sdesc.size();
+      for (size_t fi = 0; fi < large.size(); ++fi) {
+        const i64 bi = (i64)fblocks[fi].size() - 1 - (i64)si;
+        if (bi < 0) continue;
+        const Blk& B = fblocks[fi][bi];
+        if (B.c1 >= h->fL[large[fi]]) continue;
+        for (i32 r0 = 0; r0 < B.c1 - B.c0; r0 += 4)
+          sdesc.push_back(spk::SolveDesc{large[fi], B.p0, B.np, B.c0, B.c1, r0});
+      }
+      S.gemv_cnt = (int)sdesc.size() - S.gemv_off;
+      S.chain_off = (int)sdesc.size();
+      for (size_t fi = 0; fi < large.size(); ++fi) {
+        const i64 bi = (i64)fblocks[fi].size() - 1 - (i64)si;
+        if (bi < 0) continue;
+        const Blk& B = fblocks[fi][bi];
+        sdesc.push_back(spk::SolveDesc{large[fi], B.p0, B.np, B.c0, B.c1, 0});
+        S.elems += (double)(B.c1 - B.c0) * (h->fL[large[fi]] - B.c0);
+      }
+      S.chain_cnt = (int)sdesc.size() - S.chain_off;
+      lp.bsteps.push_back(S);
     }
   }
 }
@@ -428,18 +654,22 @@ struct Bufs<float> {
   static DevBuf<float>& rhs(spkhip_handle* h) { return h->d_rhs32; }
 };
 
-template <typename T, int MAXW>
+template <typename T, int MAXW, int GW, int NC>
 void set_lds_attrs() {
   static bool done = false;
   if (done) return;
   hipck(hipFuncSetAttribute((const void*)spk::small_front_kernel<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize,
                             kSmallLmax * kSmallLmax * (int)sizeof(T)));
+  hipck(hipFuncSetAttribute((const void*)spk::blk_panel_kernel<T, MAXW, GW, NC>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+  hipck(hipFuncSetAttribute((const void*)spk::blk_urows_kernel<T, MAXW, GW, NC>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
   done = true;
 }
 
 // The factorisation proper on already assembled lnz/unz (reference layout).
-template <typename T, int MAXW>
+template <typename T, int MAXW, int GW, int NC>
 void run_factor(spkhip_handle* h) {
   hipStream_t st = h->stream;
   DevBuf<T>& lnz = Bufs<T>::lnz(h);
@@ -447,35 +677,60 @@ void run_factor(spkhip_handle* h) {
   DevBuf<T>& arena = Bufs<T>::arena(h);
   DevBuf<T>& stage = Bufs<T>::stage(h);
   arena.ensure((size_t)std::max<i64>(h->arena_elems, 1));
-  stage.ensure((size_t)h->max_step_fronts * MAXW * MAXW);
+  stage.ensure((size_t)h->max_step_fronts * spk::stage_piece_elems<MAXW, GW>());
   h->d_ipvt.ensure((size_t)h->n);
   h->d_fail.ensure(1);
   hipLaunchKernelGGL(spk::fill_kernel<unsigned long long>, dim3(1), dim3(kWG), 0, st,
                      h->d_fail.p, (int64_t)1, ~0ull);
-  set_lds_attrs<T, MAXW>();
+  set_lds_attrs<T, MAXW, GW, NC>();
   for (const LevelPlan& lp : h->levels) {
+    h->cur_level = (int)(&lp - h->levels.data());
+    h->cur_grid = lp.small_cnt;
     if (lp.small_cnt > 0) {
       size_t lds = (size_t)lp.small_maxL * lp.small_maxL * sizeof(T);
+      Prof pr(h, SPKHIP_K_SMALL, lp.small_flops, lp.small_elems * sizeof(T));
       hipLaunchKernelGGL(spk::small_front_kernel<T>, dim3(lp.small_cnt), dim3(kWG), lds, st,
                          h->ds, h->d_small.p + lp.small_off, lnz.p, unz.p, h->d_ipvt.p,
                          arena.p, h->d_fail.p);
       hipck(hipGetLastError());
     }
     if (lp.items_cnt > 0) {
+      {
+      Prof pr(h, SPKHIP_K_INIT, 0.0, lp.init_elems * sizeof(T));
       hipLaunchKernelGGL(spk::large_init_kernel<T>, dim3(lp.items_cnt), dim3(kWG), 0, st, h->ds,
                          h->d_it_front.p + lp.items_off, h->d_it_col.p + lp.items_off,
                          h->d_it_ea.p + lp.items_off, h->d_ea_child.p, h->d_ea_t.p, lnz.p,
                          unz.p, arena.p);
       hipck(hipGetLastError());
-      for (const LevelPlan::Step& S : lp.steps) {
-        hipLaunchKernelGGL((spk::large_panel_kernel<T, MAXW>), dim3(S.pwg), dim3(kWG), 0, st,
-                           h->ds, h->d_pdesc.p + S.pd_off, S.pd_cnt, arena.p, stage.p,
-                           h->d_ipvt.p, h->d_fail.p);
-        hipck(hipGetLastError());
-        hipLaunchKernelGGL((spk::large_update_kernel<T, MAXW>), dim3(S.uwg), dim3(kWG), 0, st,
-                           h->ds, h->d_udesc.p + S.ud_off, S.ud_cnt, arena.p, stage.p);
+      }
+      for (const LevelPlan::Group& G : lp.groups) {
+        for (const LevelPlan::Launch& P : G.panels) {
+          Prof pr(h, SPKHIP_K_PANEL, P.flops, P.elems * sizeof(T));
+          const size_t lds =
+              (size_t)(MAXW * MAXW + 64 * MAXW + (MAXW + 1) * P.lds_extra) * sizeof(T);
+          const int wg = P.wg;
+          h->cur_grid = wg;
+          hipLaunchKernelGGL((spk::blk_panel_kernel<T, MAXW, GW, NC>), dim3(wg), dim3(kWG), lds,
+                             st, h->ds, h->d_bdesc.p + P.off, P.cnt, arena.p, stage.p,
+                             h->d_ipvt.p, h->d_fail.p);
+          hipck(hipGetLastError());
+        }
+        h->cur_grid = G.urows.wg;
+        if (G.urows.cnt > 0) {
+          Prof pr(h, SPKHIP_K_UROWS, G.urows.flops, G.urows.elems * sizeof(T));
+          const size_t lds = (size_t)((GW + 1) * 32 + MAXW * MAXW) * sizeof(T);
+          hipLaunchKernelGGL((spk::blk_urows_kernel<T, MAXW, GW, NC>), dim3(G.urows.wg), dim3(kWG),
+                             lds, st, h->ds, h->d_bdesc.p + G.urows.off, G.urows.cnt, arena.p,
+                             stage.p, h->d_ipvt.p);
+          hipck(hipGetLastError());
+        }
+        h->cur_grid = G.gemm.wg;
+        Prof pr(h, SPKHIP_K_UPDATE, G.gemm.flops, G.gemm.elems * sizeof(T));
+        hipLaunchKernelGGL((spk::blk_gemm_kernel<T, MAXW, GW, NC>), dim3(G.gemm.wg), dim3(kWG), 0,
+                           st, h->ds, h->d_bdesc.p + G.gemm.off, G.gemm.cnt, arena.p, stage.p);
         hipck(hipGetLastError());
       }
+      Prof pr(h, SPKHIP_K_WRITEBACK, 0.0, lp.wb_elems * sizeof(T));
       hipLaunchKernelGGL(spk::large_writeback_kernel<T>, dim3(lp.items_cnt), dim3(kWG), 0, st,
                          h->ds, h->d_it_front.p + lp.items_off, h->d_it_col.p + lp.items_off,
                          lnz.p, unz.p, arena.p);
@@ -496,6 +751,8 @@ void run_assemble(spkhip_handle* h) {
   DevBuf<T>& unz = Bufs<T>::unz(h);
   lnz.ensure((size_t)std::max<i64>(h->nlnz, 1));
   unz.ensure((size_t)std::max<i64>(h->nunz, 1));
+  Prof pr(h, SPKHIP_K_ASSEMBLE, 0.0,
+          (double)(h->nlnz + h->nunz) * sizeof(T) + (double)h->nnz * (16.0 + sizeof(T)));
   hipLaunchKernelGGL(spk::fill_kernel<T>, dim3(grid_for(h->nlnz)), dim3(kWG), 0, st, lnz.p,
                      h->nlnz, T(0));
   hipLaunchKernelGGL(spk::fill_kernel<T>, dim3(grid_for(h->nunz)), dim3(kWG), 0, st, unz.p,
@@ -506,8 +763,8 @@ void run_assemble(spkhip_handle* h) {
   hipck(hipGetLastError());
 }
 
-template <typename T>
-void solve_levels(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bwd) {
+template <typename T, int MAXW, int GW>
+void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bwd) {
   hipStream_t st = h->stream;
   DevBuf<T>& vec = Bufs<T>::vec(h);
   vec.ensure((size_t)std::max<i64>(h->vec_elems * nrhs, 1));
@@ -515,19 +772,67 @@ void solve_levels(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bw
   const T* unz = Bufs<T>::unz(h).p;
   if (fwd)
     for (const LevelPlan& lp : h->levels) {
-      if (lp.solve_cnt == 0) continue;
-      hipLaunchKernelGGL(spk::fwd_solve_kernel<T>, dim3(lp.solve_cnt), dim3(kWG), 0, st, h->ds,
-                         h->d_solve.p + lp.solve_off, lnz, h->d_ipvt.p, rhs, ldr, nrhs, vec.p);
-      hipck(hipGetLastError());
+      if (lp.small_cnt > 0) {
+        Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
+        hipLaunchKernelGGL((spk::fwd_small_kernel<T, MAXW>), dim3(lp.small_cnt), dim3(kWG),
+                           (size_t)lp.small_maxL * sizeof(T), st, h->ds,
+                           h->d_small.p + lp.small_off, lnz, h->d_ipvt.p, rhs, ldr, nrhs, vec.p);
+        hipck(hipGetLastError());
+      }
+      if (lp.large_cnt == 0) continue;
+      {
+        Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
+        hipLaunchKernelGGL(spk::fwd_assemble_kernel<T>, dim3(lp.large_cnt), dim3(kWG), 0, st,
+                           h->ds, h->d_large.p + lp.large_off, rhs, ldr, nrhs, vec.p);
+        hipck(hipGetLastError());
+      }
+      for (const LevelPlan::SolveStep& S : lp.fsteps) {
+        {
+          Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
+          hipLaunchKernelGGL((spk::fwd_chain_kernel<T, MAXW, GW>), dim3(S.chain_cnt), dim3(kWG),
+                             0, st, h->ds, h->d_sdesc.p + S.chain_off, lnz, h->d_ipvt.p, rhs, ldr,
+                             nrhs, vec.p);
+          hipck(hipGetLastError());
+        }
+        if (S.gemv_cnt > 0) {
+          Prof pr(h, SPKHIP_K_FWD_GEMV, 2.0 * S.elems * nrhs, S.elems * sizeof(T));
+          hipLaunchKernelGGL(spk::fwd_bgemv_kernel<T>, dim3(S.gemv_cnt), dim3(kWG), 0, st, h->ds,
+                             h->d_sdesc.p + S.gemv_off, lnz, nrhs, vec.p);
+          hipck(hipGetLastError());
+        }
+      }
     }
   if (bwd)
     for (int l = (int)h->levels.size() - 1; l >= 0; --l) {
       const LevelPlan& lp = h->levels[l];
-      if (lp.solve_cnt == 0) continue;
-      hipLaunchKernelGGL(spk::bwd_solve_kernel<T>, dim3(lp.solve_cnt), dim3(kWG), 0, st, h->ds,
-                         h->d_solve.p + lp.solve_off, lnz, unz, rhs, ldr, nrhs, vec.p);
-      hipck(hipGetLastError());
+      for (const LevelPlan::SolveStep& S : lp.bsteps) {
+        if (S.gemv_cnt > 0) {
+          Prof pr(h, SPKHIP_K_BWD_GEMV, 2.0 * S.elems * nrhs, S.elems * sizeof(T));
+          hipLaunchKernelGGL(spk::bwd_bgemv_kernel<T>, dim3(S.gemv_cnt), dim3(kWG), 0, st, h->ds,
+                             h->d_sdesc.p + S.gemv_off, unz, rhs, ldr, nrhs, vec.p);
+          hipck(hipGetLastError());
+        }
+        Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);
+        hipLaunchKernelGGL((spk::bwd_chain_kernel<T, MAXW, GW>), dim3(S.chain_cnt), dim3(kWG), 0,
+                           st, h->ds, h->d_sdesc.p + S.chain_off, lnz, unz, rhs, ldr, nrhs, vec.p);
+        hipck(hipGetLastError());
+      }
+      if (lp.small_cnt > 0) {
+        Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);
+        hipLaunchKernelGGL((spk::bwd_small_kernel<T, MAXW>), dim3(lp.small_cnt), dim3(kWG),
+                           (size_t)lp.small_maxL * sizeof(T), st, h->ds,
+                           h->d_small.p + lp.small_off, lnz, unz, rhs, ldr, nrhs);
+        hipck(hipGetLastError());
+      }
     }
+}
+
+template <typename T>
+void solve_levels(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bwd) {
+  if (h->kind == SPKHIP_KIND_LU)
+    solve_levels_w<T, 64, kGW>(h, rhs, ldr, nrhs, fwd, bwd);
+  else
+    solve_levels_w<T, 96, 96>(h, rhs, ldr, nrhs, fwd, bwd);
 }
 
 // Full solve of device B (double, original numbering) with the current
@@ -603,14 +908,18 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   h->factor_flops = fl;
   h->solve_flops = (kind == SPKHIP_KIND_LU) ? 2.0 * (2.0 * sl - (double)n) - 3.0 * n
                                             : 4.0 * sl - 3.0 * n;
-  const i64 wmax = (kind == SPKHIP_KIND_LU) ? 64 : 96;
+  const i64 wmax = (kind == SPKHIP_KIND_LU) ? 48 : 96;
   for (i64 J = 0; J < nsuper; ++J)
     if (xsuper[J + 1] - xsuper[J] > wmax) throw ApiError{-2};
   build_fronts(h);
-  std::vector<i32> small, it_front, it_col, ea_child, ea_t, solve;
+  std::vector<i32> small, it_front, it_col, ea_child, ea_t, solve, largel;
+  std::vector<spk::SolveDesc> sdesc;
   std::vector<i64> it_ea;
-  std::vector<StepDesc> pdesc, udesc;
-  build_plan(h, small, it_front, it_col, it_ea, ea_child, ea_t, pdesc, udesc, solve);
+  std::vector<BlkDesc> bdesc;
+  build_plan(h, small, it_front, it_col, it_ea, ea_child, ea_t, bdesc, solve, largel, sdesc);
+
+  if (largel.empty()) largel.push_back(0);
+  if (sdesc.empty()) sdesc.push_back(spk::SolveDesc{});
   hipStream_t st = h->stream;
   h->d_xsuper.upload(h->xsuper, st);
   h->d_snode.upload(h->snode, st);
@@ -634,6 +943,22 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   if (h->fch.empty()) h->fch.push_back(0);
   h->d_fch.upload(h->fch, st);
   h->d_fvec.upload(h->fvec, st);
+  {  // per-column base offsets of the reference layout in front coordinates
+    std::vector<i64> lcol(n), ucol(n);
+    for (i32 f = 0; f < h->nfront; ++f) {
+      const i64 fc0 = h->fc0[f];
+      for (i64 p = h->fsp0[f]; p < h->fsp0[f] + h->fnp[f]; ++p) {
+        const i64 pc0 = h->xsuper[p] - 1, nq = h->xsuper[p + 1] - h->xsuper[p];
+        const i64 cq = pc0 - fc0;
+        for (i64 g = pc0; g < pc0 + nq; ++g) {
+          lcol[g] = h->xlnz[g] - 1 - cq;
+          ucol[g] = h->xunz[g] - 1 - (cq + nq);
+        }
+      }
+    }
+    h->d_lcol.upload(lcol, st);
+    h->d_ucol.upload(ucol, st);
+  }
   if (small.empty()) small.push_back(0);
   h->d_small.upload(small, st);
   if (it_front.empty()) {
@@ -649,12 +974,12 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   }
   h->d_ea_child.upload(ea_child, st);
   h->d_ea_t.upload(ea_t, st);
-  if (pdesc.empty()) pdesc.push_back(StepDesc{});
-  if (udesc.empty()) udesc.push_back(StepDesc{});
-  h->d_pdesc.upload(pdesc, st);
-  h->d_udesc.upload(udesc, st);
+  if (bdesc.empty()) bdesc.push_back(BlkDesc{});
+  h->d_bdesc.upload(bdesc, st);
   if (solve.empty()) solve.push_back(0);
   h->d_solve.upload(solve, st);
+  h->d_large.upload(largel, st);
+  h->d_sdesc.upload(sdesc, st);
   DevSym& d = h->ds;
   d.n = n;
   d.nsuper = nsuper;
@@ -678,6 +1003,8 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   d.fchp = h->d_fchp.p;
   d.fch = h->d_fch.p;
   d.fvec = h->d_fvec.p;
+  d.lcol = h->d_lcol.p;
+  d.ucol = h->d_ucol.p;
   d.pivot = (kind == SPKHIP_KIND_LU) ? 1 : 0;
   d.sym = (kind == SPKHIP_KIND_SPD) ? 1 : 0;
   sync(h);
@@ -795,9 +1122,9 @@ int64_t lufactor_stateless(i64 n, i64 nsuper, const i64* xsuper, const i64* snod
       hipck(hipMemcpyAsync(du.p, unz, h->nunz * sizeof(T), hipMemcpyHostToDevice, st));
     const int prec = std::is_same<T, double>::value ? SPKHIP_FP64 : SPKHIP_FP32;
     if (kind == SPKHIP_KIND_LU)
-      run_factor<T, 64>(h);
+      run_factor<T, 64, kGW, 48>(h);
     else
-      run_factor<T, 96>(h);
+      run_factor<T, 96, 96, 0>(h);
     h->precision = prec;
     h->factored = true;
     download_impl<T>(h, lnz, kind == SPKHIP_KIND_LU ? unz : nullptr, ipvt);
@@ -1007,18 +1334,26 @@ int64_t spkhip_factor(spkhip_t h, int64_t precision) {
     hipck(hipSetDevice(h->device));
     hipck(hipEventRecord(h->ev0, h->stream));
     const bool lu = h->kind == SPKHIP_KIND_LU;
+    // tuning knob: SPKHIP_PANEL_LDS=1 factors the pieces' diagonal blocks
+    // through LDS instead of registers
+    const bool lds_panel = std::getenv("SPKHIP_PANEL_LDS") != nullptr;
     if (precision == SPKHIP_FP64) {
       run_assemble<double>(h);
-      if (lu) run_factor<double, 64>(h); else run_factor<double, 96>(h);
+      if (!lu) run_factor<double, 96, 96, 0>(h);
+      else if (lds_panel) run_factor<double, 64, kGW, 0>(h);
+      else run_factor<double, 64, kGW, 48>(h);
     } else {
       run_assemble<float>(h);
-      if (lu) run_factor<float, 64>(h); else run_factor<float, 96>(h);
+      if (!lu) run_factor<float, 96, 96, 0>(h);
+      else if (lds_panel) run_factor<float, 64, kGW, 0>(h);
+      else run_factor<float, 64, kGW, 48>(h);
     }
     hipck(hipEventRecord(h->ev1, h->stream));
     hipck(hipEventSynchronize(h->ev1));
     float ms = 0;
     hipck(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->t_factor = ms;
+    prof_collect(h);
     h->precision = (int)precision;
     h->factored = true;
     return h->first_fail ? -1 : 0;
@@ -1041,6 +1376,7 @@ int64_t spkhip_solve_device(spkhip_t h, int64_t nrhs, double* d_b, int64_t ldb) 
     float ms = 0;
     hipck(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->t_solve = ms;
+    prof_collect(h);
     return 0;
   });
 }
@@ -1094,9 +1430,20 @@ int64_t refine_impl(spkhip_t h, const double* d_bm, double* d_xo, int nrhs, doub
   // x0 = solve(b)
   hipLaunchKernelGGL(spk::fill_kernel<double>, dim3(grid_for(n * nrhs)), dim3(kWG), 0, st, d_xo,
                      n * nrhs, 0.0);
-  std::vector<double> nrm(3 * nrhs);
-  hipLaunchKernelGGL(spk::colmaxabs_kernel, dim3(nrhs), dim3(kWG), 0, st, d_bm, n, n,
-                     h->d_norms.p + 2 * nrhs);
+  // per (segment, column) norms: r, x, b, and ||A||_inf per segment
+  if (h->seg.size() < 2 || h->seg.back() != n) h->seg = {0, n};
+  const int nseg = (int)h->seg.size() - 1;
+  h->d_seg.upload(h->seg, st);
+  h->d_norms.ensure((size_t)3 * nseg * nrhs + nseg);
+  double* nr_r = h->d_norms.p;
+  double* nr_x = nr_r + (size_t)nseg * nrhs;
+  double* nr_b = nr_x + (size_t)nseg * nrhs;
+  double* nr_a = nr_b + (size_t)nseg * nrhs;
+  std::vector<double> nrm((size_t)3 * nseg * nrhs + nseg);
+  hipLaunchKernelGGL(spk::segmaxabs_kernel, dim3(nseg * nrhs), dim3(kWG), 0, st, d_bm, n,
+                     h->d_seg.p, nseg, nr_b);
+  hipLaunchKernelGGL(spk::segmaxabs_kernel, dim3(nseg), dim3(kWG), 0, st, h->d_rowabs.p, n,
+                     h->d_seg.p, nseg, nr_a);
   const double* src = d_bm;
   int iters = 0;
   double eta_max = 0;
@@ -1110,20 +1457,25 @@ int64_t refine_impl(spkhip_t h, const double* d_bm, double* d_xo, int nrhs, doub
     hipLaunchKernelGGL(spk::axpy_kernel<T>, dim3(grid_for(n)), dim3(kWG), 0, st, d_xo, n, tbuf, n,
                        n, nrhs);
     // r = b - A x (double), eta per column
-    hipLaunchKernelGGL(spk::residual_kernel, dim3(grid_for(n)), dim3(kWG), 0, st, h->d_rowptr.p,
-                       h->d_colind.p, h->d_csrval.p, d_xo, n, d_bm, n, h->d_r.p, n, n, nrhs);
-    hipLaunchKernelGGL(spk::colmaxabs_kernel, dim3(nrhs), dim3(kWG), 0, st, h->d_r.p, n, n,
-                       h->d_norms.p);
-    hipLaunchKernelGGL(spk::colmaxabs_kernel, dim3(nrhs), dim3(kWG), 0, st, d_xo, n, n,
-                       h->d_norms.p + nrhs);
+    {
+      Prof pr(h, SPKHIP_K_RESIDUAL, 2.0 * h->nnz * nrhs,
+              (double)h->nnz * 12.0 + (double)n * 24.0 * nrhs);
+      hipLaunchKernelGGL(spk::residual_kernel, dim3(grid_for(n)), dim3(kWG), 0, st, h->d_rowptr.p,
+                         h->d_colind.p, h->d_csrval.p, d_xo, n, d_bm, n, h->d_r.p, n, n, nrhs);
+    }
+    hipLaunchKernelGGL(spk::segmaxabs_kernel, dim3(nseg * nrhs), dim3(kWG), 0, st, h->d_r.p, n,
+                       h->d_seg.p, nseg, nr_r);
+    hipLaunchKernelGGL(spk::segmaxabs_kernel, dim3(nseg * nrhs), dim3(kWG), 0, st, d_xo, n,
+                       h->d_seg.p, nseg, nr_x);
     hipck(hipGetLastError());
-    hipck(hipMemcpyAsync(nrm.data(), h->d_norms.p, 3 * nrhs * sizeof(double),
+    hipck(hipMemcpyAsync(nrm.data(), h->d_norms.p, nrm.size() * sizeof(double),
                          hipMemcpyDeviceToHost, st));
     sync(h);
     eta_max = 0;
-    for (int q = 0; q < nrhs; ++q) {
-      double den = h->anorm * nrm[nrhs + q] + nrm[2 * nrhs + q];
-      double eta = den > 0 ? nrm[q] / den : 0.0;
+    const size_t sq = (size_t)nseg * nrhs;
+    for (size_t i = 0; i < sq; ++i) {
+      const double den = nrm[3 * sq + i % nseg] * nrm[sq + i] + nrm[2 * sq + i];
+      const double eta = den > 0 ? nrm[i] / den : 0.0;
       eta_max = std::max(eta_max, eta);
     }
     if (eta_max <= tol || iters >= max_iter || !std::isfinite(eta_max)) break;
@@ -1164,6 +1516,7 @@ int64_t spkhip_solve_refine_device(spkhip_t h, int64_t nrhs, double* d_b, int64_
     float ms = 0;
     hipck(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->t_refine = ms;
+    prof_collect(h);
     return rc;
   });
 }
@@ -1200,6 +1553,31 @@ int64_t spkhip_download_f32(spkhip_t h, float* lnz, float* unz, int64_t* ipvt) {
   if (guard(h)) return -4;
   if (!h->factored) return -3;
   return wrap([&]() -> int64_t { return download_impl<float>(h, lnz, unz, ipvt); });
+}
+
+int64_t spkhip_set_segments(spkhip_t h, int64_t nseg, const int64_t* offsets) {
+  if (guard(h) || nseg < 1 || !offsets) return -4;
+  if (!h->have_sym || offsets[0] != 0 || offsets[nseg] != h->n) return -4;
+  for (int64_t i = 0; i < nseg; ++i)
+    if (offsets[i + 1] < offsets[i]) return -4;
+  h->seg.assign(offsets, offsets + nseg + 1);
+  return 0;
+}
+
+int64_t spkhip_profile_enable(spkhip_t h, int64_t on) {
+  if (guard(h)) return -4;
+  return wrap([&]() -> int64_t {
+    prof_collect(h);
+    h->prof = on != 0;
+    for (auto& k : h->kstat) k = spkhip_kstat_t{};
+    return 0;
+  });
+}
+
+int64_t spkhip_profile_get(spkhip_t h, int64_t kclass, spkhip_kstat_t* out) {
+  if (guard(h) || !out || kclass < 0 || kclass >= SPKHIP_KCLASS_COUNT) return -4;
+  *out = h->kstat[kclass];
+  return 0;
 }
 
 int64_t spkhip_stats(spkhip_t h, spkhip_stats_t* s) {

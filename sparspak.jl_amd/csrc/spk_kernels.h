@@ -31,6 +31,7 @@ constexpr int kMaxPiece = 64; // widest piece of a small (LDS) front; the
                               // (the reference's split leaves a last piece
                               // of up to 1.5x its 30 / 60 limit)
 constexpr int kUcols = 32;    // U12 columns per panel workgroup
+constexpr int kSmallLmax = 64; // fronts with L <= this are factored in LDS
 
 // Device view of the symbolic factor and the front schedule.
 struct DevSym {
@@ -56,18 +57,33 @@ struct DevSym {
   const int32_t* fchp;    // children CSR pointer (nfront+1)
   const int32_t* fch;     // children
   const int64_t* fvec;    // offset in the solve vector arena (L per rhs)
+  const int64_t* lcol;    // per column g: lnz offset of front row 0, i.e.
+                          // L(front row r, g) = lnz[lcol[g] + r]
+  const int64_t* ucol;    // per row g: U(g, front column j) = unz[ucol[g] + j]
   int pivot;              // 1: pivot inside diagonal blocks (LU); 0: none
   int sym;                // 1: U entries are read from the transposed lnz slot
                           //    (SPD input: lower triangle only)
 };
 
-// Per-step descriptor of the large-front kernels.
-struct StepDesc {
+// Per-launch descriptor of the blocked large-front kernels (one per front
+// taking part in the launch).
+struct BlkDesc {
   int32_t front;
-  int32_t piece;     // 0-based supernode
-  int32_t wg_begin;  // prefix sum of workgroups (panel) / tiles (update)
-  int32_t nl;        // panel: L21 row blocks; update: tiles per dimension
-  int32_t slot;      // diagonal-block staging slot
+  int32_t piece;     // 0-based supernode (panel) / first piece of the block
+  int32_t np;        // pieces in the block
+  int32_t c0, c1;    // block columns [c0, c1), front-local
+  int32_t wg_begin;  // prefix sum of workgroups over the launch
+  int32_t nl;        // gemm: tiles per dimension
+  int32_t slot;      // stage slot of the (first) piece
+};
+
+// Per-workgroup descriptor of the blocked solve kernels.
+struct SolveDesc {
+  int32_t front;
+  int32_t piece;   // first piece of the block
+  int32_t np;      // pieces in the block
+  int32_t c0, c1;  // block columns [c0, c1), front-local
+  int32_t r0;      // gemv: first row of the workgroup's tile (relative)
 };
 
 }  // namespace spk

@@ -321,7 +321,7 @@ large_writeback_kernel(DevSym s, const int32_t* __restrict__ it_front,
   }
 }
 
-__device__ inline int find_desc(const StepDesc* d, int nd, int b) {
+__device__ inline int find_blk(const BlkDesc* d, int nd, int b) {
   int lo = 0, hi = nd - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
@@ -333,74 +333,204 @@ __device__ inline int find_desc(const StepDesc* d, int nd, int b) {
   return lo;
 }
 
-// Diagonal block of piece `piece` of a large front, factored redundantly by
-// every workgroup of the panel step (nq <= MAXW). D is nq x nq in LDS with
-// leading dimension MAXW.
-template <typename T, int MAXW>
-__device__ void lds_getrf_block(T* D, int nq, int pivot, int* piv, int* zero) {
-  __shared__ int s_kp;
-  __shared__ T s_inv;
-  const int tid = threadIdx.x;
-  const int ld = MAXW;
+// ---------------------------------------------------------------------------
+// Large fronts, blocked. The pieces of a front are taken in groups (blocks)
+// of at most GW columns [c0, c1). Within a block each piece p (columns
+// [cq, cq+nq)) is eliminated by blk_panel_kernel exactly as the reference
+// does for a supernode (SpkLUFactor.jl:230-240): LU with row interchanges
+// inside the nq x nq diagonal block, L21 = A21 U11^-1 for every row below,
+// and the interchange + unit-lower solve for the U12 part — here only for
+// the columns of the same block, whose rows below are updated at once.
+// blk_urows_kernel then forms the U rows of the whole block for the columns
+// right of it (per piece: the earlier pieces' contribution, the piece's own
+// interchanges, the unit-lower solve), and blk_gemm_kernel applies the
+// trailing update A22 -= L21 U12 with inner dimension c1 - c0 instead of one
+// piece width. Interchanges never touch earlier pieces' columns, as in the
+// reference. The factored diagonal block and in-block U12 of every piece are
+// kept in a staging slot until blk_gemm_kernel commits them, so that no
+// kernel reads a block another workgroup of the same launch is rewriting.
+// ---------------------------------------------------------------------------
+
+// Stage slot of one piece: D (nq x nq, leading dimension nq), at offset
+// MAXW*MAXW the in-block U12 (nq x ucols, leading dimension nq), at offset
+// MAXW*(MAXW+GW) the row permutation of the block (perm[i] = original row
+// now at position i, stored as values).
+template <int MAXW, int GW>
+__host__ __device__ constexpr int stage_piece_elems() { return MAXW * MAXW + MAXW * GW + MAXW; }
+
+// Value of v in lane l (l wave-uniform).
+__device__ inline double rdlane(double v, int l) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffull), l);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                          (unsigned long long)(unsigned)lo));
+}
+__device__ inline float rdlane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// LU with row pivoting of the nq x nq block D (LDS, leading dimension ld,
+// nq <= NC <= 64) by wave 0, each lane holding one row in registers. Rows do
+// not move: every lane tracks the position its row would occupy after
+// LAPACK's interchanges, so the pivot search (first maximum |a| over
+// positions >= k, strict >) and the final layout are those of ggetrf!
+// (GenericBlasLapackFragments.jl:56-100); a zero pivot skips the interchange
+// and the scaling. On return D holds the packed LU by position, piv[k] the
+// interchange of step k, perm[p] the original row at position p, dinv[k] =
+// 1/U(k,k).
+template <typename T, int NC>
+__device__ void reg_getrf(T* D, int ld, int nq, int pivot, int* piv, int* perm, T* dinv,
+                          int* zero) {
+  const int lane = threadIdx.x % 64;
+  const bool mine = lane < nq;
+  T d[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) d[j] = (mine && j < nq) ? D[lane + j * ld] : T(0);
+  int pos = lane;
+  // runtime k: register elements are addressed by compile-time j only
   for (int k = 0; k < nq; ++k) {
-    if (tid == 0) {
-      int kp = k;
-      if (pivot) {
-        T amax = fabs(D[k + k * ld]);
-        for (int i = k + 1; i < nq; ++i) {
-          T a = fabs(D[i + k * ld]);
-          if (a > amax) {
-            amax = a;
-            kp = i;
-          }
+    T dk = T(0);
+#pragma unroll
+    for (int j = 0; j < NC; ++j)
+      if (j == k) dk = d[j];
+    int kp = k;
+    if (pivot) {
+      const bool cand = mine && pos >= k;
+      T best = cand ? fabs(dk) : T(-1);
+      int bpos = cand ? pos : (1 << 30);
+      for (int off = 32; off > 0; off >>= 1) {
+        const T ob = __shfl_xor(best, off, 64);
+        const int op = __shfl_xor(bpos, off, 64);
+        if (ob > best || (ob == best && op < bpos)) {
+          best = ob;
+          bpos = op;
         }
       }
-      s_kp = kp;
-      T pv = D[kp + k * ld];
-      if (pv == T(0)) *zero = 1;
-      s_inv = T(1) / pv;
-      piv[k] = kp;
+      kp = __builtin_amdgcn_readfirstlane(bpos);
     }
-    __syncthreads();
-    const int kp = s_kp;
-    const bool nonzero = D[kp + k * ld] != T(0);
+    const unsigned long long mp = __ballot(mine && pos == kp);
+    const unsigned long long mk = __ballot(mine && pos == k);
+    const int blane = __builtin_amdgcn_readfirstlane(__ffsll((long long)mp) - 1);
+    const int klane = __builtin_amdgcn_readfirstlane(__ffsll((long long)mk) - 1);
+    const T pv = rdlane(dk, blane);
+    const bool nonzero = pv != T(0);
+    if (nonzero && kp != k) {
+      if (lane == klane)
+        pos = kp;
+      else if (lane == blane)
+        pos = k;
+    }
+    const T inv = T(1) / pv;
+    if (lane == 0) {
+      piv[k] = kp;
+      dinv[k] = inv;
+      if (!nonzero) *zero = 1;
+    }
+    const bool below = mine && pos > k;
+    T l = dk;
+    if (below && nonzero) l *= inv;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      if (j == k) {
+        if (below) d[j] = l;
+      } else if (j > k && j < nq) {
+        const T uj = rdlane(d[j], blane);
+        if (below) d[j] -= l * uj;
+      }
+    }
+  }
+  if (mine) {
+#pragma unroll
+    for (int j = 0; j < NC; ++j)
+      if (j < nq) D[pos + j * ld] = d[j];
+    perm[pos] = lane;
+  }
+}
+
+// The same factorisation through LDS for pieces wider than a wave (SPD
+// blocks of up to 96 columns): lane = row, rows i and i + 64.
+template <typename T>
+__device__ void wave_getrf(T* D, int ld, int nq, int pivot, int* piv, int* perm, T* dinv,
+                           int* zero) {
+  const int lane = threadIdx.x % 64;
+  for (int i = lane; i < nq; i += 64) perm[i] = i;
+  __builtin_amdgcn_wave_barrier();
+  for (int k = 0; k < nq; ++k) {
+    int kp = k;
+    if (pivot) {
+      T best = T(-1);
+      int bi = nq;
+      for (int i = k + lane; i < nq; i += 64) {
+        T a = fabs(D[i + k * ld]);
+        if (a > best) {
+          best = a;
+          bi = i;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        T ob = __shfl_xor(best, off, 64);
+        int oi = __shfl_xor(bi, off, 64);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      kp = bi;
+    }
+    const T pv = D[kp + k * ld];
+    const bool nonzero = pv != T(0);
+    if (lane == 0) {
+      piv[k] = kp;
+      if (!nonzero) *zero = 1;
+      if (kp != k && nonzero) {
+        int t = perm[k];
+        perm[k] = perm[kp];
+        perm[kp] = t;
+      }
+    }
     if (kp != k && nonzero)
-      for (int c = tid; c < nq; c += kWG) {
+      for (int c = lane; c < nq; c += 64) {
         T t = D[k + c * ld];
         D[k + c * ld] = D[kp + c * ld];
         D[kp + c * ld] = t;
       }
-    __syncthreads();
-    if (nonzero)
-      for (int r = k + 1 + tid; r < nq; r += kWG) D[r + k * ld] *= s_inv;
-    __syncthreads();
-    const int m = nq - k - 1;
-    for (int idx = tid; idx < m * m; idx += kWG) {
-      int r = k + 1 + idx % m, c = k + 1 + idx / m;
-      D[r + c * ld] -= D[r + k * ld] * D[k + c * ld];
+    __builtin_amdgcn_wave_barrier();
+    const T inv = T(1) / D[k + k * ld];
+    if (lane == 0) dinv[k] = inv;
+    for (int i = k + 1 + lane; i < nq; i += 64) {
+      T l = D[i + k * ld];
+      if (nonzero) {
+        l *= inv;
+        D[i + k * ld] = l;
+      }
+      for (int j = k + 1; j < nq; ++j) D[i + j * ld] -= l * D[k + j * ld];
     }
-    __syncthreads();
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
-// Panel step of piece `piece` for every large front at this step: factor the
-// diagonal block (each workgroup redundantly, in LDS), then L21 = A21 U11^-1
-// (row blocks) and U12 = L11^-1 P A12 (column blocks). The factored block
-// goes to a staging slot and is copied into the front by the update step,
-// so that no workgroup here reads a half-written block.
-template <typename T, int MAXW>
+// One piece of a block: see the section comment. Workgroup b == 0 also
+// publishes the factored diagonal block, in-block U12 and permutation to the
+// stage and the pivots to ipvt; every workgroup handles 64 rows below the
+// block. Dynamic LDS: D[MAXW*MAXW], U[(MAXW+1)*ucmax], X[64*MAXW].
+template <typename T, int MAXW, int GW, int NC>
 __global__ void __launch_bounds__(kWG)
-large_panel_kernel(DevSym s, const StepDesc* __restrict__ desc, int ndesc,
-                   T* __restrict__ arena, T* __restrict__ dstage,
-                   int64_t* __restrict__ ipvt,
-                   unsigned long long* __restrict__ first_fail) {
-  __shared__ T D[MAXW * MAXW];
-  __shared__ T X[MAXW * kUcols];
+blk_panel_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
+                 T* __restrict__ arena, T* __restrict__ stage,
+                 int64_t* __restrict__ ipvt,
+                 unsigned long long* __restrict__ first_fail) {
+  constexpr int UL = MAXW + 1;  // odd leading dimension: no bank conflicts
+  extern __shared__ unsigned char smem_raw[];
   __shared__ int piv[MAXW];
+  __shared__ int perm[MAXW];
+  __shared__ T dinv[MAXW];
   __shared__ int s_zero;
-  const int tid = threadIdx.x;
-  const int di = find_desc(desc, ndesc, blockIdx.x);
-  const StepDesc d = desc[di];
+  T* D = reinterpret_cast<T*>(smem_raw);
+  T* U = D + MAXW * MAXW;
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int di = find_blk(desc, ndesc, blockIdx.x);
+  const BlkDesc d = desc[di];
   const int b = blockIdx.x - d.wg_begin;
   const int f = d.front;
   const int L = s.fL[f];
@@ -409,265 +539,669 @@ large_panel_kernel(DevSym s, const StepDesc* __restrict__ desc, int ndesc,
   piece_cols(s, d.piece, pc0, nq64);
   const int nq = (int)nq64;
   const int cq = (int)(pc0 - fc0);
-  const int m = L - cq - nq;  // rows below / columns right of the block
+  const int uc = d.c1 - cq - nq;   // in-block columns right of the piece
+  const int m = L - cq - nq;       // rows below the diagonal block
   T* F = arena + s.foff[f];
+  T* X = U + UL * max(uc, 0);
   if (tid == 0) s_zero = 0;
   for (int idx = tid; idx < nq * nq; idx += kWG) {
     int r = idx % nq, c = idx / nq;
     D[r + c * MAXW] = F[(cq + r) + (int64_t)(cq + c) * L];
   }
+  for (int idx = tid; idx < nq * uc; idx += kWG) {
+    int r = idx % nq, c = idx / nq;
+    U[r + c * UL] = F[(cq + r) + (int64_t)(cq + nq + c) * L];
+  }
   __syncthreads();
-  lds_getrf_block<T, MAXW>(D, nq, s.pivot, piv, &s_zero);
+  if (wave == 0) {
+    if constexpr (NC > 0)
+      reg_getrf<T, NC>(D, MAXW, nq, s.pivot, piv, perm, dinv, &s_zero);
+    else
+      wave_getrf<T>(D, MAXW, nq, s.pivot, piv, perm, dinv, &s_zero);
+  }
+  __syncthreads();
+  // in-block U12 = L11^-1 P A12: one column per thread
+  if constexpr (NC > 0) {
+    for (int c = tid; c < uc; c += kWG) {
+      T u[NC];
+#pragma unroll
+      for (int i = 0; i < NC; ++i) u[i] = (i < nq) ? U[perm[i] + c * UL] : T(0);
+#pragma unroll
+      for (int k = 0; k < NC; ++k)
+#pragma unroll
+        for (int i = k + 1; i < NC; ++i)
+          if (i < nq) u[i] -= D[i + k * MAXW] * u[k];
+#pragma unroll
+      for (int i = 0; i < NC; ++i)
+        if (i < nq) U[i + c * UL] = u[i];
+    }
+  } else {
+    for (int c = tid; c < uc; c += kWG) {
+      T* u = U + c * UL;
+      for (int k = 0; k < nq; ++k) {
+        int kp = piv[k];
+        if (kp != k) {
+          T t = u[k];
+          u[k] = u[kp];
+          u[kp] = t;
+        }
+      }
+      for (int k = 0; k < nq; ++k) {
+        T uk = u[k];
+        for (int i = k + 1; i < nq; ++i) u[i] -= D[i + k * MAXW] * uk;
+      }
+    }
+  }
+  __syncthreads();
   if (b == 0) {
-    T* st = dstage + (int64_t)d.slot * MAXW * MAXW;
+    T* st = stage + (int64_t)d.slot * stage_piece_elems<MAXW, GW>();
     for (int idx = tid; idx < nq * nq; idx += kWG) {
       int r = idx % nq, c = idx / nq;
       st[r + c * nq] = D[r + c * MAXW];
     }
-    for (int k = tid; k < nq; k += kWG) ipvt[pc0 + k] = piv[k] + 1;
+    for (int idx = tid; idx < nq * uc; idx += kWG) {
+      int r = idx % nq, c = idx / nq;
+      st[MAXW * MAXW + r + c * nq] = U[r + c * UL];
+    }
+    for (int k = tid; k < nq; k += kWG) {
+      st[MAXW * (MAXW + GW) + k] = (T)perm[k];
+      ipvt[pc0 + k] = piv[k] + 1;
+    }
     if (tid == 0 && s_zero) atomicMin(first_fail, (unsigned long long)(d.piece + 1));
   }
   if (m <= 0) return;
-  const int nl = d.nl;
-  if (b < nl) {
-    // L21 rows: x U11 = a, column by column (gtrsm! 'r','u','n','n' order);
-    // the solved entries are read back from the row itself.
-    const int r = cq + nq + b * kWG + tid;
-    if (r < L) {
-      T* row = F + r + (int64_t)cq * L;
-      for (int j = 0; j < nq; ++j) {
-        T v = row[(int64_t)j * L];
-        for (int k = 0; k < j; ++k) v -= D[k + j * MAXW] * row[(int64_t)k * L];
-        row[(int64_t)j * L] = v * (T(1) / D[j + j * MAXW]);
-      }
-    }
-  } else {
-    // U12 columns: apply the block's interchanges, then unit-lower solve.
-    const int c0 = cq + nq + (b - nl) * kUcols;
-    const int nc = min(kUcols, L - c0);
-    for (int idx = tid; idx < nq * nc; idx += kWG) {
-      int r = idx % nq, c = idx / nq;
-      X[r + c * MAXW] = F[(cq + r) + (int64_t)(c0 + c) * L];
-    }
-    __syncthreads();
-    if (tid < nc) {
-      T* xc = X + tid * MAXW;
-      for (int k = 0; k < nq; ++k) {
-        int kp = piv[k];
-        if (kp != k) {
-          T t = xc[k];
-          xc[k] = xc[kp];
-          xc[kp] = t;
+  // 64 rows per workgroup: wave 0 forms the L21 rows (gtrsm! 'r','u','n','n'
+  // order), all waves then apply the in-block update, 8 columns at a time
+  // so that eight loads are in flight per lane
+  const int r0 = cq + nq + b * 64;
+  const int nr = min(64, L - r0);
+  if constexpr (NC > 0) {
+    if (wave == 0) {
+      T x[NC];
+      const bool act = lane < nr;
+      T* row = F + (r0 + lane) + (int64_t)cq * L;
+#pragma unroll
+      for (int j = 0; j < NC; ++j) x[j] = (act && j < nq) ? row[(int64_t)j * L] : T(0);
+#pragma unroll
+      for (int j = 0; j < NC; ++j) {
+        if (j < nq) {
+          T v = x[j];
+#pragma unroll
+          for (int kk = 0; kk < j; ++kk) v -= D[kk + j * MAXW] * x[kk];
+          x[j] = v * dinv[j];
         }
       }
-      for (int k = 0; k < nq; ++k) {
-        T xk = xc[k];
-        for (int i = k + 1; i < nq; ++i) xc[i] -= D[i + k * MAXW] * xk;
+#pragma unroll
+      for (int j = 0; j < NC; ++j)
+        if (j < nq) {
+          X[lane + j * 64] = x[j];
+          if (act) row[(int64_t)j * L] = x[j];
+        }
+    }
+  } else {
+    for (int idx = tid; idx < 64 * nq; idx += kWG) {
+      int i = idx % 64, kk = idx / 64;
+      X[i + kk * 64] = (i < nr) ? F[(r0 + i) + (int64_t)(cq + kk) * L] : T(0);
+    }
+    __syncthreads();
+    if (wave == 0) {
+      T* xr = X + lane;
+      for (int j = 0; j < nq; ++j) {
+        T v = xr[j * 64];
+        for (int kk = 0; kk < j; ++kk) v -= D[kk + j * MAXW] * xr[kk * 64];
+        xr[j * 64] = v * dinv[j];
       }
     }
     __syncthreads();
-    for (int idx = tid; idx < nq * nc; idx += kWG) {
-      int r = idx % nq, c = idx / nq;
-      F[(cq + r) + (int64_t)(c0 + c) * L] = X[r + c * MAXW];
+    for (int idx = tid; idx < 64 * nq; idx += kWG) {
+      int i = idx % 64, kk = idx / 64;
+      if (i < nr) F[(r0 + i) + (int64_t)(cq + kk) * L] = X[i + kk * 64];
+    }
+  }
+  __syncthreads();
+  if (lane < nr) {
+    T* urow = F + (r0 + lane) + (int64_t)(cq + nq) * L;
+    for (int cb = wave * 8; cb < uc; cb += 32) {
+      T old[8], acc[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        old[j] = (cb + j < uc) ? urow[(int64_t)(cb + j) * L] : T(0);
+        acc[j] = T(0);
+      }
+      for (int kk = 0; kk < nq; ++kk) {
+        const T xv = X[lane + kk * 64];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[j] += xv * U[kk + min(cb + j, uc - 1) * UL];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (cb + j < uc) urow[(int64_t)(cb + j) * L] = old[j] - acc[j];
     }
   }
 }
 
-// Trailing update A22 -= L21 U12 on kTile x kTile tiles, plus one extra
-// workgroup per front that copies the staged diagonal block into place.
-template <typename T, int MAXW>
+// U rows of the block [c0, c1) for 32 columns right of it, one workgroup per
+// (front, 32 columns). Dynamic LDS: A[(GW+1)*32], Dl[MAXW*MAXW].
+template <typename T, int MAXW, int GW, int NC>
 __global__ void __launch_bounds__(kWG)
-large_update_kernel(DevSym s, const StepDesc* __restrict__ desc, int ndesc,
-                    T* __restrict__ arena, const T* __restrict__ dstage) {
-  constexpr int KC = 32;  // k chunk staged in LDS
-  __shared__ T As[KC][kTile];
-  __shared__ T Bs[KC][kTile];
+blk_urows_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
+                 T* __restrict__ arena, const T* __restrict__ stage,
+                 const int64_t* __restrict__ ipvt) {
+  constexpr int AL = GW + 1;  // odd leading dimension: no bank conflicts
+  extern __shared__ unsigned char smem_raw[];
+  __shared__ int sperm[MAXW];
+  T* A = reinterpret_cast<T*>(smem_raw);  // A[r + c*AL], r = row - c0
+  T* Dl = A + AL * 32;
   const int tid = threadIdx.x;
-  const int di = find_desc(desc, ndesc, blockIdx.x);
-  const StepDesc d = desc[di];
+  const int di = find_blk(desc, ndesc, blockIdx.x);
+  const BlkDesc d = desc[di];
   const int b = blockIdx.x - d.wg_begin;
   const int f = d.front;
   const int L = s.fL[f];
   const int64_t fc0 = s.fc0[f];
-  int64_t pc0, nq64;
-  piece_cols(s, d.piece, pc0, nq64);
-  const int nq = (int)nq64;
-  const int cq = (int)(pc0 - fc0);
+  const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
+  const int j0 = c1 + b * 32;
+  const int nc = min(32, L - j0);
   T* F = arena + s.foff[f];
-  const int nt = d.nl;  // tiles per dimension
-  if (b == nt * nt) {
-    const T* st = dstage + (int64_t)d.slot * MAXW * MAXW;
+  for (int idx = tid; idx < bw * nc; idx += kWG) {
+    int r = idx % bw, c = idx / bw;
+    A[r + c * AL] = F[(c0 + r) + (int64_t)(j0 + c) * L];
+  }
+  __syncthreads();
+  for (int pi = 0; pi < d.np; ++pi) {
+    int64_t pc0, nq64;
+    piece_cols(s, d.piece + pi, pc0, nq64);
+    const int nq = (int)nq64;
+    const int cq = (int)(pc0 - fc0);
+    const int rq = cq - c0;  // piece rows inside A
+    // earlier pieces' contribution: A[p rows] -= L[p rows, c0:cq] U[c0:cq]
+    if (rq > 0) {
+      const int c = tid % 32, i0 = tid / 32;
+      if (c < nc)
+        for (int i = i0; i < nq; i += kWG / 32) {
+          const T* lrow = F + (cq + i) + (int64_t)c0 * L;
+          T acc = T(0);
+          int k = 0;
+          for (; k + 8 <= rq; k += 8) {
+            T lv[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) lv[j] = lrow[(int64_t)(k + j) * L];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc += lv[j] * A[k + j + c * AL];
+          }
+          for (; k < rq; ++k) acc += lrow[(int64_t)k * L] * A[k + c * AL];
+          A[rq + i + c * AL] -= acc;
+        }
+    }
+    const T* st = stage + (int64_t)(d.slot + pi) * stage_piece_elems<MAXW, GW>();
+    for (int idx = tid; idx < nq * nq; idx += kWG) Dl[idx] = st[idx];
+    for (int i = tid; i < nq; i += kWG) sperm[i] = (int)st[MAXW * (MAXW + GW) + i];
+    __syncthreads();
+    if (tid < nc) {
+      if constexpr (NC > 0) {
+        T a[NC];
+        T* col = A + tid * AL + rq;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) a[i] = (i < nq) ? col[sperm[i]] : T(0);
+#pragma unroll
+        for (int k = 0; k < NC; ++k)
+#pragma unroll
+          for (int i = k + 1; i < NC; ++i)
+            if (i < nq) a[i] -= Dl[i + k * nq] * a[k];
+#pragma unroll
+        for (int i = 0; i < NC; ++i)
+          if (i < nq) col[i] = a[i];
+      } else {
+        T* a = A + tid * AL + rq;
+        for (int k = 0; k < nq; ++k) {
+          int kp = (int)ipvt[pc0 + k] - 1;
+          if (kp != k) {
+            T t = a[k];
+            a[k] = a[kp];
+            a[kp] = t;
+          }
+        }
+        for (int k = 0; k < nq; ++k) {
+          T ak = a[k];
+          for (int i = k + 1; i < nq; ++i) a[i] -= Dl[i + k * nq] * ak;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < bw * nc; idx += kWG) {
+    int r = idx % bw, c = idx / bw;
+    F[(c0 + r) + (int64_t)(j0 + c) * L] = A[r + c * AL];
+  }
+}
+
+// Trailing update A[c1:, c1:] -= L[c1:, c0:c1] U[c0:c1, c1:] on 128 x 128
+// tiles (8 x 8 per thread, k staged 16 at a time in LDS); the last d.np
+// workgroups of a front commit the staged diagonal blocks / in-block U12.
+template <typename T, int MAXW, int GW, int NC>
+__global__ void __launch_bounds__(kWG)
+blk_gemm_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
+                T* __restrict__ arena, const T* __restrict__ stage) {
+  constexpr int TM = 128, KC = 16;
+  __shared__ T As[KC][TM];
+  __shared__ T Bs[KC][TM];
+  const int tid = threadIdx.x;
+  const int di = find_blk(desc, ndesc, blockIdx.x);
+  const BlkDesc d = desc[di];
+  const int b = blockIdx.x - d.wg_begin;
+  const int f = d.front;
+  const int L = s.fL[f];
+  const int64_t fc0 = s.fc0[f];
+  const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
+  T* F = arena + s.foff[f];
+  const int nt = d.nl;
+  if (b >= nt * nt) {  // commit one piece's staged blocks
+    const int pi = b - nt * nt;
+    int64_t pc0, nq64;
+    piece_cols(s, d.piece + pi, pc0, nq64);
+    const int nq = (int)nq64;
+    const int cq = (int)(pc0 - fc0);
+    const int uc = c1 - cq - nq;
+    const T* st = stage + (int64_t)(d.slot + pi) * stage_piece_elems<MAXW, GW>();
     for (int idx = tid; idx < nq * nq; idx += kWG) {
       int r = idx % nq, c = idx / nq;
       F[(cq + r) + (int64_t)(cq + c) * L] = st[idx];
     }
+    for (int idx = tid; idx < nq * uc; idx += kWG) {
+      int r = idx % nq, c = idx / nq;
+      F[(cq + r) + (int64_t)(cq + nq + c) * L] = st[MAXW * MAXW + idx];
+    }
     return;
   }
-  const int base = cq + nq;
   const int ti = b % nt, tj = b / nt;
-  const int r0 = base + ti * kTile, c0 = base + tj * kTile;
-  const int ri = (tid % 16) * 4, cj = (tid / 16) * 4;
-  T acc[4][4];
+  const int r0 = c1 + ti * TM, j0 = c1 + tj * TM;
+  const int tx = tid % 16, ty = tid / 16;
+  T acc[8][8];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 8; ++a)
 #pragma unroll
-    for (int bb = 0; bb < 4; ++bb) acc[a][bb] = T(0);
-  for (int k0 = 0; k0 < nq; k0 += KC) {
-    const int kc = min(KC, nq - k0);
-    for (int idx = tid; idx < kc * kTile; idx += kWG) {
-      int i = idx % kTile, k = idx / kTile;
+    for (int c = 0; c < 8; ++c) acc[a][c] = T(0);
+  for (int k0 = 0; k0 < bw; k0 += KC) {
+    const int kc = min(KC, bw - k0);
+    for (int idx = tid; idx < KC * TM; idx += kWG) {
+      int i = idx % TM, k = idx / TM;
       int r = r0 + i;
-      As[k][i] = (r < L) ? F[r + (int64_t)(cq + k0 + k) * L] : T(0);
+      As[k][i] = (k < kc && r < L) ? F[r + (int64_t)(c0 + k0 + k) * L] : T(0);
     }
-    for (int idx = tid; idx < kc * kTile; idx += kWG) {
-      int k = idx % kc, j = idx / kc;
-      int c = c0 + j;
-      Bs[k][j] = (c < L) ? F[(cq + k0 + k) + (int64_t)c * L] : T(0);
+    for (int idx = tid; idx < KC * TM; idx += kWG) {
+      int k = idx % KC, j = idx / KC;
+      int c = j0 + j;
+      Bs[k][j] = (k < kc && c < L) ? F[(c0 + k0 + k) + (int64_t)c * L] : T(0);
     }
     __syncthreads();
-    for (int k = 0; k < kc; ++k) {
-      T av[4], bv[4];
+#pragma unroll 4
+    for (int k = 0; k < KC; ++k) {
+      T av[8], bv[8];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) av[a] = As[k][ri + a];
+      for (int a = 0; a < 8; ++a) av[a] = As[k][tx * 8 + a];
 #pragma unroll
-      for (int bb = 0; bb < 4; ++bb) bv[bb] = Bs[k][cj + bb];
+      for (int c = 0; c < 8; ++c) bv[c] = Bs[k][ty * 8 + c];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < 8; ++a)
 #pragma unroll
-        for (int bb = 0; bb < 4; ++bb) acc[a][bb] += av[a] * bv[bb];
+        for (int c = 0; c < 8; ++c) acc[a][c] += av[a] * bv[c];
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int bb = 0; bb < 4; ++bb) {
-    int c = c0 + cj + bb;
-    if (c >= L) continue;
+  for (int c = 0; c < 8; ++c) {
+    const int col = j0 + ty * 8 + c;
+    if (col >= L) continue;
+    T* fc = F + (int64_t)col * L;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      int r = r0 + ri + a;
-      if (r < L) F[r + (int64_t)c * L] -= acc[a][bb];
+    for (int a = 0; a < 8; ++a) {
+      const int r = r0 + tx * 8 + a;
+      if (r < L) fc[r] -= acc[a][c];
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Triangular solves, one workgroup per front, one launch per level.
-// Forward (_lulsolve!, SpkLUFactor.jl:269-323): the front's vector y (length
-// L, per right-hand side) starts as the permuted rhs on its own columns and
-// zero elsewhere, receives the children's update vectors through the
-// relative-index maps (the multifrontal form of the reference's scatter
-// rhs[lindx] += temp), and is solved piece by piece: interchange, unit-lower
-// block solve, y[below] -= L21 y[block]. Backward (_luusolve!, :325-377):
-// gather the solution on the front's index list, then per piece in reverse
-// order y[block] -= U12 y[right] and the upper block solve.
+// Triangular solves, one elimination-tree level at a time.
+// Forward (_lulsolve!, SpkLUFactor.jl:269-323). A front's vector y (length
+// L per right-hand side, in the vector arena) starts as the permuted rhs on
+// its W own columns and zero on its R update rows, and receives the
+// children's update vectors through the relative-index maps (the
+// multifrontal form of the reference's scatter rhs[lindx] += temp). Then,
+// block by block (the factor's blocks of pieces): a one-workgroup chain
+// solves the block's pieces in LDS (interchanges, unit-lower block solve,
+// rows below inside the block) and a many-workgroup GEMV updates every row
+// below the block, y[c1:L] -= L[c1:L, c0:c1] y[c0:c1].
+// Backward (_luusolve!, :325-377), blocks in reverse: t = U[c0:c1, c1:L]
+// x[I[c1:L]] (one wave per row), then the chain: y = x - t, per piece in
+// reverse the in-block U12 part and the upper block solve.
+// Small fronts (L <= kSmallLmax) do everything in one workgroup.
+// The serial block solves run on wave 0 only: LDS operations of one wave
+// complete in issue order, so no barrier is needed between their steps.
 // ---------------------------------------------------------------------------
+template <typename T, int MAXW>
+__device__ void stage_block(T* D, const T* lp, int nq, int jl) {
+  for (int idx = threadIdx.x; idx < nq * nq; idx += kWG) {
+    int i = idx % nq, k = idx / nq;
+    D[i + k * MAXW] = lp[i + (int64_t)k * jl];
+  }
+}
+
+// sum_k L[r, pc0 + k] * y[k] for k < nq: the column bases lcol[pc0 + k] are
+// wave-uniform; loads are issued eight at a time.
 template <typename T>
+__device__ inline T dot_lrow(const DevSym& s, const T* __restrict__ lnz, int64_t pc0, int nq,
+                             int r, const T* y) {
+  T acc = T(0);
+  int k = 0;
+  for (; k + 8 <= nq; k += 8) {
+    T lv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) lv[j] = lnz[s.lcol[pc0 + k + j] + r];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc += lv[j] * y[k + j];
+  }
+  for (; k < nq; ++k) acc += lnz[s.lcol[pc0 + k] + r] * y[k];
+  return acc;
+}
+
+// Children's update vectors into y (fixed child order).
+template <typename T>
+__device__ void gather_children(const DevSym& s, int f, int q, int nrhs, T* yg, T* vec) {
+  for (int ci = s.fchp[f]; ci < s.fchp[f + 1]; ++ci) {
+    const int c = s.fch[ci];
+    const int Lc = s.fL[c], Wc = s.fW[c];
+    const int32_t* rel = s.relmap + s.frel[c];
+    const T* uc = vec + s.fvec[c] * nrhs + (int64_t)q * Lc + Wc;
+    for (int t = threadIdx.x; t < Lc - Wc; t += kWG) yg[rel[t]] += uc[t];
+    __syncthreads();
+  }
+}
+
+// Unit-lower solve of one piece's diagonal block on yb (LDS) by wave 0,
+// after the piece's interchanges (glaswp! then gtrsm! 'l','l','n','u').
+template <typename T, int MAXW>
+__device__ void wave_lower_solve(T* yb, const T* D, const int* piv, int nq) {
+  const int lane = threadIdx.x % 64;
+  if (lane == 0)
+    for (int k = 0; k < nq; ++k) {
+      int ip = piv[k];
+      if (ip != k) {
+        T t = yb[k];
+        yb[k] = yb[ip];
+        yb[ip] = t;
+      }
+    }
+  __builtin_amdgcn_wave_barrier();
+  for (int k = 0; k < nq; ++k) {
+    const T yk = yb[k];
+    if (yk != T(0))
+      for (int i = k + 1 + lane; i < nq; i += 64) yb[i] -= yk * D[i + k * MAXW];
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Upper solve (gtrsm! 'l','u','n','n') of one piece's block by wave 0.
+template <typename T, int MAXW>
+__device__ void wave_upper_solve(T* yb, const T* D, int nq) {
+  const int lane = threadIdx.x % 64;
+  for (int k = nq - 1; k >= 0; --k) {
+    T yk = yb[k];
+    if (yk != T(0)) {
+      yk /= D[k + k * MAXW];
+      __builtin_amdgcn_wave_barrier();
+      if (lane == 0) yb[k] = yk;
+      for (int i = lane; i < k; i += 64) yb[i] -= yk * D[i + k * MAXW];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Small fronts, forward: everything in one workgroup. Dynamic LDS: yW[L].
+template <typename T, int MAXW>
 __global__ void __launch_bounds__(kWG)
-fwd_solve_kernel(DevSym s, const int32_t* __restrict__ list,
-                 const T* __restrict__ lnz, const int64_t* __restrict__ ipvt,
-                 T* __restrict__ rhs, int64_t ldr, int nrhs,
+fwd_small_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict__ lnz,
+                 const int64_t* __restrict__ ipvt, T* __restrict__ rhs, int64_t ldr, int nrhs,
                  T* __restrict__ vec) {
-  const int tid = threadIdx.x;
+  extern __shared__ unsigned char smem_raw[];
+  T* yW = reinterpret_cast<T*>(smem_raw);
+  __shared__ T D[MAXW * MAXW];
+  __shared__ int spiv[MAXW];
+  const int tid = threadIdx.x, wave = tid / 64;
   const int f = list[blockIdx.x];
   const int L = s.fL[f], W = s.fW[f];
   const int64_t fc0 = s.fc0[f];
   const int sp0 = s.fsp0[f], np = s.fnp[f];
   for (int q = 0; q < nrhs; ++q) {
-    T* y = vec + s.fvec[f] * nrhs + (int64_t)q * L;
+    T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
     T* x = rhs + (int64_t)q * ldr;
-    for (int r = tid; r < L; r += kWG) y[r] = (r < W) ? x[fc0 + r] : T(0);
+    for (int r = tid; r < L; r += kWG) yg[r] = (r < W) ? x[fc0 + r] : T(0);
     __syncthreads();
-    for (int ci = s.fchp[f]; ci < s.fchp[f + 1]; ++ci) {
-      const int c = s.fch[ci];
-      const int Lc = s.fL[c], Wc = s.fW[c];
-      const int32_t* rel = s.relmap + s.frel[c];
-      const T* uc = vec + s.fvec[c] * nrhs + (int64_t)q * Lc + Wc;
-      for (int t = tid; t < Lc - Wc; t += kWG) y[rel[t]] += uc[t];
-      __syncthreads();
-    }
+    gather_children<T>(s, f, q, nrhs, yg, vec);
+    for (int r = tid; r < L; r += kWG) yW[r] = yg[r];
+    __syncthreads();
     for (int p = sp0; p < sp0 + np; ++p) {
-      int64_t pc0, nq;
-      piece_cols(s, p, pc0, nq);
+      int64_t pc0, nq64;
+      piece_cols(s, p, pc0, nq64);
+      const int nq = (int)nq64;
       const int cq = (int)(pc0 - fc0);
-      const int jl = L - cq;
-      const T* lp = lnz + (s.xlnz[pc0] - 1);
-      if (tid == 0) {
-        for (int k = 0; k < nq; ++k) {
-          int ip = (int)ipvt[pc0 + k] - 1;
-          if (ip != k) {
-            T t = y[cq + k];
-            y[cq + k] = y[cq + ip];
-            y[cq + ip] = t;
-          }
-        }
-        for (int k = 0; k < nq; ++k) {
-          T yk = y[cq + k];
-          if (yk != T(0))
-            for (int i = k + 1; i < nq; ++i) y[cq + i] -= yk * lp[i + (int64_t)k * jl];
-        }
-      }
+      stage_block<T, MAXW>(D, lnz + (s.xlnz[pc0] - 1), nq, L - cq);
+      for (int i = tid; i < nq; i += kWG) spiv[i] = (int)ipvt[pc0 + i] - 1;
       __syncthreads();
-      for (int r = (int)nq + tid; r < jl; r += kWG) {
-        T acc = T(0);
-        for (int k = 0; k < nq; ++k) acc += lp[r + (int64_t)k * jl] * y[cq + k];
-        y[cq + r] -= acc;
-      }
+      if (wave == 0) wave_lower_solve<T, MAXW>(yW + cq, D, spiv, nq);
+      __syncthreads();
+      for (int r = cq + nq + tid; r < L; r += kWG)
+        yW[r] -= dot_lrow<T>(s, lnz, pc0, nq, r, yW + cq);
       __syncthreads();
     }
-    for (int r = tid; r < W; r += kWG) x[fc0 + r] = y[r];
+    for (int r = tid; r < L; r += kWG) yg[r] = yW[r];
+    for (int r = tid; r < W; r += kWG) x[fc0 + r] = yW[r];
     __syncthreads();
   }
 }
 
-template <typename T>
+// Small fronts, backward. Dynamic LDS: yW[L].
+template <typename T, int MAXW>
 __global__ void __launch_bounds__(kWG)
-bwd_solve_kernel(DevSym s, const int32_t* __restrict__ list,
-                 const T* __restrict__ lnz, const T* __restrict__ unz,
-                 T* __restrict__ rhs, int64_t ldr, int nrhs,
-                 T* __restrict__ vec) {
-  __shared__ T red[kWG];
-  const int tid = threadIdx.x;
+bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict__ lnz,
+                 const T* __restrict__ unz, T* __restrict__ rhs, int64_t ldr, int nrhs) {
+  extern __shared__ unsigned char smem_raw[];
+  T* yW = reinterpret_cast<T*>(smem_raw);
+  __shared__ T D[MAXW * MAXW];
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const int f = list[blockIdx.x];
   const int L = s.fL[f], W = s.fW[f];
   const int64_t fc0 = s.fc0[f];
   const int sp0 = s.fsp0[f], np = s.fnp[f];
   const int32_t* I = s.lindx + (s.xlindx[sp0] - 1);
   for (int q = 0; q < nrhs; ++q) {
-    T* y = vec + s.fvec[f] * nrhs + (int64_t)q * L;
     T* x = rhs + (int64_t)q * ldr;
-    for (int r = tid; r < L; r += kWG) y[r] = x[I[r] - 1];
+    for (int r = tid; r < L; r += kWG) yW[r] = x[I[r] - 1];
     __syncthreads();
     for (int p = sp0 + np - 1; p >= sp0; --p) {
-      int64_t pc0, nq;
-      piece_cols(s, p, pc0, nq);
+      int64_t pc0, nq64;
+      piece_cols(s, p, pc0, nq64);
+      const int nq = (int)nq64;
       const int cq = (int)(pc0 - fc0);
-      const int jl = L - cq;
-      const int ul = jl - (int)nq;
-      const T* lp = lnz + (s.xlnz[pc0] - 1);
-      const T* up = unz + (s.xunz[pc0] - 1);
-      // y[cq+c] -= sum_r U12t[r,c] y[cq+nq+r]: one wave per column c.
-      const int wave = tid / 64, lane = tid % 64;
-      for (int c = wave; c < nq; c += kWG / 64) {
+      stage_block<T, MAXW>(D, lnz + (s.xlnz[pc0] - 1), nq, L - cq);
+      for (int c = wave; c < nq; c += 4) {
+        const T* u = unz + s.ucol[pc0 + c];
         T acc = T(0);
-        for (int r = lane; r < ul; r += 64) acc += up[r + (int64_t)c * ul] * y[cq + nq + r];
+        for (int j = cq + nq + lane; j < L; j += 64) acc += u[j] * yW[j];
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-        if (lane == 0) red[c] = acc;
+        if (lane == 0) yW[cq + c] -= acc;
       }
       __syncthreads();
-      if (tid == 0) {
-        for (int c = 0; c < nq; ++c) y[cq + c] -= red[c];
-        for (int k = (int)nq - 1; k >= 0; --k) {
-          T yk = y[cq + k];
-          if (yk != T(0)) {
-            yk /= lp[k + (int64_t)k * jl];
-            y[cq + k] = yk;
-            for (int i = 0; i < k; ++i) y[cq + i] -= yk * lp[i + (int64_t)k * jl];
-          }
-        }
-      }
+      if (wave == 0) wave_upper_solve<T, MAXW>(yW + cq, D, nq);
       __syncthreads();
     }
-    for (int r = tid; r < W; r += kWG) x[fc0 + r] = y[r];
+    for (int c = tid; c < W; c += kWG) x[fc0 + c] = yW[c];
+    __syncthreads();
+  }
+}
+
+// Large fronts, forward: y = [rhs on own columns; 0] + children (one
+// workgroup per front).
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+fwd_assemble_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict__ rhs,
+                    int64_t ldr, int nrhs, T* __restrict__ vec) {
+  const int f = list[blockIdx.x];
+  const int L = s.fL[f], W = s.fW[f];
+  const int64_t fc0 = s.fc0[f];
+  for (int q = 0; q < nrhs; ++q) {
+    T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
+    const T* x = rhs + (int64_t)q * ldr;
+    for (int r = threadIdx.x; r < L; r += kWG) yg[r] = (r < W) ? x[fc0 + r] : T(0);
+    __syncthreads();
+    gather_children<T>(s, f, q, nrhs, yg, vec);
+  }
+}
+
+// Forward chain of one block (one workgroup per front of the step).
+template <typename T, int MAXW, int GW>
+__global__ void __launch_bounds__(kWG)
+fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ lnz,
+                 const int64_t* __restrict__ ipvt, T* __restrict__ rhs, int64_t ldr, int nrhs,
+                 T* __restrict__ vec) {
+  __shared__ T yb[GW];
+  __shared__ T D[MAXW * MAXW];
+  __shared__ int spiv[MAXW];
+  const int tid = threadIdx.x, wave = tid / 64;
+  const SolveDesc d = desc[blockIdx.x];
+  const int f = d.front;
+  const int L = s.fL[f];
+  const int64_t fc0 = s.fc0[f];
+  const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
+  for (int q = 0; q < nrhs; ++q) {
+    T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
+    T* x = rhs + (int64_t)q * ldr;
+    for (int i = tid; i < bw; i += kWG) yb[i] = yg[c0 + i];
+    __syncthreads();
+    for (int pi = 0; pi < d.np; ++pi) {
+      int64_t pc0, nq64;
+      piece_cols(s, d.piece + pi, pc0, nq64);
+      const int nq = (int)nq64;
+      const int cq = (int)(pc0 - fc0);
+      stage_block<T, MAXW>(D, lnz + (s.xlnz[pc0] - 1), nq, L - cq);
+      for (int i = tid; i < nq; i += kWG) spiv[i] = (int)ipvt[pc0 + i] - 1;
+      __syncthreads();
+      if (wave == 0) wave_lower_solve<T, MAXW>(yb + (cq - c0), D, spiv, nq);
+      __syncthreads();
+      for (int r = cq + nq + tid; r < c1; r += kWG)
+        yb[r - c0] -= dot_lrow<T>(s, lnz, pc0, nq, r, yb + (cq - c0));
+      __syncthreads();
+    }
+    for (int i = tid; i < bw; i += kWG) {
+      yg[c0 + i] = yb[i];
+      x[fc0 + c0 + i] = yb[i];
+    }
+    __syncthreads();
+  }
+}
+
+// Forward: y[c1 + r0 + lane] -= L[., c0:c1] y[c0:c1]; wave w takes a
+// quarter of the block's columns, partial sums added in a fixed order.
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+fwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ lnz,
+                 int nrhs, T* __restrict__ vec) {
+  __shared__ T red[4][64];
+  const int tid = threadIdx.x, lane = tid % 64;
+  const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+  const SolveDesc d = desc[blockIdx.x];
+  const int f = d.front;
+  const int L = s.fL[f];
+  const int64_t fc0 = s.fc0[f];
+  const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
+  const int r = c1 + d.r0 + lane;
+  const int cb = c0 + bw * wave / 4, ce = c0 + bw * (wave + 1) / 4;
+  for (int q = 0; q < nrhs; ++q) {
+    T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
+    T acc = T(0);
+    if (r < L) acc = dot_lrow<T>(s, lnz, fc0 + cb, ce - cb, r, yg + cb);
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && r < L) yg[r] -= ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+    __syncthreads();
+  }
+}
+
+// Backward: t[c] = U[c, c1:L] x[I[c1:L]] for rows c = c0 + r0 + wave of
+// the block (one wave per row), stored in y[c].
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ unz,
+                 const T* __restrict__ rhs, int64_t ldr, int nrhs, T* __restrict__ vec) {
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const SolveDesc d = desc[blockIdx.x];
+  const int f = d.front;
+  const int c = d.c0 + d.r0 + wave;
+  if (c >= d.c1) return;
+  const int L = s.fL[f];
+  const int64_t fc0 = s.fc0[f];
+  const int32_t* I = s.lindx + (s.xlindx[s.fsp0[f]] - 1);
+  const T* u = unz + s.ucol[fc0 + c];
+  for (int q = 0; q < nrhs; ++q) {
+    const T* x = rhs + (int64_t)q * ldr;
+    T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
+    T acc = T(0);
+    int j = d.c1 + lane;
+    for (; j + 192 < L; j += 256) {
+      int g[4];
+      T uv[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        g[t] = I[j + 64 * t] - 1;
+        uv[t] = u[j + 64 * t];
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc += uv[t] * x[g[t]];
+    }
+    for (; j < L; j += 64) acc += u[j] * x[I[j] - 1];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) yg[c] = acc;
+  }
+}
+
+// Backward chain of one block.
+template <typename T, int MAXW, int GW>
+__global__ void __launch_bounds__(kWG)
+bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ lnz,
+                 const T* __restrict__ unz, T* __restrict__ rhs, int64_t ldr, int nrhs,
+                 T* __restrict__ vec) {
+  __shared__ T yb[GW];
+  __shared__ T D[MAXW * MAXW];
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const SolveDesc d = desc[blockIdx.x];
+  const int f = d.front;
+  const int L = s.fL[f];
+  const int64_t fc0 = s.fc0[f];
+  const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
+  for (int q = 0; q < nrhs; ++q) {
+    T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
+    T* x = rhs + (int64_t)q * ldr;
+    for (int i = tid; i < bw; i += kWG)
+      yb[i] = (c1 < L) ? x[fc0 + c0 + i] - yg[c0 + i] : x[fc0 + c0 + i];
+    __syncthreads();
+    for (int pi = d.np - 1; pi >= 0; --pi) {
+      int64_t pc0, nq64;
+      piece_cols(s, d.piece + pi, pc0, nq64);
+      const int nq = (int)nq64;
+      const int cq = (int)(pc0 - fc0);
+      stage_block<T, MAXW>(D, lnz + (s.xlnz[pc0] - 1), nq, L - cq);
+      for (int c = wave; c < nq; c += 4) {
+        const T* u = unz + s.ucol[pc0 + c];
+        T acc = T(0);
+        for (int j = cq + nq + lane; j < c1; j += 64) acc += u[j] * yb[j - c0];
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+        if (lane == 0) yb[cq - c0 + c] -= acc;
+      }
+      __syncthreads();
+      if (wave == 0) wave_upper_solve<T, MAXW>(yb + (cq - c0), D, nq);
+      __syncthreads();
+    }
+    for (int i = tid; i < bw; i += kWG) x[fc0 + c0 + i] = yb[i];
     __syncthreads();
   }
 }
@@ -731,6 +1265,24 @@ __global__ void colmaxabs_kernel(const double* __restrict__ v, int64_t ld,
     __syncthreads();
   }
   if (threadIdx.x == 0) out[q] = red[0];
+}
+
+// Max |v| over each segment of rows [seg[i], seg[i+1]) of each column:
+// out[i + q * nseg]; one block per (segment, column).
+__global__ void segmaxabs_kernel(const double* __restrict__ v, int64_t ld,
+                                 const int64_t* __restrict__ seg, int nseg,
+                                 double* __restrict__ out) {
+  __shared__ double red[kWG];
+  const int i = blockIdx.x % nseg, q = blockIdx.x / nseg;
+  double m = 0.0;
+  for (int64_t r = seg[i] + threadIdx.x; r < seg[i + 1]; r += kWG) m = fmax(m, fabs(v[r + q * ld]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = kWG / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[i + (int64_t)q * nseg] = red[0];
 }
 
 // x += d (d in the factor precision, already in original numbering).

@@ -1,0 +1,135 @@
+# SparspakHIP — the reference-side binding a Sparspak.jl maintainer would add
+# to route the numeric phase of SparseSolver / SparseSpdSolver to
+# libsparspak_hip.so (MI355X). Julia is not available in the build image, so
+# this file is not exercised by the tests here; its ccall signatures follow
+# include/sparspak_hip.h one to one.
+module SparspakHIP
+
+const libspk = get(ENV, "SPARSPAK_HIP_LIB",
+                   joinpath(@__DIR__, "..", "sparspak_hip", "libsparspak_hip.so"))
+
+const SPKHIP_KIND_LU = 0
+const SPKHIP_KIND_SPD = 1
+const SPKHIP_FP64 = 0
+const SPKHIP_FP32 = 1
+
+check(rc, what) = rc == 0 ? nothing :
+    rc == -1 ? error("$what: zero pivot in a diagonal block") :
+    rc == -3 ? error("$what: sequence error") :
+    rc <= -1000 ? error("$what: HIP error $(-1000 - rc)") : error("$what: error $rc")
+
+# ---- stateless drop-ins: same arguments as the reference's internal calls ----
+
+# replaces _lufactor!(n, nsuper, xsuper, snode, xlindx, lindx, xlnz, lnz,
+#                     xunz, unz, ipvt)   (SpkLUFactor.jl:60)
+function lufactor!(n, nsuper, xsuper::Vector{Int64}, snode::Vector{Int64},
+                   xlindx::Vector{Int64}, lindx::Vector{Int64}, xlnz::Vector{Int64},
+                   lnz::Vector{Float64}, xunz::Vector{Int64}, unz::Vector{Float64},
+                   ipvt::Vector{Int64})
+    GC.@preserve xsuper snode xlindx lindx xlnz lnz xunz unz ipvt begin
+        ccall((:spkhip_lufactor_f64, libspk), Int64,
+              (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+               Ptr{Float64}, Ptr{Int64}, Ptr{Float64}, Ptr{Int64}),
+              n, nsuper, xsuper, snode, xlindx, lindx, xlnz, lnz, xunz, unz, ipvt)
+    end
+end
+
+# replaces _lulsolve!(nsuper, xsuper, xlindx, lindx, xlnz, lnz, ipiv, rhs)
+function lulsolve!(nsuper, xsuper, xlindx, lindx, xlnz, lnz::Vector{Float64}, ipiv,
+                   rhs::Vector{Float64})
+    GC.@preserve xsuper xlindx lindx xlnz lnz ipiv rhs begin
+        ccall((:spkhip_lulsolve_f64, libspk), Int64,
+              (Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Float64},
+               Ptr{Int64}, Ptr{Float64}),
+              nsuper, xsuper, xlindx, lindx, xlnz, lnz, ipiv, rhs)
+    end
+end
+
+# replaces _luusolve!(n, nsuper, xsuper, xlindx, lindx, xlnz, lnz, xunz, unz, rhs)
+function luusolve!(n, nsuper, xsuper, xlindx, lindx, xlnz, lnz::Vector{Float64}, xunz,
+                   unz::Vector{Float64}, rhs::Vector{Float64})
+    GC.@preserve xsuper xlindx lindx xlnz lnz xunz unz rhs begin
+        ccall((:spkhip_luusolve_f64, libspk), Int64,
+              (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+               Ptr{Float64}, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}),
+              n, nsuper, xsuper, xlindx, lindx, xlnz, lnz, xunz, unz, rhs)
+    end
+end
+
+# replaces _ldltfactor! / _ldltsolve! (SpkLDLtFactor.jl:58, 266); correct
+# LDL^T numerics in the reference's layout
+function ldltfactor!(n, nsuper, xsuper, snode, xlindx, lindx, xlnz, lnz::Vector{Float64})
+    GC.@preserve xsuper snode xlindx lindx xlnz lnz begin
+        ccall((:spkhip_ldltfactor_f64, libspk), Int64,
+              (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+               Ptr{Float64}), n, nsuper, xsuper, snode, xlindx, lindx, xlnz, lnz)
+    end
+end
+
+function ldltsolve!(nsuper, xsuper, xlindx, lindx, xlnz, lnz::Vector{Float64},
+                    rhs::Vector{Float64})
+    GC.@preserve xsuper xlindx lindx xlnz lnz rhs begin
+        ccall((:spkhip_ldltsolve_f64, libspk), Int64,
+              (Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Float64},
+               Ptr{Float64}), nsuper, xsuper, xlindx, lindx, xlnz, lnz, rhs)
+    end
+end
+
+# ---- stateful handle: resident symbolic factor, values and factor ----
+
+mutable struct Handle
+    ptr::Ptr{Cvoid}
+end
+
+function Handle(device::Integer=0)
+    r = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:spkhip_create, libspk), Int64, (Ptr{Ptr{Cvoid}}, Int64), r, device),
+          "spkhip_create")
+    h = Handle(r[])
+    finalizer(h) do x
+        x.ptr == C_NULL || ccall((:spkhip_destroy, libspk), Int64, (Ptr{Cvoid},), x.ptr)
+        x.ptr = C_NULL
+    end
+    return h
+end
+
+# after _symbolicfactor!(s.slvr) of a SparseSolver built on a SparseMatrixCSC m
+function upload!(h::Handle, s, m; kind=SPKHIP_KIND_LU)
+    b = s.slvr
+    xunz = kind == SPKHIP_KIND_LU ? b.xunz : zeros(Int64, b.n + 1)
+    GC.@preserve b xunz m begin
+        rc = ccall((:spkhip_symbolic_upload, libspk), Int64,
+                   (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                    Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64},
+                    Ptr{Int64}),
+                   h.ptr, kind, b.n, b.nsuper, b.xsuper, b.snode, b.xlindx, b.lindx, b.xlnz,
+                   xunz, b.order.rperm, b.order.rinvp, m.colptr, m.rowval)
+    end
+    rc == -1 && error("No space for matrix element.")
+    check(rc, "spkhip_symbolic_upload")
+end
+
+inmatrix!(h::Handle, m) = GC.@preserve m check(
+    ccall((:spkhip_inmatrix, libspk), Int64, (Ptr{Cvoid}, Ptr{Float64}), h.ptr, m.nzval),
+    "spkhip_inmatrix")
+
+factor!(h::Handle; precision=SPKHIP_FP64) = check(
+    ccall((:spkhip_factor, libspk), Int64, (Ptr{Cvoid}, Int64), h.ptr, precision),
+    "An empty problem. No matrix.")   # the message _factor! raises (SpkSparseBase.jl:387)
+
+function solve!(h::Handle, B::StridedVecOrMat{Float64}; refine=false, tol=1e-14, maxiter=10)
+    nrhs = size(B, 2)
+    ldb = stride(B, 2)
+    GC.@preserve B begin
+        rc = refine ?
+            ccall((:spkhip_solve_refine, libspk), Int64,
+                  (Ptr{Cvoid}, Int64, Ptr{Float64}, Int64, Float64, Int64),
+                  h.ptr, nrhs, B, ldb, tol, maxiter) :
+            ccall((:spkhip_solve, libspk), Int64, (Ptr{Cvoid}, Int64, Ptr{Float64}, Int64),
+                  h.ptr, nrhs, B, ldb)
+    end
+    (rc == 0 || rc == -5) || check(rc, "Triangular Solve.")
+    return B
+end
+
+end # module

@@ -33,6 +33,15 @@ class Stats(ctypes.Structure):
     ]
 
 
+class KStat(ctypes.Structure):
+    _fields_ = [("launches", i64), ("ms", ctypes.c_double), ("flops", ctypes.c_double),
+                ("bytes", ctypes.c_double)]
+
+
+KCLASSES = ["assemble", "small_front", "large_init", "large_panel", "large_update",
+            "large_writeback", "fwd_solve", "bwd_solve", "residual", "large_urows",
+            "fwd_gemv", "bwd_gemv"]
+
 # name -> (restype, argtypes)
 _SIGS = {
     # host symbolic phase
@@ -63,6 +72,7 @@ _SIGS = {
     "spkhip_solve_refine": (i64, [vp, i64, dp, i64, ctypes.c_double, i64]),
     "spkhip_solve_refine_device": (i64, [vp, i64, vp, i64, ctypes.c_double,
                                          i64]),
+    "spkhip_set_segments": (i64, [vp, i64, i64p]),
     "spkhip_download": (i64, [vp, dp, dp, i64p]),
     "spkhip_download_f32": (i64, [vp, fp, fp, i64p]),
     "spkhip_stats": (i64, [vp, ctypes.POINTER(Stats)]),
@@ -81,6 +91,8 @@ _SIGS = {
     "spkhip_ldltfactor_f64": (i64, [i64, i64, i64p, i64p, i64p, i64p, i64p,
                                     dp]),
     "spkhip_ldltsolve_f64": (i64, [i64, i64p, i64p, i64p, i64p, dp, dp]),
+    "spkhip_profile_enable": (i64, [vp, i64]),
+    "spkhip_profile_get": (i64, [vp, i64, ctypes.POINTER(KStat)]),
     "spkhip_device_count": (i64, []),
     "spkhip_version": (ctypes.c_char_p, []),
 }
