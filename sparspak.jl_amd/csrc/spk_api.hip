@@ -89,6 +89,7 @@ struct LevelPlan {
   double small_flops = 0, small_elems = 0, init_elems = 0, wb_elems = 0, solve_elems = 0;
   int max_w = 0;                    // widest front of the level
   int large_off = 0, large_cnt = 0; // large fronts of the level
+  int ut_off = 0, ut_cnt = 0;       // U-part tiles (front, piece, 64 rows)
   struct SolveStep {
     int chain_off = 0, chain_cnt = 0, gemv_off = 0, gemv_cnt = 0;
     double elems = 0;               // factor entries read by the step
@@ -124,7 +125,7 @@ struct spkhip_handle {
       d_relmap, d_small, d_it_front, d_it_col, d_ea_child, d_ea_t, d_solve;
   DevBuf<i64> d_foff, d_fcb, d_frel, d_fvec, d_it_ea, d_lcol, d_ucol;
   DevBuf<BlkDesc> d_bdesc;
-  DevBuf<i32> d_large;
+  DevBuf<i32> d_large, d_ut_front, d_ut_piece, d_ut_r0;
   DevBuf<spk::SolveDesc> d_sdesc;
   DevSym ds{};
   // matrix
@@ -164,6 +165,7 @@ struct spkhip_handle {
     int level, grid;
   };
   int cur_level = -1, cur_grid = 0;   // tags of the next profiled launches
+  bool gemm_valu = false;              // trailing update on VALU instead of MFMA
   std::vector<Rec> recs;
   ~spkhip_handle() {
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
@@ -359,7 +361,8 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
                 std::vector<i32>& it_col, std::vector<i64>& it_ea,
                 std::vector<i32>& ea_child, std::vector<i32>& ea_t,
                 std::vector<BlkDesc>& bdesc, std::vector<i32>& solve, std::vector<i32>& largel,
-                std::vector<spk::SolveDesc>& sdesc) {
+                std::vector<spk::SolveDesc>& sdesc, std::vector<i32>& ut_front,
+                std::vector<i32>& ut_piece, std::vector<i32>& ut_r0) {
   const i32 nf = h->nfront;
   std::vector<std::vector<i32>> bylev(h->nlevel);
   for (i32 f = 0; f < nf; ++f) bylev[h->flev[f]].push_back(f);
@@ -389,6 +392,20 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
       if (h->fL[f] > kSmallLmax) largel.push_back(f);
     }
     lp.large_cnt = (int)largel.size() - lp.large_off;
+    lp.ut_off = (int)ut_front.size();
+    for (int li = lp.large_off; li < lp.large_off + lp.large_cnt; ++li) {
+      const i32 f = largel[li];
+      for (i32 p = h->fsp0[f]; p < h->fsp0[f] + h->fnp[f]; ++p) {
+        const i32 cq = (i32)(h->xsuper[p] - 1 - h->fc0[f]);
+        const i32 nq = (i32)(h->xsuper[p + 1] - h->xsuper[p]);
+        for (i32 r0 = 0; r0 < h->fL[f] - cq - nq; r0 += 64) {
+          ut_front.push_back(f);
+          ut_piece.push_back(p);
+          ut_r0.push_back(r0);
+        }
+      }
+    }
+    lp.ut_cnt = (int)ut_front.size() - lp.ut_off;
     for (i32 f : bylev[l]) {
       const double L = h->fL[f], W = h->fW[f], R = L - W;
       double fl = 0;
@@ -697,10 +714,14 @@ void run_factor(spkhip_handle* h) {
     if (lp.items_cnt > 0) {
       {
       Prof pr(h, SPKHIP_K_INIT, 0.0, lp.init_elems * sizeof(T));
+      if (lp.ut_cnt > 0)
+        hipLaunchKernelGGL((spk::large_utile_kernel<T, false>), dim3(lp.ut_cnt), dim3(kWG), 0, st,
+                           h->ds, h->d_ut_front.p + lp.ut_off, h->d_ut_piece.p + lp.ut_off,
+                           h->d_ut_r0.p + lp.ut_off, lnz.p, unz.p, arena.p);
       hipLaunchKernelGGL(spk::large_init_kernel<T>, dim3(lp.items_cnt), dim3(kWG), 0, st, h->ds,
                          h->d_it_front.p + lp.items_off, h->d_it_col.p + lp.items_off,
                          h->d_it_ea.p + lp.items_off, h->d_ea_child.p, h->d_ea_t.p, lnz.p,
-                         unz.p, arena.p);
+                         arena.p);
       hipck(hipGetLastError());
       }
       for (const LevelPlan::Group& G : lp.groups) {
@@ -726,11 +747,21 @@ void run_factor(spkhip_handle* h) {
         }
         h->cur_grid = G.gemm.wg;
         Prof pr(h, SPKHIP_K_UPDATE, G.gemm.flops, G.gemm.elems * sizeof(T));
-        hipLaunchKernelGGL((spk::blk_gemm_kernel<T, MAXW, GW, NC>), dim3(G.gemm.wg), dim3(kWG), 0,
-                           st, h->ds, h->d_bdesc.p + G.gemm.off, G.gemm.cnt, arena.p, stage.p);
+        if (h->gemm_valu)
+          hipLaunchKernelGGL((spk::blk_gemm_kernel<T, MAXW, GW, NC>), dim3(G.gemm.wg), dim3(kWG),
+                             0, st, h->ds, h->d_bdesc.p + G.gemm.off, G.gemm.cnt, arena.p,
+                             stage.p);
+        else
+          hipLaunchKernelGGL((spk::blk_gemm_mfma_kernel<T, MAXW, GW, NC>), dim3(G.gemm.wg),
+                             dim3(kWG), 0, st, h->ds, h->d_bdesc.p + G.gemm.off, G.gemm.cnt,
+                             arena.p, stage.p);
         hipck(hipGetLastError());
       }
       Prof pr(h, SPKHIP_K_WRITEBACK, 0.0, lp.wb_elems * sizeof(T));
+      if (lp.ut_cnt > 0)
+        hipLaunchKernelGGL((spk::large_utile_kernel<T, true>), dim3(lp.ut_cnt), dim3(kWG), 0, st,
+                           h->ds, h->d_ut_front.p + lp.ut_off, h->d_ut_piece.p + lp.ut_off,
+                           h->d_ut_r0.p + lp.ut_off, lnz.p, unz.p, arena.p);
       hipLaunchKernelGGL(spk::large_writeback_kernel<T>, dim3(lp.items_cnt), dim3(kWG), 0, st,
                          h->ds, h->d_it_front.p + lp.items_off, h->d_it_col.p + lp.items_off,
                          lnz.p, unz.p, arena.p);
@@ -912,11 +943,13 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   for (i64 J = 0; J < nsuper; ++J)
     if (xsuper[J + 1] - xsuper[J] > wmax) throw ApiError{-2};
   build_fronts(h);
-  std::vector<i32> small, it_front, it_col, ea_child, ea_t, solve, largel;
+  std::vector<i32> small, it_front, it_col, ea_child, ea_t, solve, largel, ut_front, ut_piece,
+      ut_r0;
   std::vector<spk::SolveDesc> sdesc;
   std::vector<i64> it_ea;
   std::vector<BlkDesc> bdesc;
-  build_plan(h, small, it_front, it_col, it_ea, ea_child, ea_t, bdesc, solve, largel, sdesc);
+  build_plan(h, small, it_front, it_col, it_ea, ea_child, ea_t, bdesc, solve, largel, sdesc,
+             ut_front, ut_piece, ut_r0);
 
   if (largel.empty()) largel.push_back(0);
   if (sdesc.empty()) sdesc.push_back(spk::SolveDesc{});
@@ -979,6 +1012,14 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   if (solve.empty()) solve.push_back(0);
   h->d_solve.upload(solve, st);
   h->d_large.upload(largel, st);
+  if (ut_front.empty()) {
+    ut_front.push_back(0);
+    ut_piece.push_back(0);
+    ut_r0.push_back(0);
+  }
+  h->d_ut_front.upload(ut_front, st);
+  h->d_ut_piece.upload(ut_piece, st);
+  h->d_ut_r0.upload(ut_r0, st);
   h->d_sdesc.upload(sdesc, st);
   DevSym& d = h->ds;
   d.n = n;
@@ -1122,7 +1163,7 @@ int64_t lufactor_stateless(i64 n, i64 nsuper, const i64* xsuper, const i64* snod
       hipck(hipMemcpyAsync(du.p, unz, h->nunz * sizeof(T), hipMemcpyHostToDevice, st));
     const int prec = std::is_same<T, double>::value ? SPKHIP_FP64 : SPKHIP_FP32;
     if (kind == SPKHIP_KIND_LU)
-      run_factor<T, 64, kGW, 48>(h);
+      run_factor<T, 64, kGW, 0>(h);
     else
       run_factor<T, 96, 96, 0>(h);
     h->precision = prec;
@@ -1334,19 +1375,21 @@ int64_t spkhip_factor(spkhip_t h, int64_t precision) {
     hipck(hipSetDevice(h->device));
     hipck(hipEventRecord(h->ev0, h->stream));
     const bool lu = h->kind == SPKHIP_KIND_LU;
-    // tuning knob: SPKHIP_PANEL_LDS=1 factors the pieces' diagonal blocks
-    // through LDS instead of registers
-    const bool lds_panel = std::getenv("SPKHIP_PANEL_LDS") != nullptr;
+    // tuning knobs: SPKHIP_PANEL_REG=1 factors the pieces' diagonal blocks
+    // in registers instead of LDS; SPKHIP_GEMM_VALU=1 runs the trailing
+    // update on the vector ALUs instead of the matrix cores
+    const bool reg_panel = std::getenv("SPKHIP_PANEL_REG") != nullptr;
+    h->gemm_valu = std::getenv("SPKHIP_GEMM_VALU") != nullptr;
     if (precision == SPKHIP_FP64) {
       run_assemble<double>(h);
       if (!lu) run_factor<double, 96, 96, 0>(h);
-      else if (lds_panel) run_factor<double, 64, kGW, 0>(h);
-      else run_factor<double, 64, kGW, 48>(h);
+      else if (reg_panel) run_factor<double, 64, kGW, 48>(h);
+      else run_factor<double, 64, kGW, 0>(h);
     } else {
       run_assemble<float>(h);
       if (!lu) run_factor<float, 96, 96, 0>(h);
-      else if (lds_panel) run_factor<float, 64, kGW, 0>(h);
-      else run_factor<float, 64, kGW, 48>(h);
+      else if (reg_panel) run_factor<float, 64, kGW, 48>(h);
+      else run_factor<float, 64, kGW, 0>(h);
     }
     hipck(hipEventRecord(h->ev1, h->stream));
     hipck(hipEventSynchronize(h->ev1));

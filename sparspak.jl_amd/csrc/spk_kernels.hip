@@ -13,6 +13,11 @@ __device__ inline void piece_cols(const DevSym& s, int64_t p, int64_t& c0,
   w = s.xsuper[p + 1] - s.xsuper[p];
 }
 
+// First (front-local) column of the piece holding own column c.
+__device__ inline int piece_start_of_col(const DevSym& s, int64_t fc0, int c) {
+  return (int)(s.xsuper[s.snode[fc0 + c] - 1] - 1 - fc0);
+}
+
 // Unit-stride element of the reference layout for front entry (r, j).
 // Returns the lnz (is_u = false) or unz (is_u = true) offset, or -1 for the
 // update-matrix block (r >= W and j >= W), which has no slot.
@@ -256,9 +261,61 @@ small_front_kernel(DevSym s, const int32_t* __restrict__ list,
 // Large fronts: the L x L work matrix lives in the arena.
 // ---------------------------------------------------------------------------
 
-// One workgroup per (front, column j): load column j from lnz/unz (zero in
-// the update block), then add the children's update-matrix columns that map
-// to j, one after the other.
+// U part of a large front (rows of a piece, columns right of it) moves
+// between unz (U12^T panels: a piece's U rows are its panel's columns) and
+// the work matrix through 64-row x nq tiles transposed in LDS, so that both
+// sides are read/written along their contiguous dimension. For SPD input the
+// U entries are the transposed L entries (lnz panel rows below the block).
+template <typename T, bool STORE>
+__global__ void __launch_bounds__(kWG)
+large_utile_kernel(DevSym s, const int32_t* __restrict__ it_front,
+                   const int32_t* __restrict__ it_piece, const int32_t* __restrict__ it_r0,
+                   T* __restrict__ lnz, T* __restrict__ unz, T* __restrict__ arena) {
+  __shared__ T tile[64 * 97];
+  const int f = it_front[blockIdx.x];
+  const int p = it_piece[blockIdx.x];
+  const int r0 = it_r0[blockIdx.x];
+  const int L = s.fL[f];
+  const int64_t fc0 = s.fc0[f];
+  int64_t pc0, nq64;
+  piece_cols(s, p, pc0, nq64);
+  const int nq = (int)nq64;
+  const int cq = (int)(pc0 - fc0);
+  const int ul = L - cq - nq;
+  const int nr = min(64, ul - r0);
+  const int ld = nq + 1;
+  T* F = arena + s.foff[f];
+  const bool from_l = s.sym && !STORE;
+  T* src = from_l ? lnz + (s.xlnz[pc0] - 1) + nq : unz + (s.xunz[pc0] - 1);
+  const int64_t sld = from_l ? (int64_t)(L - cq) : (int64_t)ul;
+  if (!STORE) {
+    for (int idx = threadIdx.x; idx < 64 * nq; idx += kWG) {
+      const int r = idx % 64, c = idx / 64;
+      if (r < nr) tile[r * ld + c] = src[(r0 + r) + c * sld];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 64 * nq; idx += kWG) {
+      const int c = idx % nq, r = idx / nq;
+      if (r < nr) F[(cq + c) + (int64_t)(cq + nq + r0 + r) * L] = tile[r * ld + c];
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < 64 * nq; idx += kWG) {
+      const int c = idx % nq, r = idx / nq;
+      if (r < nr) tile[r * ld + c] = F[(cq + c) + (int64_t)(cq + nq + r0 + r) * L];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 64 * nq; idx += kWG) {
+      const int r = idx % 64, c = idx / 64;
+      if (r < nr) unz[(s.xunz[pc0] - 1) + (r0 + r) + (int64_t)c * ul] = tile[r * ld + c];
+    }
+  }
+}
+
+// One workgroup per (front, column j): the column's L part from lnz (rows
+// from its piece's first row), zeros in the update block, then the
+// children's update-matrix columns that map to j, one child after the other
+// (rows of different children may coincide). Runs after large_utile_kernel
+// has placed the U part.
 template <typename T>
 __global__ void __launch_bounds__(kWG)
 large_init_kernel(DevSym s, const int32_t* __restrict__ it_front,
@@ -266,23 +323,25 @@ large_init_kernel(DevSym s, const int32_t* __restrict__ it_front,
                   const int64_t* __restrict__ it_ea,
                   const int32_t* __restrict__ ea_child,
                   const int32_t* __restrict__ ea_t, const T* __restrict__ lnz,
-                  const T* __restrict__ unz, T* __restrict__ arena) {
+                  T* __restrict__ arena) {
   const int item = blockIdx.x;
   const int f = it_front[item];
   const int j = it_col[item];
   const int L = s.fL[f], W = s.fW[f];
   const int64_t fc0 = s.fc0[f];
   T* F = arena + s.foff[f] + (int64_t)j * L;
-  for (int r = threadIdx.x; r < L; r += kWG) {
-    bool is_u = false;
-    if (s.sym) {  // lower triangle only: U entry (r, j) is L entry (j, r)
-      int rr = max(r, j), cc = min(r, j);
-      int64_t o = (cc < W) ? slot_of(s, fc0, W, rr, cc, is_u) : -1;
-      F[r] = (o < 0) ? T(0) : lnz[o];
+  if (j < W) {
+    const int cp = piece_start_of_col(s, fc0, j);
+    const T* lc = lnz + s.lcol[fc0 + j];
+    if (s.sym) {  // lower triangle only: the diagonal block's upper part is U
+      for (int r = cp + threadIdx.x; r < L; r += kWG)
+        if (r >= j) F[r] = lc[r];
+        else F[r] = lnz[s.lcol[fc0 + r] + j];
     } else {
-      int64_t o = slot_of(s, fc0, W, r, j, is_u);
-      F[r] = (o < 0) ? T(0) : (is_u ? unz[o] : lnz[o]);
+      for (int r = cp + threadIdx.x; r < L; r += kWG) F[r] = lc[r];
     }
+  } else {
+    for (int r = W + threadIdx.x; r < L; r += kWG) F[r] = T(0);
   }
   __syncthreads();
   for (int64_t e = it_ea[item]; e < it_ea[item + 1]; ++e) {
@@ -296,7 +355,7 @@ large_init_kernel(DevSym s, const int32_t* __restrict__ it_front,
   }
 }
 
-// Inverse of large_init_kernel's first half: panel entries back to lnz/unz.
+// L part of the own columns back to lnz (contiguous per column).
 template <typename T>
 __global__ void __launch_bounds__(kWG)
 large_writeback_kernel(DevSym s, const int32_t* __restrict__ it_front,
@@ -307,18 +366,12 @@ large_writeback_kernel(DevSym s, const int32_t* __restrict__ it_front,
   const int f = it_front[item];
   const int j = it_col[item];
   const int L = s.fL[f], W = s.fW[f];
+  if (j >= W) return;
   const int64_t fc0 = s.fc0[f];
   const T* F = arena + s.foff[f] + (int64_t)j * L;
-  const int rend = (j < W) ? L : W;
-  for (int r = threadIdx.x; r < rend; r += kWG) {
-    bool is_u = false;
-    int64_t o = slot_of(s, fc0, W, r, j, is_u);
-    if (o < 0) continue;
-    if (is_u)
-      unz[o] = F[r];
-    else
-      lnz[o] = F[r];
-  }
+  const int cp = piece_start_of_col(s, fc0, j);
+  T* lc = lnz + s.lcol[fc0 + j];
+  for (int r = cp + threadIdx.x; r < L; r += kWG) lc[r] = F[r];
 }
 
 __device__ inline int find_blk(const BlkDesc* d, int nd, int b) {
@@ -858,6 +911,142 @@ blk_gemm_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
       const int r = r0 + tx * 8 + a;
       if (r < L) fc[r] -= acc[a][c];
     }
+  }
+}
+
+// The same trailing update on the matrix cores: 128 x 128 tiles, each of
+// the four waves owns a 64 x 64 quadrant as 4 x 4 MFMA 16x16x4 tiles. The
+// operands are passed as (U^T, L^T) so that the accumulator holds the tile
+// transposed: lane l keeps row l % 16 of four columns, and each group of 16
+// lanes writes 16 contiguous rows of the column-major front.
+// Both instructions take A(i, k) and B(k, j) from lane i + 16 k (j + 16 k);
+// accumulator element v of lane l is D(i, l % 16) with i = 4 (l / 16) + v
+// for the f32 form and i = 4 v + l / 16 for the f64 form.
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lgrp, int v) { return 4 * v + lgrp; }
+};
+template <>
+struct Mfma16<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lgrp, int v) { return 4 * lgrp + v; }
+};
+
+template <typename T, int MAXW, int GW, int NC>
+__global__ void __launch_bounds__(kWG, 2)
+blk_gemm_mfma_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
+                     T* __restrict__ arena, const T* __restrict__ stage) {
+  constexpr int TM = 128, KC = 16, KP = KC + 1, PER = TM * KC / kWG;  // 8 loads each
+  typedef typename Mfma16<T>::v4 v4;
+  // double-buffered: As[buf][i * KP + k] = L(r0 + i, c0 + k0 + k),
+  //                  Bs[buf][j * KP + k] = U(c0 + k0 + k, j0 + j)
+  __shared__ T As[2][TM * KP];
+  __shared__ T Bs[2][TM * KP];
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int di = find_blk(desc, ndesc, blockIdx.x);
+  const BlkDesc d = desc[di];
+  const int b = blockIdx.x - d.wg_begin;
+  const int f = d.front;
+  const int L = s.fL[f];
+  const int64_t fc0 = s.fc0[f];
+  const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
+  T* F = arena + s.foff[f];
+  const int nt = d.nl;
+  if (b >= nt * nt) {  // commit one piece's staged blocks
+    const int pi = b - nt * nt;
+    int64_t pc0, nq64;
+    piece_cols(s, d.piece + pi, pc0, nq64);
+    const int nq = (int)nq64;
+    const int cq = (int)(pc0 - fc0);
+    const int uc = c1 - cq - nq;
+    const T* st = stage + (int64_t)(d.slot + pi) * stage_piece_elems<MAXW, GW>();
+    for (int idx = tid; idx < nq * nq; idx += kWG) {
+      int r = idx % nq, c = idx / nq;
+      F[(cq + r) + (int64_t)(cq + c) * L] = st[idx];
+    }
+    for (int idx = tid; idx < nq * uc; idx += kWG) {
+      int r = idx % nq, c = idx / nq;
+      F[(cq + r) + (int64_t)(cq + nq + c) * L] = st[MAXW * MAXW + idx];
+    }
+    return;
+  }
+  const int ti0 = b % nt, tj0 = b / nt;
+  const int r0 = c1 + ti0 * TM, j0 = c1 + tj0 * TM;
+  const int wr = (wave % 2) * 64, wc = (wave / 2) * 64;
+  const int l16 = lane % 16, lk = lane / 16;
+  // this thread's share of a chunk: A element (ai, ak + 2 t), B element
+  // (bk, bj + 16 t) for t < 8
+  const int ai = tid % TM, ak = tid / TM;       // ak in {0, 1}
+  const int bk = tid % KC, bj = tid / KC;       // bj in 0..15
+  const bool arow = r0 + ai < L;
+  const T* Ap = F + (r0 + ai) + (int64_t)c0 * L;
+  const T* Bp = F + c0 + bk;
+  T ra[PER], rb[PER];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int kk = ak + 2 * t;
+      ra[t] = (arow && k0 + kk < bw) ? Ap[(int64_t)(k0 + kk) * L] : T(0);
+      const int c = j0 + bj + 16 * t;
+      rb[t] = (k0 + bk < bw && c < L) ? Bp[k0 + (int64_t)c * L] : T(0);
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      As[buf][ai * KP + ak + 2 * t] = ra[t];
+      Bs[buf][(bj + 16 * t) * KP + bk] = rb[t];
+    }
+  };
+  v4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[a][c] = v4{T(0), T(0), T(0), T(0)};
+  const int nch = (bw + KC - 1) / KC;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ++ch) {
+    const int cur = ch & 1;
+    if (ch + 1 < nch) gload((ch + 1) * KC);
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ++ks) {
+      const int kk = ks * 4 + lk;
+      T av[4], bv[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        av[t] = As[cur][(wr + t * 16 + l16) * KP + kk];
+        bv[t] = Bs[cur][(wc + t * 16 + l16) * KP + kk];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[a][c] = Mfma16<T>::op(bv[c], av[a], acc[a][c]);
+    }
+    if (ch + 1 < nch) lstore(cur ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int r = r0 + wr + a * 16 + l16;
+    if (r >= L) continue;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int col = j0 + wc + c * 16 + Mfma16<T>::row(lk, v);
+        if (col < L) F[r + (int64_t)col * L] -= acc[a][c][v];
+      }
   }
 }
 
