@@ -47,6 +47,9 @@ def main():
     ap.add_argument("--cpu-grid", type=int, default=36)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--paths", default="fp64,fp32")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (three times), the "
+                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the multi-GPU path (subtree-to-GPU "
@@ -55,6 +58,14 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.calibrate_copy:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {src.numel() * 8} bytes three times")
+        return
     from sparspak_hip import SparseSolver, findorder, symbolicfactor
     from sparspak_hip import _lib
     from sparspak_hip.gallery import convdiff3d, uniform
@@ -153,7 +164,7 @@ def main():
     kst = R["kernels"]
     dom = max(kst, key=lambda k: kst[k]["ms"])
     dk = kst[dom]
-    compute_bound = dom in ("large_update", "small_front", "large_panel")
+    compute_bound = dom in ("large_update", "small_front", "large_diag", "large_offdiag")
     prec_name = "fp32" if head == "fp32" else "fp64"
     if compute_bound and dk["flops"] > 0:
         achieved = dk["flops"] / (dk["ms"] / 1e3) / 1e12

@@ -215,13 +215,13 @@ int64_t spkhip_ldltsolve_f64(int64_t nsuper, const int64_t* xsuper,
 #define SPKHIP_K_ASSEMBLE 0   /* zero lnz/unz + scatter A (inmatrix)     */
 #define SPKHIP_K_SMALL 1      /* whole small front in LDS                */
 #define SPKHIP_K_INIT 2       /* large front: load panels + extend-add   */
-#define SPKHIP_K_PANEL 3      /* large front: one piece inside its block */
+#define SPKHIP_K_DIAG 3       /* large front: diagonal region of a block */
 #define SPKHIP_K_UPDATE 4     /* large front: block trailing update      */
 #define SPKHIP_K_WRITEBACK 5  /* large front: panels back to lnz/unz     */
 #define SPKHIP_K_FWD 6        /* forward solve level                     */
 #define SPKHIP_K_BWD 7        /* backward solve level                    */
 #define SPKHIP_K_RESIDUAL 8   /* r = b - A x (refinement)                */
-#define SPKHIP_K_UROWS 9      /* large front: U rows of a block          */
+#define SPKHIP_K_OFFDIAG 9    /* large front: L rows below, U rows right */
 #define SPKHIP_K_FWD_GEMV 10  /* forward solve: rows below a block       */
 #define SPKHIP_K_BWD_GEMV 11  /* backward solve: U rows right of a block */
 #define SPKHIP_KCLASS_COUNT 12

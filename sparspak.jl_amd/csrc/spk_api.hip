@@ -78,13 +78,12 @@ struct LevelPlan {
   int small_off = 0, small_cnt = 0, small_maxL = 0;
   int items_off = 0, items_cnt = 0;
   struct Launch {
-    int off = 0, cnt = 0, wg = 0, lds_extra = 0;
-    int off256 = 0, wg256 = 0;  // panel: descriptors for 256 rows per workgroup
+    int off = 0, cnt = 0, wg = 0;
+    int lds = 0;                  // dynamic LDS elements per workgroup
     double flops = 0, elems = 0;  // algorithmic work of the launch
   };
   struct Group {  // one block step of the level's large fronts
-    std::vector<Launch> panels;  // one launch per piece index in the block
-    Launch urows, gemm;
+    Launch diag, offd, gemm;
   };
   double small_flops = 0, small_elems = 0, init_elems = 0, wb_elems = 0, solve_elems = 0;
   int max_w = 0;                    // widest front of the level
@@ -117,7 +116,6 @@ struct spkhip_handle {
   std::vector<i64> foff, fcb, frel, fvec;
   i64 arena_elems = 0, vec_elems = 0;
   std::vector<LevelPlan> levels;
-  i64 max_step_fronts = 0;
   double factor_flops = 0, solve_flops = 0;
   // device symbolic
   DevBuf<i64> d_xsuper, d_snode, d_xlnz, d_xunz, d_xlindx, d_perm, d_invp;
@@ -140,8 +138,8 @@ struct spkhip_handle {
   // numeric
   int precision = -1;  // of the current factor
   bool factored = false;
-  DevBuf<double> d_lnz64, d_unz64, d_arena64, d_stage64, d_vec64, d_rhs64;
-  DevBuf<float> d_lnz32, d_unz32, d_arena32, d_stage32, d_vec32, d_rhs32, d_tmp32;
+  DevBuf<double> d_lnz64, d_unz64, d_arena64, d_vec64, d_rhs64;
+  DevBuf<float> d_lnz32, d_unz32, d_arena32, d_vec32, d_rhs32, d_tmp32;
   DevBuf<i64> d_ipvt;
   DevBuf<unsigned long long> d_fail;
   i64 first_fail = 0;
@@ -368,7 +366,6 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
   for (i32 f = 0; f < nf; ++f) bylev[h->flev[f]].push_back(f);
   h->levels.assign(h->nlevel, LevelPlan{});
   it_ea.assign(1, 0);
-  h->max_step_fronts = 1;
   std::vector<i32> colcnt;
   for (i32 l = 0; l < h->nlevel; ++l) {
     LevelPlan& lp = h->levels[l];
@@ -483,66 +480,54 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
     }
     for (size_t gi = 0; gi < maxg; ++gi) {
       LevelPlan::Group G;
-      int maxp = 0;
-      std::vector<int> slot_base(large.size(), 0);
-      int slots = 0;
-      for (size_t fi = 0; fi < large.size(); ++fi)
-        if (gi < fblocks[fi].size()) {
-          slot_base[fi] = slots;
-          slots += fblocks[fi][gi].np;
-          maxp = std::max(maxp, fblocks[fi][gi].np);
+      G.diag.off = (int)bdesc.size();
+      for (size_t fi = 0; fi < large.size(); ++fi) {
+        if (gi >= fblocks[fi].size()) continue;
+        const Blk& B = fblocks[fi][gi];
+        const i32 f = large[fi];
+        bdesc.push_back(BlkDesc{f, B.p0, B.np, B.c0, B.c1, G.diag.wg, 0, 0});
+        G.diag.wg += 1;
+        const i32 bw = B.c1 - B.c0;
+        G.diag.lds = std::max(G.diag.lds, bw * (bw + 1));
+        for (i32 p = B.p0; p < B.p0 + B.np; ++p) {
+          const double q = (double)(h->xsuper[p + 1] - h->xsuper[p]);
+          const double u = (double)(B.c1 - (h->xsuper[p + 1] - 1 - h->fc0[f]));
+          G.diag.flops += 2.0 * q * q * q / 3.0 + 2.0 * q * q * u + 2.0 * u * u * q;
         }
-      h->max_step_fronts = std::max<i64>(h->max_step_fronts, slots);
-      for (int pi = 0; pi < maxp; ++pi) {
-        LevelPlan::Launch P;
-        P.off = (int)bdesc.size();
-        std::vector<BlkDesc> p256;
-        for (size_t fi = 0; fi < large.size(); ++fi) {
-          if (gi >= fblocks[fi].size() || pi >= fblocks[fi][gi].np) continue;
-          const Blk& B = fblocks[fi][gi];
-          const i32 f = large[fi];
-          const i32 p = B.p0 + pi;
-          const i32 cq = (i32)(h->xsuper[p] - 1 - h->fc0[f]);
-          const i32 nq = (i32)(h->xsuper[p + 1] - h->xsuper[p]);
-          const i32 m = h->fL[f] - cq - nq;
-          const i32 uc = B.c1 - cq - nq;
-          bdesc.push_back(BlkDesc{f, p, B.np, B.c0, B.c1, P.wg, 0, slot_base[fi] + pi});
-          P.wg += std::max(1, (m + 63) / 64);
-          p256.push_back(BlkDesc{f, p, B.np, B.c0, B.c1, P.wg256, 0, slot_base[fi] + pi});
-          P.wg256 += std::max(1, (m + 255) / 256);
-          P.lds_extra = std::max(P.lds_extra, uc);
-          const double md = m, qd = nq, ud = uc;
-          P.flops += 2.0 * qd * qd * qd / 3.0 + qd * qd * ud + md * qd * qd + 2.0 * md * qd * ud;
-          P.elems += 2.0 * (md + qd) * (qd + ud);
-        }
-        P.cnt = (int)bdesc.size() - P.off;
-        P.off256 = (int)bdesc.size();
-        bdesc.insert(bdesc.end(), p256.begin(), p256.end());
-        G.panels.push_back(P);
+        G.diag.elems += 2.0 * bw * bw;
       }
-      G.urows.off = (int)bdesc.size();
+      G.diag.cnt = (int)bdesc.size() - G.diag.off;
+      G.offd.off = (int)bdesc.size();
       for (size_t fi = 0; fi < large.size(); ++fi) {
         if (gi >= fblocks[fi].size()) continue;
         const Blk& B = fblocks[fi][gi];
         const i32 f = large[fi];
         const i32 t = h->fL[f] - B.c1;
         if (t <= 0) continue;
-        bdesc.push_back(BlkDesc{f, B.p0, B.np, B.c0, B.c1, G.urows.wg, 0, slot_base[fi]});
-        G.urows.wg += (t + 31) / 32;
-        const double bw = B.c1 - B.c0;
-        G.urows.flops += bw * bw * t;
-        G.urows.elems += 2.0 * bw * t;
+        const i32 nt = (t + 63) / 64;
+        bdesc.push_back(BlkDesc{f, B.p0, B.np, B.c0, B.c1, G.offd.wg, nt, 0});
+        G.offd.wg += 2 * nt;
+        const i32 bw = B.c1 - B.c0;
+        i32 strip = 0;
+        for (i32 p = B.p0; p < B.p0 + B.np; ++p) {
+          const i32 q = (i32)(h->xsuper[p + 1] - h->xsuper[p]);
+          strip = std::max(strip, (i32)(h->xsuper[p + 1] - 1 - h->fc0[f] - B.c0) * q);
+        }
+        G.offd.lds = std::max(G.offd.lds, 64 * (bw + 1) + strip);
+        G.offd.flops += 2.0 * (double)bw * bw * t;
+        G.offd.elems += 4.0 * (double)bw * t;
       }
-      G.urows.cnt = (int)bdesc.size() - G.urows.off;
+      G.offd.cnt = (int)bdesc.size() - G.offd.off;
       G.gemm.off = (int)bdesc.size();
       for (size_t fi = 0; fi < large.size(); ++fi) {
         if (gi >= fblocks[fi].size()) continue;
         const Blk& B = fblocks[fi][gi];
         const i32 f = large[fi];
         const i32 t = h->fL[f] - B.c1;
-        const i32 nt = t > 0 ? (t + 127) / 128 : 0;
-        bdesc.push_back(BlkDesc{f, B.p0, B.np, B.c0, B.c1, G.gemm.wg, nt, slot_base[fi]});
-        G.gemm.wg += nt * nt + B.np;
+        if (t <= 0) continue;
+        const i32 nt = (t + 127) / 128;
+        bdesc.push_back(BlkDesc{f, B.p0, B.np, B.c0, B.c1, G.gemm.wg, nt, 0});
+        G.gemm.wg += nt * nt;
         const double td = t, bw = B.c1 - B.c0;
         G.gemm.flops += 2.0 * td * td * bw;
         G.gemm.elems += 2.0 * td * td + 2.0 * td * bw;
@@ -657,7 +642,6 @@ struct Bufs<double> {
   static DevBuf<double>& lnz(spkhip_handle* h) { return h->d_lnz64; }
   static DevBuf<double>& unz(spkhip_handle* h) { return h->d_unz64; }
   static DevBuf<double>& arena(spkhip_handle* h) { return h->d_arena64; }
-  static DevBuf<double>& stage(spkhip_handle* h) { return h->d_stage64; }
   static DevBuf<double>& vec(spkhip_handle* h) { return h->d_vec64; }
   static DevBuf<double>& rhs(spkhip_handle* h) { return h->d_rhs64; }
 };
@@ -666,40 +650,37 @@ struct Bufs<float> {
   static DevBuf<float>& lnz(spkhip_handle* h) { return h->d_lnz32; }
   static DevBuf<float>& unz(spkhip_handle* h) { return h->d_unz32; }
   static DevBuf<float>& arena(spkhip_handle* h) { return h->d_arena32; }
-  static DevBuf<float>& stage(spkhip_handle* h) { return h->d_stage32; }
   static DevBuf<float>& vec(spkhip_handle* h) { return h->d_vec32; }
   static DevBuf<float>& rhs(spkhip_handle* h) { return h->d_rhs32; }
 };
 
-template <typename T, int MAXW, int GW, int NC>
+template <typename T>
 void set_lds_attrs() {
   static bool done = false;
   if (done) return;
   hipck(hipFuncSetAttribute((const void*)spk::small_front_kernel<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize,
                             kSmallLmax * kSmallLmax * (int)sizeof(T)));
-  hipck(hipFuncSetAttribute((const void*)spk::blk_panel_kernel<T, MAXW, GW, NC>,
+  hipck(hipFuncSetAttribute((const void*)spk::blk_diag_kernel<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-  hipck(hipFuncSetAttribute((const void*)spk::blk_urows_kernel<T, MAXW, GW, NC>,
+  hipck(hipFuncSetAttribute((const void*)spk::blk_offdiag_kernel<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
   done = true;
 }
 
 // The factorisation proper on already assembled lnz/unz (reference layout).
-template <typename T, int MAXW, int GW, int NC>
+template <typename T>
 void run_factor(spkhip_handle* h) {
   hipStream_t st = h->stream;
   DevBuf<T>& lnz = Bufs<T>::lnz(h);
   DevBuf<T>& unz = Bufs<T>::unz(h);
   DevBuf<T>& arena = Bufs<T>::arena(h);
-  DevBuf<T>& stage = Bufs<T>::stage(h);
   arena.ensure((size_t)std::max<i64>(h->arena_elems, 1));
-  stage.ensure((size_t)h->max_step_fronts * spk::stage_piece_elems<MAXW, GW>());
   h->d_ipvt.ensure((size_t)h->n);
   h->d_fail.ensure(1);
   hipLaunchKernelGGL(spk::fill_kernel<unsigned long long>, dim3(1), dim3(kWG), 0, st,
                      h->d_fail.p, (int64_t)1, ~0ull);
-  set_lds_attrs<T, MAXW, GW, NC>();
+  set_lds_attrs<T>();
   for (const LevelPlan& lp : h->levels) {
     h->cur_level = (int)(&lp - h->levels.data());
     h->cur_grid = lp.small_cnt;
@@ -725,37 +706,33 @@ void run_factor(spkhip_handle* h) {
       hipck(hipGetLastError());
       }
       for (const LevelPlan::Group& G : lp.groups) {
-        for (const LevelPlan::Launch& P : G.panels) {
-          Prof pr(h, SPKHIP_K_PANEL, P.flops, P.elems * sizeof(T));
-          const size_t lds =
-              (size_t)(MAXW * MAXW + 64 * MAXW + (MAXW + 1) * P.lds_extra) * sizeof(T);
-          const int wg = P.wg;
-          h->cur_grid = wg;
-          hipLaunchKernelGGL((spk::blk_panel_kernel<T, MAXW, GW, NC>), dim3(wg), dim3(kWG), lds,
-                             st, h->ds, h->d_bdesc.p + P.off, P.cnt, arena.p, stage.p,
-                             h->d_ipvt.p, h->d_fail.p);
+        {
+          h->cur_grid = G.diag.wg;
+          Prof pr(h, SPKHIP_K_DIAG, G.diag.flops, G.diag.elems * sizeof(T));
+          hipLaunchKernelGGL(spk::blk_diag_kernel<T>, dim3(G.diag.wg), dim3(kWG),
+                             (size_t)G.diag.lds * sizeof(T), st, h->ds,
+                             h->d_bdesc.p + G.diag.off, arena.p, h->d_ipvt.p, h->d_fail.p);
           hipck(hipGetLastError());
         }
-        h->cur_grid = G.urows.wg;
-        if (G.urows.cnt > 0) {
-          Prof pr(h, SPKHIP_K_UROWS, G.urows.flops, G.urows.elems * sizeof(T));
-          const size_t lds = (size_t)((GW + 1) * 32 + MAXW * MAXW) * sizeof(T);
-          hipLaunchKernelGGL((spk::blk_urows_kernel<T, MAXW, GW, NC>), dim3(G.urows.wg), dim3(kWG),
-                             lds, st, h->ds, h->d_bdesc.p + G.urows.off, G.urows.cnt, arena.p,
-                             stage.p, h->d_ipvt.p);
+        if (G.offd.cnt > 0) {
+          h->cur_grid = G.offd.wg;
+          Prof pr(h, SPKHIP_K_OFFDIAG, G.offd.flops, G.offd.elems * sizeof(T));
+          hipLaunchKernelGGL(spk::blk_offdiag_kernel<T>, dim3(G.offd.wg), dim3(kWG),
+                             (size_t)G.offd.lds * sizeof(T), st, h->ds,
+                             h->d_bdesc.p + G.offd.off, G.offd.cnt, arena.p, h->d_ipvt.p);
           hipck(hipGetLastError());
         }
-        h->cur_grid = G.gemm.wg;
-        Prof pr(h, SPKHIP_K_UPDATE, G.gemm.flops, G.gemm.elems * sizeof(T));
-        if (h->gemm_valu)
-          hipLaunchKernelGGL((spk::blk_gemm_kernel<T, MAXW, GW, NC>), dim3(G.gemm.wg), dim3(kWG),
-                             0, st, h->ds, h->d_bdesc.p + G.gemm.off, G.gemm.cnt, arena.p,
-                             stage.p);
-        else
-          hipLaunchKernelGGL((spk::blk_gemm_mfma_kernel<T, MAXW, GW, NC>), dim3(G.gemm.wg),
-                             dim3(kWG), 0, st, h->ds, h->d_bdesc.p + G.gemm.off, G.gemm.cnt,
-                             arena.p, stage.p);
-        hipck(hipGetLastError());
+        if (G.gemm.cnt > 0) {
+          h->cur_grid = G.gemm.wg;
+          Prof pr(h, SPKHIP_K_UPDATE, G.gemm.flops, G.gemm.elems * sizeof(T));
+          if (h->gemm_valu)
+            hipLaunchKernelGGL(spk::blk_gemm_kernel<T>, dim3(G.gemm.wg), dim3(kWG), 0, st,
+                               h->ds, h->d_bdesc.p + G.gemm.off, G.gemm.cnt, arena.p);
+          else
+            hipLaunchKernelGGL(spk::blk_gemm_mfma_kernel<T>, dim3(G.gemm.wg), dim3(kWG), 0, st,
+                               h->ds, h->d_bdesc.p + G.gemm.off, G.gemm.cnt, arena.p);
+          hipck(hipGetLastError());
+        }
       }
       Prof pr(h, SPKHIP_K_WRITEBACK, 0.0, lp.wb_elems * sizeof(T));
       if (lp.ut_cnt > 0)
@@ -1162,10 +1139,7 @@ int64_t lufactor_stateless(i64 n, i64 nsuper, const i64* xsuper, const i64* snod
     if (kind == SPKHIP_KIND_LU && h->nunz > 0)
       hipck(hipMemcpyAsync(du.p, unz, h->nunz * sizeof(T), hipMemcpyHostToDevice, st));
     const int prec = std::is_same<T, double>::value ? SPKHIP_FP64 : SPKHIP_FP32;
-    if (kind == SPKHIP_KIND_LU)
-      run_factor<T, 64, kGW, 0>(h);
-    else
-      run_factor<T, 96, 96, 0>(h);
+    run_factor<T>(h);
     h->precision = prec;
     h->factored = true;
     download_impl<T>(h, lnz, kind == SPKHIP_KIND_LU ? unz : nullptr, ipvt);
@@ -1375,21 +1349,15 @@ int64_t spkhip_factor(spkhip_t h, int64_t precision) {
     hipck(hipSetDevice(h->device));
     hipck(hipEventRecord(h->ev0, h->stream));
     const bool lu = h->kind == SPKHIP_KIND_LU;
-    // tuning knobs: SPKHIP_PANEL_REG=1 factors the pieces' diagonal blocks
-    // in registers instead of LDS; SPKHIP_GEMM_VALU=1 runs the trailing
-    // update on the vector ALUs instead of the matrix cores
-    const bool reg_panel = std::getenv("SPKHIP_PANEL_REG") != nullptr;
+    // tuning knob: SPKHIP_GEMM_VALU=1 runs the trailing update on the
+    // vector ALUs instead of the matrix cores
     h->gemm_valu = std::getenv("SPKHIP_GEMM_VALU") != nullptr;
     if (precision == SPKHIP_FP64) {
       run_assemble<double>(h);
-      if (!lu) run_factor<double, 96, 96, 0>(h);
-      else if (reg_panel) run_factor<double, 64, kGW, 48>(h);
-      else run_factor<double, 64, kGW, 0>(h);
+      run_factor<double>(h);
     } else {
       run_assemble<float>(h);
-      if (!lu) run_factor<float, 96, 96, 0>(h);
-      else if (reg_panel) run_factor<float, 64, kGW, 48>(h);
-      else run_factor<float, 64, kGW, 0>(h);
+      run_factor<float>(h);
     }
     hipck(hipEventRecord(h->ev1, h->stream));
     hipck(hipEventSynchronize(h->ev1));

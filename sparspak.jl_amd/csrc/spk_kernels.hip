@@ -388,33 +388,65 @@ __device__ inline int find_blk(const BlkDesc* d, int nd, int b) {
 
 // ---------------------------------------------------------------------------
 // Large fronts, blocked. The pieces of a front are taken in groups (blocks)
-// of at most GW columns [c0, c1). Within a block each piece p (columns
-// [cq, cq+nq)) is eliminated by blk_panel_kernel exactly as the reference
-// does for a supernode (SpkLUFactor.jl:230-240): LU with row interchanges
-// inside the nq x nq diagonal block, L21 = A21 U11^-1 for every row below,
-// and the interchange + unit-lower solve for the U12 part — here only for
-// the columns of the same block, whose rows below are updated at once.
-// blk_urows_kernel then forms the U rows of the whole block for the columns
-// right of it (per piece: the earlier pieces' contribution, the piece's own
-// interchanges, the unit-lower solve), and blk_gemm_kernel applies the
-// trailing update A22 -= L21 U12 with inner dimension c1 - c0 instead of one
-// piece width. Interchanges never touch earlier pieces' columns, as in the
-// reference. The factored diagonal block and in-block U12 of every piece are
-// kept in a staging slot until blk_gemm_kernel commits them, so that no
-// kernel reads a block another workgroup of the same launch is rewriting.
+// of consecutive pieces spanning columns [c0, c1) (at most 128 columns).
+// Each block takes three launches:
+//
+//  blk_diag_kernel     one workgroup per block factors the bw x bw diagonal
+//                      region [c0, c1)^2 in LDS, piece by piece, exactly as
+//                      the reference eliminates a supernode
+//                      (SpkLUFactor.jl:230-240): LU with row interchanges
+//                      inside the piece's nq x nq diagonal block, the
+//                      interchanges applied to the piece's own columns and
+//                      those right of it but never to earlier pieces' L
+//                      columns, L21 = A21 U11^-1 for the region rows below,
+//                      U12 = L11^-1 P A12 for the region columns right of
+//                      it, then the update of the rest of the region.
+//  blk_offdiag_kernel  rows below the block: X = A U_blk^-1 (U_blk = the
+//                      upper triangle of the factored region, one piece at a
+//                      time), and columns right of the block: per piece the
+//                      earlier pieces' contribution, the piece's interchanges
+//                      and its unit-lower solve — the left-looking order of
+//                      the reference, so every entry is the reference's.
+//  blk_gemm_*_kernel   trailing update A22 -= L21 U12 with inner dimension
+//                      bw instead of one piece width.
 // ---------------------------------------------------------------------------
 
-// Stage slot of one piece: D (nq x nq, leading dimension nq), at offset
-// MAXW*MAXW the in-block U12 (nq x ucols, leading dimension nq), at offset
-// MAXW*(MAXW+GW) the row permutation of the block (perm[i] = original row
-// now at position i, stored as values).
-template <int MAXW, int GW>
-__host__ __device__ constexpr int stage_piece_elems() { return MAXW * MAXW + MAXW * GW + MAXW; }
+// dst(idx, src(idx)) for idx < n over the workgroup, eight loads in flight
+// per thread (tile copies between the front in HBM and LDS).
+template <typename T, typename FS, typename FD>
+__device__ inline void copy8(int n, FS src, FD dst) {
+  for (int base = threadIdx.x; base < n; base += 8 * kWG) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int idx = base + u * kWG;
+      v[u] = (idx < n) ? src(idx) : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int idx = base + u * kWG;
+      if (idx < n) dst(idx, v[u]);
+    }
+  }
+}
 
-// Value of v in lane l (l wave-uniform).
+// Cross-lane moves inside rows of 16 lanes (DPP): quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_half_mirror, row_mirror.
+template <int CTRL>
+__device__ inline double dpp_mov(double v) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xffffffffll), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                          (unsigned long long)(unsigned)lo));
+}
+template <int CTRL>
+__device__ inline float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
 __device__ inline double rdlane(double v, int l) {
-  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffull), l);
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), l);
   const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
                                           (unsigned long long)(unsigned)lo));
@@ -423,113 +455,46 @@ __device__ inline float rdlane(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-// LU with row pivoting of the nq x nq block D (LDS, leading dimension ld,
-// nq <= NC <= 64) by wave 0, each lane holding one row in registers. Rows do
-// not move: every lane tracks the position its row would occupy after
-// LAPACK's interchanges, so the pivot search (first maximum |a| over
-// positions >= k, strict >) and the final layout are those of ggetrf!
-// (GenericBlasLapackFragments.jl:56-100); a zero pivot skips the interchange
-// and the scaling. On return D holds the packed LU by position, piv[k] the
-// interchange of step k, perm[p] the original row at position p, dinv[k] =
-// 1/U(k,k).
-template <typename T, int NC>
-__device__ void reg_getrf(T* D, int ld, int nq, int pivot, int* piv, int* perm, T* dinv,
-                          int* zero) {
-  const int lane = threadIdx.x % 64;
-  const bool mine = lane < nq;
-  T d[NC];
-#pragma unroll
-  for (int j = 0; j < NC; ++j) d[j] = (mine && j < nq) ? D[lane + j * ld] : T(0);
-  int pos = lane;
-  // runtime k: register elements are addressed by compile-time j only
-  for (int k = 0; k < nq; ++k) {
-    T dk = T(0);
-#pragma unroll
-    for (int j = 0; j < NC; ++j)
-      if (j == k) dk = d[j];
-    int kp = k;
-    if (pivot) {
-      const bool cand = mine && pos >= k;
-      T best = cand ? fabs(dk) : T(-1);
-      int bpos = cand ? pos : (1 << 30);
-      for (int off = 32; off > 0; off >>= 1) {
-        const T ob = __shfl_xor(best, off, 64);
-        const int op = __shfl_xor(bpos, off, 64);
-        if (ob > best || (ob == best && op < bpos)) {
-          best = ob;
-          bpos = op;
-        }
-      }
-      kp = __builtin_amdgcn_readfirstlane(bpos);
-    }
-    const unsigned long long mp = __ballot(mine && pos == kp);
-    const unsigned long long mk = __ballot(mine && pos == k);
-    const int blane = __builtin_amdgcn_readfirstlane(__ffsll((long long)mp) - 1);
-    const int klane = __builtin_amdgcn_readfirstlane(__ffsll((long long)mk) - 1);
-    const T pv = rdlane(dk, blane);
-    const bool nonzero = pv != T(0);
-    if (nonzero && kp != k) {
-      if (lane == klane)
-        pos = kp;
-      else if (lane == blane)
-        pos = k;
-    }
-    const T inv = T(1) / pv;
-    if (lane == 0) {
-      piv[k] = kp;
-      dinv[k] = inv;
-      if (!nonzero) *zero = 1;
-    }
-    const bool below = mine && pos > k;
-    T l = dk;
-    if (below && nonzero) l *= inv;
-#pragma unroll
-    for (int j = 0; j < NC; ++j) {
-      if (j == k) {
-        if (below) d[j] = l;
-      } else if (j > k && j < nq) {
-        const T uj = rdlane(d[j], blane);
-        if (below) d[j] -= l * uj;
-      }
-    }
-  }
-  if (mine) {
-#pragma unroll
-    for (int j = 0; j < NC; ++j)
-      if (j < nq) D[pos + j * ld] = d[j];
-    perm[pos] = lane;
-  }
+// Wave-uniform maximum of v over the 64 lanes (all lanes active).
+template <typename T>
+__device__ inline T wave_max(T v) {
+  v = fmax(v, dpp_mov<0xB1>(v));   // lanes ^1
+  v = fmax(v, dpp_mov<0x4E>(v));   // lanes ^2
+  v = fmax(v, dpp_mov<0x141>(v));  // mirror inside 8
+  v = fmax(v, dpp_mov<0x140>(v));  // mirror inside 16
+  return fmax(fmax(rdlane(v, 0), rdlane(v, 16)), fmax(rdlane(v, 32), rdlane(v, 48)));
 }
 
-// The same factorisation through LDS for pieces wider than a wave (SPD
-// blocks of up to 96 columns): lane = row, rows i and i + 64.
+// LU with row pivoting of the nq x nq block D (LDS, leading dimension ld,
+// nq <= 128) by one wave: lane = row, rows lane and lane + 64. The pivot is
+// the first maximum |a| over rows >= k (strict >), as ggetrf!
+// (GenericBlasLapackFragments.jl:56-100); a zero pivot skips the interchange
+// and the scaling and sets *zero. Interchanges are applied to the ncols >= nq
+// columns starting at D (the block and the columns right of it); piv[k] is
+// the row swapped with row k at step k, perm[p] the original row at position
+// p, dinv[k] = 1 / U(k, k). The update of a row is issued eight columns at a
+// time so that its LDS loads overlap.
 template <typename T>
-__device__ void wave_getrf(T* D, int ld, int nq, int pivot, int* piv, int* perm, T* dinv,
-                           int* zero) {
+__device__ void wave_getrf(T* D, int ld, int nq, int ncols, int pivot, int* piv, int* perm,
+                           T* dinv, int* zero) {
   const int lane = threadIdx.x % 64;
   for (int i = lane; i < nq; i += 64) perm[i] = i;
   __builtin_amdgcn_wave_barrier();
   for (int k = 0; k < nq; ++k) {
     int kp = k;
     if (pivot) {
-      T best = T(-1);
-      int bi = nq;
-      for (int i = k + lane; i < nq; i += 64) {
-        T a = fabs(D[i + k * ld]);
-        if (a > best) {
-          best = a;
-          bi = i;
-        }
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        T ob = __shfl_xor(best, off, 64);
-        int oi = __shfl_xor(bi, off, 64);
-        if (ob > best || (ob == best && oi < bi)) {
-          best = ob;
-          bi = oi;
-        }
-      }
-      kp = bi;
+      // first row holding the maximum |a| (rows k + lane, then k + 64 + lane)
+      const int r1 = k + lane, r2 = k + 64 + lane;
+      const T a1 = (r1 < nq) ? fabs(D[r1 + k * ld]) : T(-1);
+      const T a2 = (r2 < nq) ? fabs(D[r2 + k * ld]) : T(-1);
+      const T m = wave_max(fmax(a1, a2));
+      const unsigned long long m1 = __ballot(r1 < nq && a1 == m);
+      const unsigned long long m2 = __ballot(r2 < nq && a2 == m);
+      if (m1)
+        kp = k + __ffsll((long long)m1) - 1;
+      else if (m2)
+        kp = k + 64 + __ffsll((long long)m2) - 1;
+      kp = __builtin_amdgcn_readfirstlane(kp);
     }
     const T pv = D[kp + k * ld];
     const bool nonzero = pv != T(0);
@@ -543,7 +508,7 @@ __device__ void wave_getrf(T* D, int ld, int nq, int pivot, int* piv, int* perm,
       }
     }
     if (kp != k && nonzero)
-      for (int c = lane; c < nq; c += 64) {
+      for (int c = lane; c < ncols; c += 64) {
         T t = D[k + c * ld];
         D[k + c * ld] = D[kp + c * ld];
         D[kp + c * ld] = t;
@@ -557,30 +522,148 @@ __device__ void wave_getrf(T* D, int ld, int nq, int pivot, int* piv, int* perm,
         l *= inv;
         D[i + k * ld] = l;
       }
-      for (int j = k + 1; j < nq; ++j) D[i + j * ld] -= l * D[k + j * ld];
+      for (int j0 = k + 1; j0 < nq; j0 += 8) {
+        T u[8], x[8];
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+          const int j = min(j0 + jj, nq - 1);
+          u[jj] = D[k + j * ld];
+          x[jj] = D[i + j * ld];
+        }
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj)
+          if (j0 + jj < nq) D[i + (j0 + jj) * ld] = x[jj] - l * u[jj];
+      }
     }
     __builtin_amdgcn_wave_barrier();
   }
 }
 
-// One piece of a block: see the section comment. Workgroup b == 0 also
-// publishes the factored diagonal block, in-block U12 and permutation to the
-// stage and the pivots to ipvt; every workgroup handles 64 rows below the
-// block. Dynamic LDS: D[MAXW*MAXW], U[(MAXW+1)*ucmax], X[64*MAXW].
-template <typename T, int MAXW, int GW, int NC>
+// Most columns of one piece (the SPD limit 90, rounded up).
+constexpr int kMaxNq = 96;
+
+// Diagonal region of one block per workgroup (see the section comment).
+// Dynamic LDS: bw * (bw + 1) elements.
+template <typename T>
 __global__ void __launch_bounds__(kWG)
-blk_panel_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
-                 T* __restrict__ arena, T* __restrict__ stage,
-                 int64_t* __restrict__ ipvt,
-                 unsigned long long* __restrict__ first_fail) {
-  constexpr int UL = MAXW + 1;  // odd leading dimension: no bank conflicts
+blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ arena,
+                int64_t* __restrict__ ipvt, unsigned long long* __restrict__ first_fail) {
   extern __shared__ unsigned char smem_raw[];
-  __shared__ int piv[MAXW];
-  __shared__ int perm[MAXW];
-  __shared__ T dinv[MAXW];
+  __shared__ int piv[kMaxNq];
+  __shared__ int perm[kMaxNq];
+  __shared__ T dinv[kMaxNq];
   __shared__ int s_zero;
-  T* D = reinterpret_cast<T*>(smem_raw);
-  T* U = D + MAXW * MAXW;
+  T* R = reinterpret_cast<T*>(smem_raw);
+  const int tid = threadIdx.x, wave = tid / 64;
+  const BlkDesc d = desc[blockIdx.x];
+  const int f = d.front;
+  const int L = s.fL[f];
+  const int64_t fc0 = s.fc0[f];
+  const int c0 = d.c0, bw = d.c1 - d.c0, ld = bw + 1;
+  T* F = arena + s.foff[f];
+  T* Fr = F + c0 + (int64_t)c0 * L;
+  copy8<T>(
+      bw * bw, [&](int idx) { return Fr[idx % bw + (int64_t)(idx / bw) * L]; },
+      [&](int idx, T v) { R[idx % bw + (idx / bw) * ld] = v; });
+  __syncthreads();
+  for (int pi = 0; pi < d.np; ++pi) {
+    int64_t pc0, nq64;
+    piece_cols(s, d.piece + pi, pc0, nq64);
+    const int nq = (int)nq64;
+    const int o = (int)(pc0 - fc0) - c0;  // piece offset inside the region
+    const int rest = bw - o - nq;         // region rows below / columns right
+    T* D = R + o + o * ld;
+    if (wave == 0) {
+      if (tid == 0) s_zero = 0;
+      __builtin_amdgcn_wave_barrier();
+      // interchanges reach the region columns right of the piece too
+      wave_getrf<T>(D, ld, nq, nq + rest, s.pivot, piv, perm, dinv, &s_zero);
+    }
+    __syncthreads();
+    if (tid < nq) ipvt[pc0 + tid] = piv[tid] + 1;
+    if (tid == 0 && s_zero) atomicMin(first_fail, (unsigned long long)(d.piece + pi + 1));
+    if (tid >= 128) {
+      // U12 of the region (rows already interchanged): unit-lower solve,
+      // u_i = a_i - sum_k<i L(i, k) u_k
+      for (int c = tid - 128; c < rest; c += 128) {
+        T* u = D + (nq + c) * ld;
+        for (int i = 1; i < nq; ++i) {
+          T v = u[i];
+#pragma unroll 8
+          for (int kk = 0; kk < i; ++kk) v -= D[i + kk * ld] * u[kk];
+          u[i] = v;
+        }
+      }
+    } else {
+      // L21 of the region: x_j = (a_j - sum_k<j x_k U(k, j)) / U(j, j)
+      for (int r = tid; r < rest; r += 128) {
+        T* x = D + nq + r;
+        for (int j = 0; j < nq; ++j) {
+          T v = x[j * ld];
+#pragma unroll 8
+          for (int k = 0; k < j; ++k) v -= x[k * ld] * D[k + j * ld];
+          x[j * ld] = v * dinv[j];
+        }
+      }
+    }
+    __syncthreads();
+    if (rest > 0) {
+      // rest x rest update, 4 x 4 entries per thread and pass
+      const int nb = (rest + 3) / 4;
+      const T* Lb = D + nq;           // L21 rows, leading dimension ld
+      const T* Ub = D + nq * ld;      // U12 columns
+      T* C = D + nq + nq * ld;
+      for (int idx = tid; idx < nb * nb; idx += kWG) {
+        const int i0 = (idx % nb) * 4, j0 = (idx / nb) * 4;
+        int ir[4], jc[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          ir[a] = min(i0 + a, rest - 1);
+          jc[a] = min(j0 + a, rest - 1) * ld;
+        }
+        T acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) acc[a][c] = T(0);
+        for (int k = 0; k < nq; ++k) {
+          T lv[4], uv[4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            lv[a] = Lb[ir[a] + k * ld];
+            uv[a] = Ub[k + jc[a]];
+          }
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[a][c] += lv[a] * uv[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+            if (i0 + a < rest && j0 + c < rest) C[(i0 + a) + (j0 + c) * ld] -= acc[a][c];
+      }
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < bw * bw; idx += kWG) {
+    const int r = idx % bw, c = idx / bw;
+    Fr[r + (int64_t)c * L] = R[r + c * ld];
+  }
+}
+
+// Off-diagonal parts of one block after blk_diag_kernel (see the section
+// comment). Workgroups b < d.nl take 64 rows below the block, the others 64
+// columns right of it. Dynamic LDS: 64 * (bw + 1) + the largest piece strip
+// (o + nq) * nq of the block, in elements.
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+blk_offdiag_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
+                   T* __restrict__ arena, const int64_t* __restrict__ ipvt) {
+  extern __shared__ unsigned char smem_raw[];
+  __shared__ int spiv[kMaxNq];
+  T* S = reinterpret_cast<T*>(smem_raw);
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const int di = find_blk(desc, ndesc, blockIdx.x);
   const BlkDesc d = desc[di];
@@ -588,253 +671,134 @@ blk_panel_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
   const int f = d.front;
   const int L = s.fL[f];
   const int64_t fc0 = s.fc0[f];
-  int64_t pc0, nq64;
-  piece_cols(s, d.piece, pc0, nq64);
-  const int nq = (int)nq64;
-  const int cq = (int)(pc0 - fc0);
-  const int uc = d.c1 - cq - nq;   // in-block columns right of the piece
-  const int m = L - cq - nq;       // rows below the diagonal block
-  T* F = arena + s.foff[f];
-  T* X = U + UL * max(uc, 0);
-  if (tid == 0) s_zero = 0;
-  for (int idx = tid; idx < nq * nq; idx += kWG) {
-    int r = idx % nq, c = idx / nq;
-    D[r + c * MAXW] = F[(cq + r) + (int64_t)(cq + c) * L];
-  }
-  for (int idx = tid; idx < nq * uc; idx += kWG) {
-    int r = idx % nq, c = idx / nq;
-    U[r + c * UL] = F[(cq + r) + (int64_t)(cq + nq + c) * L];
-  }
-  __syncthreads();
-  if (wave == 0) {
-    if constexpr (NC > 0)
-      reg_getrf<T, NC>(D, MAXW, nq, s.pivot, piv, perm, dinv, &s_zero);
-    else
-      wave_getrf<T>(D, MAXW, nq, s.pivot, piv, perm, dinv, &s_zero);
-  }
-  __syncthreads();
-  // in-block U12 = L11^-1 P A12: one column per thread
-  if constexpr (NC > 0) {
-    for (int c = tid; c < uc; c += kWG) {
-      T u[NC];
-#pragma unroll
-      for (int i = 0; i < NC; ++i) u[i] = (i < nq) ? U[perm[i] + c * UL] : T(0);
-#pragma unroll
-      for (int k = 0; k < NC; ++k)
-#pragma unroll
-        for (int i = k + 1; i < NC; ++i)
-          if (i < nq) u[i] -= D[i + k * MAXW] * u[k];
-#pragma unroll
-      for (int i = 0; i < NC; ++i)
-        if (i < nq) U[i + c * UL] = u[i];
-    }
-  } else {
-    for (int c = tid; c < uc; c += kWG) {
-      T* u = U + c * UL;
-      for (int k = 0; k < nq; ++k) {
-        int kp = piv[k];
-        if (kp != k) {
-          T t = u[k];
-          u[k] = u[kp];
-          u[kp] = t;
-        }
-      }
-      for (int k = 0; k < nq; ++k) {
-        T uk = u[k];
-        for (int i = k + 1; i < nq; ++i) u[i] -= D[i + k * MAXW] * uk;
-      }
-    }
-  }
-  __syncthreads();
-  if (b == 0) {
-    T* st = stage + (int64_t)d.slot * stage_piece_elems<MAXW, GW>();
-    for (int idx = tid; idx < nq * nq; idx += kWG) {
-      int r = idx % nq, c = idx / nq;
-      st[r + c * nq] = D[r + c * MAXW];
-    }
-    for (int idx = tid; idx < nq * uc; idx += kWG) {
-      int r = idx % nq, c = idx / nq;
-      st[MAXW * MAXW + r + c * nq] = U[r + c * UL];
-    }
-    for (int k = tid; k < nq; k += kWG) {
-      st[MAXW * (MAXW + GW) + k] = (T)perm[k];
-      ipvt[pc0 + k] = piv[k] + 1;
-    }
-    if (tid == 0 && s_zero) atomicMin(first_fail, (unsigned long long)(d.piece + 1));
-  }
-  if (m <= 0) return;
-  // 64 rows per workgroup: wave 0 forms the L21 rows (gtrsm! 'r','u','n','n'
-  // order), all waves then apply the in-block update, 8 columns at a time
-  // so that eight loads are in flight per lane
-  const int r0 = cq + nq + b * 64;
-  const int nr = min(64, L - r0);
-  if constexpr (NC > 0) {
-    if (wave == 0) {
-      T x[NC];
-      const bool act = lane < nr;
-      T* row = F + (r0 + lane) + (int64_t)cq * L;
-#pragma unroll
-      for (int j = 0; j < NC; ++j) x[j] = (act && j < nq) ? row[(int64_t)j * L] : T(0);
-#pragma unroll
-      for (int j = 0; j < NC; ++j) {
-        if (j < nq) {
-          T v = x[j];
-#pragma unroll
-          for (int kk = 0; kk < j; ++kk) v -= D[kk + j * MAXW] * x[kk];
-          x[j] = v * dinv[j];
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < NC; ++j)
-        if (j < nq) {
-          X[lane + j * 64] = x[j];
-          if (act) row[(int64_t)j * L] = x[j];
-        }
-    }
-  } else {
-    for (int idx = tid; idx < 64 * nq; idx += kWG) {
-      int i = idx % 64, kk = idx / 64;
-      X[i + kk * 64] = (i < nr) ? F[(r0 + i) + (int64_t)(cq + kk) * L] : T(0);
-    }
-    __syncthreads();
-    if (wave == 0) {
-      T* xr = X + lane;
-      for (int j = 0; j < nq; ++j) {
-        T v = xr[j * 64];
-        for (int kk = 0; kk < j; ++kk) v -= D[kk + j * MAXW] * xr[kk * 64];
-        xr[j * 64] = v * dinv[j];
-      }
-    }
-    __syncthreads();
-    for (int idx = tid; idx < 64 * nq; idx += kWG) {
-      int i = idx % 64, kk = idx / 64;
-      if (i < nr) F[(r0 + i) + (int64_t)(cq + kk) * L] = X[i + kk * 64];
-    }
-  }
-  __syncthreads();
-  if (lane < nr) {
-    T* urow = F + (r0 + lane) + (int64_t)(cq + nq) * L;
-    for (int cb = wave * 8; cb < uc; cb += 32) {
-      T old[8], acc[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        old[j] = (cb + j < uc) ? urow[(int64_t)(cb + j) * L] : T(0);
-        acc[j] = T(0);
-      }
-      for (int kk = 0; kk < nq; ++kk) {
-        const T xv = X[lane + kk * 64];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) acc[j] += xv * U[kk + min(cb + j, uc - 1) * UL];
-      }
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        if (cb + j < uc) urow[(int64_t)(cb + j) * L] = old[j] - acc[j];
-    }
-  }
-}
-
-// U rows of the block [c0, c1) for 32 columns right of it, one workgroup per
-// (front, 32 columns). Dynamic LDS: A[(GW+1)*32], Dl[MAXW*MAXW].
-template <typename T, int MAXW, int GW, int NC>
-__global__ void __launch_bounds__(kWG)
-blk_urows_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
-                 T* __restrict__ arena, const T* __restrict__ stage,
-                 const int64_t* __restrict__ ipvt) {
-  constexpr int AL = GW + 1;  // odd leading dimension: no bank conflicts
-  extern __shared__ unsigned char smem_raw[];
-  __shared__ int sperm[MAXW];
-  T* A = reinterpret_cast<T*>(smem_raw);  // A[r + c*AL], r = row - c0
-  T* Dl = A + AL * 32;
-  const int tid = threadIdx.x;
-  const int di = find_blk(desc, ndesc, blockIdx.x);
-  const BlkDesc d = desc[di];
-  const int b = blockIdx.x - d.wg_begin;
-  const int f = d.front;
-  const int L = s.fL[f];
-  const int64_t fc0 = s.fc0[f];
   const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
-  const int j0 = c1 + b * 32;
-  const int nc = min(32, L - j0);
   T* F = arena + s.foff[f];
-  for (int idx = tid; idx < bw * nc; idx += kWG) {
-    int r = idx % bw, c = idx / bw;
-    A[r + c * AL] = F[(c0 + r) + (int64_t)(j0 + c) * L];
-  }
-  __syncthreads();
-  for (int pi = 0; pi < d.np; ++pi) {
-    int64_t pc0, nq64;
-    piece_cols(s, d.piece + pi, pc0, nq64);
-    const int nq = (int)nq64;
-    const int cq = (int)(pc0 - fc0);
-    const int rq = cq - c0;  // piece rows inside A
-    // earlier pieces' contribution: A[p rows] -= L[p rows, c0:cq] U[c0:cq]
-    if (rq > 0) {
-      const int c = tid % 32, i0 = tid / 32;
-      if (c < nc)
-        for (int i = i0; i < nq; i += kWG / 32) {
-          const T* lrow = F + (cq + i) + (int64_t)c0 * L;
-          T acc = T(0);
-          int k = 0;
-          for (; k + 8 <= rq; k += 8) {
-            T lv[8];
+  if (b < d.nl) {
+    // rows [r0, r0 + nr): X(i, k) at X[i + 64 k]
+    const int r0 = c1 + b * 64, nr = min(64, L - r0);
+    T* X = S;
+    T* Us = S + 64 * (bw + 1);  // strip of the piece's columns: Us[k * nq + j]
+    const T* Fx = F + r0 + (int64_t)c0 * L;
+    copy8<T>(
+        64 * bw,
+        [&](int idx) { return (idx % 64 < nr) ? Fx[idx % 64 + (int64_t)(idx / 64) * L] : T(0); },
+        [&](int idx, T v) { X[idx] = v; });
+    for (int pi = 0; pi < d.np; ++pi) {
+      int64_t pc0, nq64;
+      piece_cols(s, d.piece + pi, pc0, nq64);
+      const int nq = (int)nq64;
+      const int o = (int)(pc0 - fc0) - c0;
+      const int w = o + nq;
+      const T* Fu = F + c0 + (int64_t)(c0 + o) * L;
+      copy8<T>(
+          w * nq, [&](int idx) { return Fu[idx % w + (int64_t)(idx / w) * L]; },
+          [&](int idx, T v) { Us[(idx % w) * nq + idx / w] = v; });
+      __syncthreads();
+      if (o > 0) {  // earlier pieces of the block: X_q -= X_<q U_<q,q
+        for (int cb = wave * 8; cb < nq; cb += 32) {
+          T acc[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) lv[j] = lrow[(int64_t)(k + j) * L];
+          for (int jj = 0; jj < 8; ++jj) acc[jj] = T(0);
+          for (int k = 0; k < o; ++k) {
+            const T xv = X[lane + k * 64];
+            const T* ur = Us + k * nq;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) acc += lv[j] * A[k + j + c * AL];
+            for (int jj = 0; jj < 8; ++jj) acc[jj] += xv * ur[min(cb + jj, nq - 1)];
           }
-          for (; k < rq; ++k) acc += lrow[(int64_t)k * L] * A[k + c * AL];
-          A[rq + i + c * AL] -= acc;
+#pragma unroll
+          for (int jj = 0; jj < 8; ++jj)
+            if (cb + jj < nq) X[lane + (o + cb + jj) * 64] -= acc[jj];
         }
+        __syncthreads();
+      }
+      if (wave == 0) {  // X_q U_qq^-1, one row per lane
+        T* xr = X + lane + o * 64;
+        const T* uq = Us + o * nq;
+        for (int j = 0; j < nq; ++j) {
+          T v = xr[j * 64];
+#pragma unroll 8
+          for (int k = 0; k < j; ++k) v -= xr[k * 64] * uq[k * nq + j];
+          xr[j * 64] = v * (T(1) / uq[j * nq + j]);
+        }
+      }
+      __syncthreads();
     }
-    const T* st = stage + (int64_t)(d.slot + pi) * stage_piece_elems<MAXW, GW>();
-    for (int idx = tid; idx < nq * nq; idx += kWG) Dl[idx] = st[idx];
-    for (int i = tid; i < nq; i += kWG) sperm[i] = (int)st[MAXW * (MAXW + GW) + i];
-    __syncthreads();
-    if (tid < nc) {
-      if constexpr (NC > 0) {
-        T a[NC];
-        T* col = A + tid * AL + rq;
+    for (int idx = tid; idx < 64 * bw; idx += kWG) {
+      const int i = idx % 64, k = idx / 64;
+      if (i < nr) F[(r0 + i) + (int64_t)(c0 + k) * L] = X[idx];
+    }
+  } else {
+    // columns [j0, j0 + nc): A(r, c) at A[r + c * lda]
+    const int j0 = c1 + (b - d.nl) * 64, nc = min(64, L - j0);
+    const int lda = bw + 1;
+    T* A = S;
+    T* Ls = S + 64 * lda;  // the piece's rows of L: Ls[i + k * nq], k < o + nq
+    const T* Fa = F + c0 + (int64_t)j0 * L;
+    copy8<T>(
+        bw * 64,
+        [&](int idx) { return (idx / bw < nc) ? Fa[idx % bw + (int64_t)(idx / bw) * L] : T(0); },
+        [&](int idx, T v) { A[idx % bw + (idx / bw) * lda] = v; });
+    for (int pi = 0; pi < d.np; ++pi) {
+      int64_t pc0, nq64;
+      piece_cols(s, d.piece + pi, pc0, nq64);
+      const int nq = (int)nq64;
+      const int o = (int)(pc0 - fc0) - c0;
+      const int w = o + nq;
+      const T* Fl = F + c0 + o + (int64_t)c0 * L;
+      copy8<T>(
+          nq * w, [&](int idx) { return Fl[idx % nq + (int64_t)(idx / nq) * L]; },
+          [&](int idx, T v) { Ls[idx] = v; });
+      if (tid < nq) spiv[tid] = (int)ipvt[pc0 + tid] - 1;
+      __syncthreads();
+      if (o > 0) {  // earlier pieces: A_q -= L_q,<q A_<q (rows not yet swapped)
+        T* ac = A + lane * lda;
+        for (int ib = wave * 8; ib < nq; ib += 32) {
+          T acc[8];
 #pragma unroll
-        for (int i = 0; i < NC; ++i) a[i] = (i < nq) ? col[sperm[i]] : T(0);
+          for (int ii = 0; ii < 8; ++ii) acc[ii] = T(0);
+          for (int k = 0; k < o; ++k) {
+            const T av = ac[k];
+            const T* lk = Ls + k * nq;
 #pragma unroll
-        for (int k = 0; k < NC; ++k)
+            for (int ii = 0; ii < 8; ++ii) acc[ii] += lk[min(ib + ii, nq - 1)] * av;
+          }
 #pragma unroll
-          for (int i = k + 1; i < NC; ++i)
-            if (i < nq) a[i] -= Dl[i + k * nq] * a[k];
-#pragma unroll
-        for (int i = 0; i < NC; ++i)
-          if (i < nq) col[i] = a[i];
-      } else {
-        T* a = A + tid * AL + rq;
+          for (int ii = 0; ii < 8; ++ii)
+            if (ib + ii < nq) ac[o + ib + ii] -= acc[ii];
+        }
+        __syncthreads();
+      }
+      if (wave == 0) {  // interchanges, then L_qq^-1, one column per lane
+        T* a = A + lane * lda + o;
         for (int k = 0; k < nq; ++k) {
-          int kp = (int)ipvt[pc0 + k] - 1;
+          const int kp = spiv[k];
           if (kp != k) {
-            T t = a[k];
+            const T t = a[k];
             a[k] = a[kp];
             a[kp] = t;
           }
         }
-        for (int k = 0; k < nq; ++k) {
-          T ak = a[k];
-          for (int i = k + 1; i < nq; ++i) a[i] -= Dl[i + k * nq] * ak;
+        const T* lq = Ls + o * nq;
+        for (int i = 1; i < nq; ++i) {
+          T v = a[i];
+#pragma unroll 8
+          for (int kk = 0; kk < i; ++kk) v -= lq[i + kk * nq] * a[kk];
+          a[i] = v;
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
-  }
-  for (int idx = tid; idx < bw * nc; idx += kWG) {
-    int r = idx % bw, c = idx / bw;
-    F[(c0 + r) + (int64_t)(j0 + c) * L] = A[r + c * AL];
+    for (int idx = tid; idx < bw * 64; idx += kWG) {
+      const int r = idx % bw, c = idx / bw;
+      if (c < nc) F[(c0 + r) + (int64_t)(j0 + c) * L] = A[r + c * lda];
+    }
   }
 }
 
-// Trailing update A[c1:, c1:] -= L[c1:, c0:c1] U[c0:c1, c1:] on 128 x 128
-// tiles (8 x 8 per thread, k staged 16 at a time in LDS); the last d.np
-// workgroups of a front commit the staged diagonal blocks / in-block U12.
-template <typename T, int MAXW, int GW, int NC>
+template <typename T>
 __global__ void __launch_bounds__(kWG)
 blk_gemm_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
-                T* __restrict__ arena, const T* __restrict__ stage) {
+                T* __restrict__ arena) {
   constexpr int TM = 128, KC = 16;
   __shared__ T As[KC][TM];
   __shared__ T Bs[KC][TM];
@@ -848,24 +812,6 @@ blk_gemm_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
   const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
   T* F = arena + s.foff[f];
   const int nt = d.nl;
-  if (b >= nt * nt) {  // commit one piece's staged blocks
-    const int pi = b - nt * nt;
-    int64_t pc0, nq64;
-    piece_cols(s, d.piece + pi, pc0, nq64);
-    const int nq = (int)nq64;
-    const int cq = (int)(pc0 - fc0);
-    const int uc = c1 - cq - nq;
-    const T* st = stage + (int64_t)(d.slot + pi) * stage_piece_elems<MAXW, GW>();
-    for (int idx = tid; idx < nq * nq; idx += kWG) {
-      int r = idx % nq, c = idx / nq;
-      F[(cq + r) + (int64_t)(cq + c) * L] = st[idx];
-    }
-    for (int idx = tid; idx < nq * uc; idx += kWG) {
-      int r = idx % nq, c = idx / nq;
-      F[(cq + r) + (int64_t)(cq + nq + c) * L] = st[MAXW * MAXW + idx];
-    }
-    return;
-  }
   const int ti = b % nt, tj = b / nt;
   const int r0 = c1 + ti * TM, j0 = c1 + tj * TM;
   const int tx = tid % 16, ty = tid / 16;
@@ -941,10 +887,10 @@ struct Mfma16<float> {
   static __device__ inline int row(int lgrp, int v) { return 4 * lgrp + v; }
 };
 
-template <typename T, int MAXW, int GW, int NC>
+template <typename T>
 __global__ void __launch_bounds__(kWG, 2)
 blk_gemm_mfma_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
-                     T* __restrict__ arena, const T* __restrict__ stage) {
+                     T* __restrict__ arena) {
   constexpr int TM = 128, KC = 16, KP = KC + 1, PER = TM * KC / kWG;  // 8 loads each
   typedef typename Mfma16<T>::v4 v4;
   // double-buffered: As[buf][i * KP + k] = L(r0 + i, c0 + k0 + k),
@@ -961,24 +907,6 @@ blk_gemm_mfma_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
   const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
   T* F = arena + s.foff[f];
   const int nt = d.nl;
-  if (b >= nt * nt) {  // commit one piece's staged blocks
-    const int pi = b - nt * nt;
-    int64_t pc0, nq64;
-    piece_cols(s, d.piece + pi, pc0, nq64);
-    const int nq = (int)nq64;
-    const int cq = (int)(pc0 - fc0);
-    const int uc = c1 - cq - nq;
-    const T* st = stage + (int64_t)(d.slot + pi) * stage_piece_elems<MAXW, GW>();
-    for (int idx = tid; idx < nq * nq; idx += kWG) {
-      int r = idx % nq, c = idx / nq;
-      F[(cq + r) + (int64_t)(cq + c) * L] = st[idx];
-    }
-    for (int idx = tid; idx < nq * uc; idx += kWG) {
-      int r = idx % nq, c = idx / nq;
-      F[(cq + r) + (int64_t)(cq + nq + c) * L] = st[MAXW * MAXW + idx];
-    }
-    return;
-  }
   const int ti0 = b % nt, tj0 = b / nt;
   const int r0 = c1 + ti0 * TM, j0 = c1 + tj0 * TM;
   const int wr = (wave % 2) * 64, wc = (wave / 2) * 64;

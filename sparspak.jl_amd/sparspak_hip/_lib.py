@@ -38,8 +38,8 @@ class KStat(ctypes.Structure):
                 ("bytes", ctypes.c_double)]
 
 
-KCLASSES = ["assemble", "small_front", "large_init", "large_panel", "large_update",
-            "large_writeback", "fwd_solve", "bwd_solve", "residual", "large_urows",
+KCLASSES = ["assemble", "small_front", "large_init", "large_diag", "large_update",
+            "large_writeback", "fwd_solve", "bwd_solve", "residual", "large_offdiag",
             "fwd_gemv", "bwd_gemv"]
 
 # name -> (restype, argtypes)
