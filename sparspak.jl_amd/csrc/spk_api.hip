@@ -28,6 +28,8 @@ using spk::kWG;
 using spk::kSmallLmax;
 constexpr int kGW = 128;    // widest block of pieces (LU)
 constexpr int kGMaxP = 8;   // most pieces in one block
+constexpr int kNqLU = 48;   // widest LU piece the blocked kernels take
+constexpr int kNqSPD = 96;  // widest SPD piece
 
 struct HipFail {
   hipError_t e;
@@ -488,7 +490,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         bdesc.push_back(BlkDesc{f, B.p0, B.np, B.c0, B.c1, G.diag.wg, 0, 0});
         G.diag.wg += 1;
         const i32 bw = B.c1 - B.c0;
-        G.diag.lds = std::max(G.diag.lds, bw * (bw + 1));
+        G.diag.lds = std::max(G.diag.lds, bw * spk::kRLD);
         for (i32 p = B.p0; p < B.p0 + B.np; ++p) {
           const double q = (double)(h->xsuper[p + 1] - h->xsuper[p]);
           const double u = (double)(B.c1 - (h->xsuper[p + 1] - 1 - h->fc0[f]));
@@ -508,12 +510,10 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         bdesc.push_back(BlkDesc{f, B.p0, B.np, B.c0, B.c1, G.offd.wg, nt, 0});
         G.offd.wg += 2 * nt;
         const i32 bw = B.c1 - B.c0;
-        i32 strip = 0;
-        for (i32 p = B.p0; p < B.p0 + B.np; ++p) {
-          const i32 q = (i32)(h->xsuper[p + 1] - h->xsuper[p]);
-          strip = std::max(strip, (i32)(h->xsuper[p + 1] - 1 - h->fc0[f] - B.c0) * q);
-        }
-        G.offd.lds = std::max(G.offd.lds, 64 * (bw + 1) + strip);
+        // strip of the widest extent o + nq, NQ elements per row
+        const i32 nqmax = (h->kind == SPKHIP_KIND_LU) ? kNqLU : kNqSPD;
+        const i32 strip = (i32)(h->xsuper[B.p0 + B.np] - 1 - h->fc0[f] - B.c0) * nqmax;
+        G.offd.lds = std::max(G.offd.lds, 64 * spk::kRLD + strip);
         G.offd.flops += 2.0 * (double)bw * bw * t;
         G.offd.elems += 4.0 * (double)bw * t;
       }
@@ -661,9 +661,13 @@ void set_lds_attrs() {
   hipck(hipFuncSetAttribute((const void*)spk::small_front_kernel<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize,
                             kSmallLmax * kSmallLmax * (int)sizeof(T)));
-  hipck(hipFuncSetAttribute((const void*)spk::blk_diag_kernel<T>,
+  hipck(hipFuncSetAttribute((const void*)spk::blk_diag_kernel<T, kNqLU>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-  hipck(hipFuncSetAttribute((const void*)spk::blk_offdiag_kernel<T>,
+  hipck(hipFuncSetAttribute((const void*)spk::blk_diag_kernel<T, kNqSPD>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+  hipck(hipFuncSetAttribute((const void*)spk::blk_offdiag_kernel<T, kNqLU>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+  hipck(hipFuncSetAttribute((const void*)spk::blk_offdiag_kernel<T, kNqSPD>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
   done = true;
 }
@@ -681,6 +685,7 @@ void run_factor(spkhip_handle* h) {
   hipLaunchKernelGGL(spk::fill_kernel<unsigned long long>, dim3(1), dim3(kWG), 0, st,
                      h->d_fail.p, (int64_t)1, ~0ull);
   set_lds_attrs<T>();
+  const bool lu = h->kind == SPKHIP_KIND_LU;
   for (const LevelPlan& lp : h->levels) {
     h->cur_level = (int)(&lp - h->levels.data());
     h->cur_grid = lp.small_cnt;
@@ -709,17 +714,27 @@ void run_factor(spkhip_handle* h) {
         {
           h->cur_grid = G.diag.wg;
           Prof pr(h, SPKHIP_K_DIAG, G.diag.flops, G.diag.elems * sizeof(T));
-          hipLaunchKernelGGL(spk::blk_diag_kernel<T>, dim3(G.diag.wg), dim3(kWG),
-                             (size_t)G.diag.lds * sizeof(T), st, h->ds,
-                             h->d_bdesc.p + G.diag.off, arena.p, h->d_ipvt.p, h->d_fail.p);
+          if (lu)
+            hipLaunchKernelGGL((spk::blk_diag_kernel<T, kNqLU>), dim3(G.diag.wg), dim3(kWG),
+                               (size_t)G.diag.lds * sizeof(T), st, h->ds,
+                               h->d_bdesc.p + G.diag.off, arena.p, h->d_ipvt.p, h->d_fail.p);
+          else
+            hipLaunchKernelGGL((spk::blk_diag_kernel<T, kNqSPD>), dim3(G.diag.wg), dim3(kWG),
+                               (size_t)G.diag.lds * sizeof(T), st, h->ds,
+                               h->d_bdesc.p + G.diag.off, arena.p, h->d_ipvt.p, h->d_fail.p);
           hipck(hipGetLastError());
         }
         if (G.offd.cnt > 0) {
           h->cur_grid = G.offd.wg;
           Prof pr(h, SPKHIP_K_OFFDIAG, G.offd.flops, G.offd.elems * sizeof(T));
-          hipLaunchKernelGGL(spk::blk_offdiag_kernel<T>, dim3(G.offd.wg), dim3(kWG),
-                             (size_t)G.offd.lds * sizeof(T), st, h->ds,
-                             h->d_bdesc.p + G.offd.off, G.offd.cnt, arena.p, h->d_ipvt.p);
+          if (lu)
+            hipLaunchKernelGGL((spk::blk_offdiag_kernel<T, kNqLU>), dim3(G.offd.wg), dim3(kWG),
+                               (size_t)G.offd.lds * sizeof(T), st, h->ds,
+                               h->d_bdesc.p + G.offd.off, G.offd.cnt, arena.p, h->d_ipvt.p);
+          else
+            hipLaunchKernelGGL((spk::blk_offdiag_kernel<T, kNqSPD>), dim3(G.offd.wg), dim3(kWG),
+                               (size_t)G.offd.lds * sizeof(T), st, h->ds,
+                               h->d_bdesc.p + G.offd.off, G.offd.cnt, arena.p, h->d_ipvt.p);
           hipck(hipGetLastError());
         }
         if (G.gemm.cnt > 0) {
@@ -916,7 +931,7 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   h->factor_flops = fl;
   h->solve_flops = (kind == SPKHIP_KIND_LU) ? 2.0 * (2.0 * sl - (double)n) - 3.0 * n
                                             : 4.0 * sl - 3.0 * n;
-  const i64 wmax = (kind == SPKHIP_KIND_LU) ? 48 : 96;
+  const i64 wmax = (kind == SPKHIP_KIND_LU) ? kNqLU : kNqSPD;
   for (i64 J = 0; J < nsuper; ++J)
     if (xsuper[J + 1] - xsuper[J] > wmax) throw ApiError{-2};
   build_fronts(h);

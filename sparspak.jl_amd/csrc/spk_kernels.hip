@@ -411,23 +411,33 @@ __device__ inline int find_blk(const BlkDesc* d, int nd, int b) {
 //                      bw instead of one piece width.
 // ---------------------------------------------------------------------------
 
-// dst(idx, src(idx)) for idx < n over the workgroup, eight loads in flight
-// per thread (tile copies between the front in HBM and LDS).
-template <typename T, typename FS, typename FD>
-__device__ inline void copy8(int n, FS src, FD dst) {
-  for (int base = threadIdx.x; base < n; base += 8 * kWG) {
-    T v[8];
+// Copy of an nr x nc tile (nr <= 128) from a column-major HBM array (leading
+// dimension lds) into LDS at dst[r * DR + c * DC]: lane = row (two halves),
+// the four waves take interleaved columns, eight loads in flight per thread.
+template <typename T, int DR, int DC>
+__device__ inline void load_tile(const T* __restrict__ src, int64_t lds, int nr, int nc,
+                                 T* __restrict__ dst) {
+  const int w = threadIdx.x / 64;
+  for (int r = threadIdx.x % 64; r < nr; r += 64) {
+    for (int cb = w; cb < nc; cb += 32) {
+      const T* sp = src + r + (int64_t)cb * lds;
+      T v[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int idx = base + u * kWG;
-      v[u] = (idx < n) ? src(idx) : T(0);
-    }
+      for (int u = 0; u < 8; ++u) v[u] = (cb + 4 * u < nc) ? sp[(int64_t)(4 * u) * lds] : T(0);
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int idx = base + u * kWG;
-      if (idx < n) dst(idx, v[u]);
+      for (int u = 0; u < 8; ++u)
+        if (cb + 4 * u < nc) dst[r * DR + (cb + 4 * u) * DC] = v[u];
     }
   }
+}
+
+// The reverse copy: dst[r + c * ldd] = src[r * DR + c * DC].
+template <typename T, int DR, int DC>
+__device__ inline void store_tile(T* __restrict__ dst, int64_t ldd, int nr, int nc,
+                                  const T* __restrict__ src) {
+  const int w = threadIdx.x / 64;
+  for (int r = threadIdx.x % 64; r < nr; r += 64)
+    for (int c = w; c < nc; c += 4) dst[r + (int64_t)c * ldd] = src[r * DR + c * DC];
 }
 
 // Cross-lane moves inside rows of 16 lanes (DPP): quad_perm [1,0,3,2],
@@ -465,93 +475,169 @@ __device__ inline T wave_max(T v) {
   return fmax(fmax(rdlane(v, 0), rdlane(v, 16)), fmax(rdlane(v, 32), rdlane(v, 48)));
 }
 
-// LU with row pivoting of the nq x nq block D (LDS, leading dimension ld,
-// nq <= 128) by one wave: lane = row, rows lane and lane + 64. The pivot is
-// the first maximum |a| over rows >= k (strict >), as ggetrf!
-// (GenericBlasLapackFragments.jl:56-100); a zero pivot skips the interchange
-// and the scaling and sets *zero. Interchanges are applied to the ncols >= nq
-// columns starting at D (the block and the columns right of it); piv[k] is
-// the row swapped with row k at step k, perm[p] the original row at position
-// p, dinv[k] = 1 / U(k, k). The update of a row is issued eight columns at a
-// time so that its LDS loads overlap.
-template <typename T>
-__device__ void wave_getrf(T* D, int ld, int nq, int ncols, int pivot, int* piv, int* perm,
-                           T* dinv, int* zero) {
+// Wave-uniform minimum of v over the 64 lanes (all lanes active).
+__device__ inline int wave_min(int v) {
+  v = min(v, __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(0, v, 0x141, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(0, v, 0x140, 0xf, 0xf, false));
+  return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+             min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+// Leading dimension of the diagonal region in LDS (blocks are at most 128
+// columns wide); a compile-time stride turns every column offset into an
+// immediate LDS offset.
+constexpr int kRLD = 129;
+// Most columns of one piece (the SPD limit 90, rounded up).
+constexpr int kMaxNq = 96;
+
+// Pivoted LU of the nq x nq block D (LDS, leading dimension LD, nq <= 64) by
+// one wave, with rows that never move during the elimination: lane i owns
+// LDS row i of the block and tracks the position pos its row holds after the
+// interchanges so far. Step k takes the row of smallest position among those
+// holding the maximum |a| over positions >= k (the first maximum of ggetrf!,
+// GenericBlasLapackFragments.jl:56-100; a zero maximum keeps row k and skips
+// the scaling), reads the pivot row where it lies and updates the rows of
+// position > k, eight columns per batch of loads. At the end every row moves
+// to its position, together with the columns [nq, ncols) of D. piv[k] is the
+// position swapped with k at step k, perm[p] the row now at position p,
+// dinv[k] = 1 / U(k, k); *zero is set by a zero pivot.
+template <typename T, int LD>
+__device__ void pos_getrf(T* D, int nq, int ncols, int* piv, int* perm, T* dinv, int* zero) {
   const int lane = threadIdx.x % 64;
-  for (int i = lane; i < nq; i += 64) perm[i] = i;
-  __builtin_amdgcn_wave_barrier();
+  const bool mine = lane < nq;
+  int pos = lane;
   for (int k = 0; k < nq; ++k) {
-    int kp = k;
-    if (pivot) {
-      // first row holding the maximum |a| (rows k + lane, then k + 64 + lane)
-      const int r1 = k + lane, r2 = k + 64 + lane;
-      const T a1 = (r1 < nq) ? fabs(D[r1 + k * ld]) : T(-1);
-      const T a2 = (r2 < nq) ? fabs(D[r2 + k * ld]) : T(-1);
-      const T m = wave_max(fmax(a1, a2));
-      const unsigned long long m1 = __ballot(r1 < nq && a1 == m);
-      const unsigned long long m2 = __ballot(r2 < nq && a2 == m);
-      if (m1)
-        kp = k + __ffsll((long long)m1) - 1;
-      else if (m2)
-        kp = k + 64 + __ffsll((long long)m2) - 1;
-      kp = __builtin_amdgcn_readfirstlane(kp);
+    const bool cand = mine && pos >= k;
+    const T v = mine ? D[lane + k * LD] : T(0);
+    const T a = cand ? fabs(v) : T(-1);
+    const T m = wave_max(a);
+    const bool tie = cand && a == m;
+    const unsigned long long msk = __ballot(tie);
+    int blane;
+    if (msk == 0ull) {  // no comparable candidate (NaN column): keep row k
+      blane = __ffsll((long long)__ballot(mine && pos == k)) - 1;
+    } else if (__popcll(msk) == 1) {
+      blane = __ffsll((long long)msk) - 1;
+    } else {
+      const int mp = wave_min(tie ? pos : (1 << 20));
+      blane = __ffsll((long long)__ballot(tie && pos == mp)) - 1;
     }
-    const T pv = D[kp + k * ld];
+    blane = __builtin_amdgcn_readfirstlane(blane);
+    const int kp = __builtin_amdgcn_readlane(pos, blane);
+    const T pv = rdlane(v, blane);
     const bool nonzero = pv != T(0);
+    if (nonzero && kp != k) {
+      if (mine && pos == k)
+        pos = kp;
+      else if (lane == blane)
+        pos = k;
+    }
+    const T inv = T(1) / pv;
     if (lane == 0) {
       piv[k] = kp;
+      dinv[k] = inv;
       if (!nonzero) *zero = 1;
-      if (kp != k && nonzero) {
-        int t = perm[k];
-        perm[k] = perm[kp];
-        perm[kp] = t;
+    }
+    const bool below = mine && pos > k;
+    const T l = nonzero ? v * inv : v;
+    if (below && nonzero) D[lane + k * LD] = l;
+    int j0 = k + 1;
+    for (; j0 + 8 <= nq; j0 += 8) {
+      const T* pr = D + blane + j0 * LD;
+      T* xr = D + lane + j0 * LD;
+      T u[8], x[8];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        u[jj] = pr[jj * LD];
+        x[jj] = xr[jj * LD];
+      }
+      if (below) {
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) xr[jj * LD] = x[jj] - l * u[jj];
       }
     }
-    if (kp != k && nonzero)
-      for (int c = lane; c < ncols; c += 64) {
-        T t = D[k + c * ld];
-        D[k + c * ld] = D[kp + c * ld];
-        D[kp + c * ld] = t;
-      }
+    for (; j0 < nq; ++j0)
+      if (below) D[lane + j0 * LD] -= l * D[blane + j0 * LD];
     __builtin_amdgcn_wave_barrier();
-    const T inv = T(1) / D[k + k * ld];
-    if (lane == 0) dinv[k] = inv;
+  }
+  if (mine) perm[pos] = lane;
+  __builtin_amdgcn_wave_barrier();
+  // rows to their positions: position p takes LDS row perm[p]
+  const T* srow = D + (mine ? perm[lane] : 0);
+  T* drow = D + lane;
+  int c0 = 0;
+  for (; c0 + 8 <= ncols; c0 += 8) {
+    T x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = srow[(c0 + u) * LD];
+    __builtin_amdgcn_wave_barrier();
+    if (mine) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) drow[(c0 + u) * LD] = x[u];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  for (; c0 < ncols; ++c0) {
+    const T x = srow[c0 * LD];
+    __builtin_amdgcn_wave_barrier();
+    if (mine) drow[c0 * LD] = x;
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// The same factorisation without pivoting for SPD pieces of up to kMaxNq
+// columns (lane = row, rows lane and lane + 64): a zero pivot sets *zero and
+// skips the scaling; piv[k] = k.
+template <typename T, int LD>
+__device__ void nopiv_getrf(T* D, int nq, int* piv, T* dinv, int* zero) {
+  const int lane = threadIdx.x % 64;
+  for (int k = 0; k < nq; ++k) {
+    const T pv = D[k + k * LD];
+    const bool nonzero = pv != T(0);
+    const T inv = T(1) / pv;
+    if (lane == 0) {
+      piv[k] = k;
+      dinv[k] = inv;
+      if (!nonzero) *zero = 1;
+    }
     for (int i = k + 1 + lane; i < nq; i += 64) {
-      T l = D[i + k * ld];
+      T l = D[i + k * LD];
       if (nonzero) {
         l *= inv;
-        D[i + k * ld] = l;
+        D[i + k * LD] = l;
       }
-      for (int j0 = k + 1; j0 < nq; j0 += 8) {
+      int j0 = k + 1;
+      for (; j0 + 8 <= nq; j0 += 8) {
+        const T* pr = D + k + j0 * LD;
+        T* xr = D + i + j0 * LD;
         T u[8], x[8];
 #pragma unroll
         for (int jj = 0; jj < 8; ++jj) {
-          const int j = min(j0 + jj, nq - 1);
-          u[jj] = D[k + j * ld];
-          x[jj] = D[i + j * ld];
+          u[jj] = pr[jj * LD];
+          x[jj] = xr[jj * LD];
         }
 #pragma unroll
-        for (int jj = 0; jj < 8; ++jj)
-          if (j0 + jj < nq) D[i + (j0 + jj) * ld] = x[jj] - l * u[jj];
+        for (int jj = 0; jj < 8; ++jj) xr[jj * LD] = x[jj] - l * u[jj];
       }
+      for (; j0 < nq; ++j0) D[i + j0 * LD] -= l * D[k + j0 * LD];
     }
     __builtin_amdgcn_wave_barrier();
   }
 }
 
-// Most columns of one piece (the SPD limit 90, rounded up).
-constexpr int kMaxNq = 96;
-
-// Diagonal region of one block per workgroup (see the section comment).
-// Dynamic LDS: bw * (bw + 1) elements.
-template <typename T>
+// Diagonal region of one block per workgroup (see the section comment);
+// NQ bounds the piece width. Dynamic LDS: 128 * kRLD elements.
+template <typename T, int NQ>
 __global__ void __launch_bounds__(kWG)
 blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ arena,
                 int64_t* __restrict__ ipvt, unsigned long long* __restrict__ first_fail) {
+  constexpr int LD = kRLD;
   extern __shared__ unsigned char smem_raw[];
-  __shared__ int piv[kMaxNq];
-  __shared__ int perm[kMaxNq];
-  __shared__ T dinv[kMaxNq];
+  __shared__ int piv[NQ];
+  __shared__ int perm[NQ];
+  __shared__ T dinv[NQ];
   __shared__ int s_zero;
   T* R = reinterpret_cast<T*>(smem_raw);
   const int tid = threadIdx.x, wave = tid / 64;
@@ -559,12 +645,9 @@ blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ aren
   const int f = d.front;
   const int L = s.fL[f];
   const int64_t fc0 = s.fc0[f];
-  const int c0 = d.c0, bw = d.c1 - d.c0, ld = bw + 1;
-  T* F = arena + s.foff[f];
-  T* Fr = F + c0 + (int64_t)c0 * L;
-  copy8<T>(
-      bw * bw, [&](int idx) { return Fr[idx % bw + (int64_t)(idx / bw) * L]; },
-      [&](int idx, T v) { R[idx % bw + (idx / bw) * ld] = v; });
+  const int c0 = d.c0, bw = d.c1 - d.c0;
+  T* Fr = arena + s.foff[f] + c0 + (int64_t)c0 * L;
+  load_tile<T, 1, LD>(Fr, L, bw, bw, R);
   __syncthreads();
   for (int pi = 0; pi < d.np; ++pi) {
     int64_t pc0, nq64;
@@ -572,12 +655,15 @@ blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ aren
     const int nq = (int)nq64;
     const int o = (int)(pc0 - fc0) - c0;  // piece offset inside the region
     const int rest = bw - o - nq;         // region rows below / columns right
-    T* D = R + o + o * ld;
+    T* D = R + o + o * LD;
     if (wave == 0) {
       if (tid == 0) s_zero = 0;
       __builtin_amdgcn_wave_barrier();
       // interchanges reach the region columns right of the piece too
-      wave_getrf<T>(D, ld, nq, nq + rest, s.pivot, piv, perm, dinv, &s_zero);
+      if (s.pivot)
+        pos_getrf<T, LD>(D, nq, nq + rest, piv, perm, dinv, &s_zero);
+      else
+        nopiv_getrf<T, LD>(D, nq, piv, dinv, &s_zero);
     }
     __syncthreads();
     if (tid < nq) ipvt[pc0 + tid] = piv[tid] + 1;
@@ -586,11 +672,12 @@ blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ aren
       // U12 of the region (rows already interchanged): unit-lower solve,
       // u_i = a_i - sum_k<i L(i, k) u_k
       for (int c = tid - 128; c < rest; c += 128) {
-        T* u = D + (nq + c) * ld;
+        T* u = D + (nq + c) * LD;
         for (int i = 1; i < nq; ++i) {
           T v = u[i];
+          const T* li = D + i;
 #pragma unroll 8
-          for (int kk = 0; kk < i; ++kk) v -= D[i + kk * ld] * u[kk];
+          for (int kk = 0; kk < i; ++kk) v -= li[kk * LD] * u[kk];
           u[i] = v;
         }
       }
@@ -599,10 +686,11 @@ blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ aren
       for (int r = tid; r < rest; r += 128) {
         T* x = D + nq + r;
         for (int j = 0; j < nq; ++j) {
-          T v = x[j * ld];
+          T v = x[j * LD];
+          const T* uj = D + j * LD;
 #pragma unroll 8
-          for (int k = 0; k < j; ++k) v -= x[k * ld] * D[k + j * ld];
-          x[j * ld] = v * dinv[j];
+          for (int k = 0; k < j; ++k) v -= x[k * LD] * uj[k];
+          x[j * LD] = v * dinv[j];
         }
       }
     }
@@ -610,16 +698,21 @@ blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ aren
     if (rest > 0) {
       // rest x rest update, 4 x 4 entries per thread and pass
       const int nb = (rest + 3) / 4;
-      const T* Lb = D + nq;           // L21 rows, leading dimension ld
-      const T* Ub = D + nq * ld;      // U12 columns
-      T* C = D + nq + nq * ld;
+      const T* Lb = D + nq;       // L21 rows
+      const T* Ub = D + nq * LD;  // U12 columns
+      T* C = D + nq + nq * LD;
       for (int idx = tid; idx < nb * nb; idx += kWG) {
-        const int i0 = (idx % nb) * 4, j0 = (idx / nb) * 4;
-        int ir[4], jc[4];
+        // tiles at the edge shift back to stay inside (rest >= 4) and then
+        // write only their own entries; smaller regions clamp their reads
+        const int ib = (idx % nb) * 4, jb = (idx / nb) * 4;
+        const int i0 = rest >= 4 ? min(ib, rest - 4) : 0;
+        const int j0 = rest >= 4 ? min(jb, rest - 4) : 0;
+        const T* lp[4];
+        const T* up[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-          ir[a] = min(i0 + a, rest - 1);
-          jc[a] = min(j0 + a, rest - 1) * ld;
+          lp[a] = Lb + min(i0 + a, rest - 1);
+          up[a] = Ub + min(j0 + a, rest - 1) * LD;
         }
         T acc[4][4];
 #pragma unroll
@@ -630,8 +723,8 @@ blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ aren
           T lv[4], uv[4];
 #pragma unroll
           for (int a = 0; a < 4; ++a) {
-            lv[a] = Lb[ir[a] + k * ld];
-            uv[a] = Ub[k + jc[a]];
+            lv[a] = lp[a][k * LD];
+            uv[a] = up[a][k];
           }
 #pragma unroll
           for (int a = 0; a < 4; ++a)
@@ -641,28 +734,27 @@ blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ aren
 #pragma unroll
         for (int c = 0; c < 4; ++c)
 #pragma unroll
-          for (int a = 0; a < 4; ++a)
-            if (i0 + a < rest && j0 + c < rest) C[(i0 + a) + (j0 + c) * ld] -= acc[a][c];
+          for (int a = 0; a < 4; ++a) {
+            const int i = i0 + a, j = j0 + c;
+            if (i >= ib && j >= jb && i < rest && j < rest) C[i + j * LD] -= acc[a][c];
+          }
       }
     }
     __syncthreads();
   }
-  for (int idx = tid; idx < bw * bw; idx += kWG) {
-    const int r = idx % bw, c = idx / bw;
-    Fr[r + (int64_t)c * L] = R[r + c * ld];
-  }
+  store_tile<T, 1, LD>(Fr, L, bw, bw, R);
 }
 
 // Off-diagonal parts of one block after blk_diag_kernel (see the section
-// comment). Workgroups b < d.nl take 64 rows below the block, the others 64
-// columns right of it. Dynamic LDS: 64 * (bw + 1) + the largest piece strip
-// (o + nq) * nq of the block, in elements.
-template <typename T>
+// comment); NQ bounds the piece width. Workgroups b < d.nl take 64 rows
+// below the block, the others 64 columns right of it. Dynamic LDS:
+// 64 * kRLD + 128 * NQ elements.
+template <typename T, int NQ>
 __global__ void __launch_bounds__(kWG)
 blk_offdiag_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
                    T* __restrict__ arena, const int64_t* __restrict__ ipvt) {
   extern __shared__ unsigned char smem_raw[];
-  __shared__ int spiv[kMaxNq];
+  __shared__ int spiv[NQ];
   T* S = reinterpret_cast<T*>(smem_raw);
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const int di = find_blk(desc, ndesc, blockIdx.x);
@@ -677,99 +769,91 @@ blk_offdiag_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
     // rows [r0, r0 + nr): X(i, k) at X[i + 64 k]
     const int r0 = c1 + b * 64, nr = min(64, L - r0);
     T* X = S;
-    T* Us = S + 64 * (bw + 1);  // strip of the piece's columns: Us[k * nq + j]
-    const T* Fx = F + r0 + (int64_t)c0 * L;
-    copy8<T>(
-        64 * bw,
-        [&](int idx) { return (idx % 64 < nr) ? Fx[idx % 64 + (int64_t)(idx / 64) * L] : T(0); },
-        [&](int idx, T v) { X[idx] = v; });
+    T* Us = S + 64 * kRLD;  // strip of the piece's columns: Us[k * NQ + j]
+    T* Fx = F + r0 + (int64_t)c0 * L;
+    for (int idx = tid; idx < 64 * bw; idx += kWG) X[idx] = T(0);
+    __syncthreads();
+    load_tile<T, 1, 64>(Fx, L, nr, bw, X);
     for (int pi = 0; pi < d.np; ++pi) {
       int64_t pc0, nq64;
       piece_cols(s, d.piece + pi, pc0, nq64);
       const int nq = (int)nq64;
       const int o = (int)(pc0 - fc0) - c0;
-      const int w = o + nq;
-      const T* Fu = F + c0 + (int64_t)(c0 + o) * L;
-      copy8<T>(
-          w * nq, [&](int idx) { return Fu[idx % w + (int64_t)(idx / w) * L]; },
-          [&](int idx, T v) { Us[(idx % w) * nq + idx / w] = v; });
+      load_tile<T, NQ, 1>(F + c0 + (int64_t)(c0 + o) * L, L, o + nq, nq, Us);
       __syncthreads();
       if (o > 0) {  // earlier pieces of the block: X_q -= X_<q U_<q,q
         for (int cb = wave * 8; cb < nq; cb += 32) {
           T acc[8];
 #pragma unroll
           for (int jj = 0; jj < 8; ++jj) acc[jj] = T(0);
+          const T* xr = X + lane;
+          const T* ur = Us + min(cb, nq - 8 >= 0 ? nq - 8 : 0);
           for (int k = 0; k < o; ++k) {
-            const T xv = X[lane + k * 64];
-            const T* ur = Us + k * nq;
+            const T xv = xr[k * 64];
 #pragma unroll
-            for (int jj = 0; jj < 8; ++jj) acc[jj] += xv * ur[min(cb + jj, nq - 1)];
+            for (int jj = 0; jj < 8; ++jj) acc[jj] += xv * ur[k * NQ + jj];
           }
+          const int cs = min(cb, nq - 8 >= 0 ? nq - 8 : 0);  // shifted start
 #pragma unroll
-          for (int jj = 0; jj < 8; ++jj)
-            if (cb + jj < nq) X[lane + (o + cb + jj) * 64] -= acc[jj];
+          for (int jj = 0; jj < 8; ++jj) {
+            const int j = cs + jj;
+            if (j >= cb && j < nq) X[lane + (o + j) * 64] -= acc[jj];
+          }
         }
         __syncthreads();
       }
       if (wave == 0) {  // X_q U_qq^-1, one row per lane
         T* xr = X + lane + o * 64;
-        const T* uq = Us + o * nq;
+        const T* uq = Us + o * NQ;
         for (int j = 0; j < nq; ++j) {
           T v = xr[j * 64];
 #pragma unroll 8
-          for (int k = 0; k < j; ++k) v -= xr[k * 64] * uq[k * nq + j];
-          xr[j * 64] = v * (T(1) / uq[j * nq + j]);
+          for (int k = 0; k < j; ++k) v -= xr[k * 64] * uq[k * NQ + j];
+          xr[j * 64] = v * (T(1) / uq[j * NQ + j]);
         }
       }
       __syncthreads();
     }
-    for (int idx = tid; idx < 64 * bw; idx += kWG) {
-      const int i = idx % 64, k = idx / 64;
-      if (i < nr) F[(r0 + i) + (int64_t)(c0 + k) * L] = X[idx];
-    }
+    store_tile<T, 1, 64>(Fx, L, nr, bw, X);
   } else {
-    // columns [j0, j0 + nc): A(r, c) at A[r + c * lda]
+    // columns [j0, j0 + nc): A(r, c) at A[r + c * kRLD]
     const int j0 = c1 + (b - d.nl) * 64, nc = min(64, L - j0);
-    const int lda = bw + 1;
     T* A = S;
-    T* Ls = S + 64 * lda;  // the piece's rows of L: Ls[i + k * nq], k < o + nq
-    const T* Fa = F + c0 + (int64_t)j0 * L;
-    copy8<T>(
-        bw * 64,
-        [&](int idx) { return (idx / bw < nc) ? Fa[idx % bw + (int64_t)(idx / bw) * L] : T(0); },
-        [&](int idx, T v) { A[idx % bw + (idx / bw) * lda] = v; });
+    T* Ls = S + 64 * kRLD;  // the piece's rows of L: Ls[i + k * NQ], k < o + nq
+    T* Fa = F + c0 + (int64_t)j0 * L;
+    for (int idx = tid; idx < 64 * kRLD; idx += kWG) A[idx] = T(0);
+    __syncthreads();
+    load_tile<T, 1, kRLD>(Fa, L, bw, nc, A);
     for (int pi = 0; pi < d.np; ++pi) {
       int64_t pc0, nq64;
       piece_cols(s, d.piece + pi, pc0, nq64);
       const int nq = (int)nq64;
       const int o = (int)(pc0 - fc0) - c0;
-      const int w = o + nq;
-      const T* Fl = F + c0 + o + (int64_t)c0 * L;
-      copy8<T>(
-          nq * w, [&](int idx) { return Fl[idx % nq + (int64_t)(idx / nq) * L]; },
-          [&](int idx, T v) { Ls[idx] = v; });
+      load_tile<T, 1, NQ>(F + c0 + o + (int64_t)c0 * L, L, nq, o + nq, Ls);
       if (tid < nq) spiv[tid] = (int)ipvt[pc0 + tid] - 1;
       __syncthreads();
       if (o > 0) {  // earlier pieces: A_q -= L_q,<q A_<q (rows not yet swapped)
-        T* ac = A + lane * lda;
+        T* ac = A + lane * kRLD;
         for (int ib = wave * 8; ib < nq; ib += 32) {
+          const int is = min(ib, nq - 8 >= 0 ? nq - 8 : 0);
           T acc[8];
 #pragma unroll
           for (int ii = 0; ii < 8; ++ii) acc[ii] = T(0);
           for (int k = 0; k < o; ++k) {
             const T av = ac[k];
-            const T* lk = Ls + k * nq;
 #pragma unroll
-            for (int ii = 0; ii < 8; ++ii) acc[ii] += lk[min(ib + ii, nq - 1)] * av;
+            for (int ii = 0; ii < 8; ++ii) acc[ii] += Ls[is + ii + k * NQ] * av;
           }
 #pragma unroll
-          for (int ii = 0; ii < 8; ++ii)
-            if (ib + ii < nq) ac[o + ib + ii] -= acc[ii];
+          for (int ii = 0; ii < 8; ++ii) {
+            const int i = is + ii;
+            if (i >= ib && i < nq) ac[o + i] -= acc[ii];
+          }
         }
         __syncthreads();
       }
       if (wave == 0) {  // interchanges, then L_qq^-1, one column per lane
-        T* a = A + lane * lda + o;
+        T* a = A + lane * kRLD + o;
         for (int k = 0; k < nq; ++k) {
           const int kp = spiv[k];
           if (kp != k) {
@@ -778,20 +862,17 @@ blk_offdiag_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
             a[kp] = t;
           }
         }
-        const T* lq = Ls + o * nq;
+        const T* lq = Ls + o * NQ;
         for (int i = 1; i < nq; ++i) {
           T v = a[i];
 #pragma unroll 8
-          for (int kk = 0; kk < i; ++kk) v -= lq[i + kk * nq] * a[kk];
+          for (int kk = 0; kk < i; ++kk) v -= lq[i + kk * NQ] * a[kk];
           a[i] = v;
         }
       }
       __syncthreads();
     }
-    for (int idx = tid; idx < bw * 64; idx += kWG) {
-      const int r = idx % bw, c = idx / bw;
-      if (c < nc) F[(c0 + r) + (int64_t)(j0 + c) * L] = A[r + c * lda];
-    }
+    store_tile<T, 1, kRLD>(Fa, L, bw, nc, A);
   }
 }
 
