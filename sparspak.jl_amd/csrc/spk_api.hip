@@ -125,6 +125,7 @@ struct spkhip_handle {
       d_relmap, d_small, d_it_front, d_it_col, d_ea_child, d_ea_t, d_solve;
   DevBuf<i64> d_foff, d_fcb, d_frel, d_fvec, d_it_ea, d_lcol, d_ucol;
   DevBuf<BlkDesc> d_bdesc;
+  DevBuf<spk::GemmDesc> d_gdesc;
   DevBuf<i32> d_large, d_ut_front, d_ut_piece, d_ut_r0;
   DevBuf<spk::SolveDesc> d_sdesc;
   DevSym ds{};
@@ -165,7 +166,6 @@ struct spkhip_handle {
     int level, grid;
   };
   int cur_level = -1, cur_grid = 0;   // tags of the next profiled launches
-  bool gemm_valu = false;              // trailing update on VALU instead of MFMA
   std::vector<Rec> recs;
   ~spkhip_handle() {
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
@@ -360,7 +360,8 @@ void build_fronts(spkhip_handle* h) {
 void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_front,
                 std::vector<i32>& it_col, std::vector<i64>& it_ea,
                 std::vector<i32>& ea_child, std::vector<i32>& ea_t,
-                std::vector<BlkDesc>& bdesc, std::vector<i32>& solve, std::vector<i32>& largel,
+                std::vector<BlkDesc>& bdesc, std::vector<spk::GemmDesc>& gdesc,
+                std::vector<i32>& solve, std::vector<i32>& largel,
                 std::vector<spk::SolveDesc>& sdesc, std::vector<i32>& ut_front,
                 std::vector<i32>& ut_piece, std::vector<i32>& ut_r0) {
   const i32 nf = h->nfront;
@@ -518,21 +519,36 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         G.offd.elems += 4.0 * (double)bw * t;
       }
       G.offd.cnt = (int)bdesc.size() - G.offd.off;
-      G.gemm.off = (int)bdesc.size();
+      // trailing updates, blocks taken in pairs (see the kernel file):
+      // first of a pair -> border of the second; second -> the rest with
+      // both blocks; an unpaired last block -> the rest with itself
+      G.gemm.off = (int)gdesc.size();
+      auto rect = [&](i32 f, i32 k0, i32 k1, i32 r0, i32 r1, i32 j0, i32 j1) {
+        if (r1 <= r0 || j1 <= j0 || k1 <= k0) return;
+        const i32 ntr = (r1 - r0 + 127) / 128, ntc = (j1 - j0 + 127) / 128;
+        gdesc.push_back(spk::GemmDesc{f, k0, k1, r0, r1 - r0, j0, j1 - j0, G.gemm.wg, ntr});
+        G.gemm.wg += ntr * ntc;
+        const double m = r1 - r0, nn = j1 - j0, kk = k1 - k0;
+        G.gemm.flops += 2.0 * m * nn * kk;
+        G.gemm.elems += 2.0 * m * nn + (m + nn) * kk;
+      };
       for (size_t fi = 0; fi < large.size(); ++fi) {
         if (gi >= fblocks[fi].size()) continue;
         const Blk& B = fblocks[fi][gi];
         const i32 f = large[fi];
-        const i32 t = h->fL[f] - B.c1;
-        if (t <= 0) continue;
-        const i32 nt = (t + 127) / 128;
-        bdesc.push_back(BlkDesc{f, B.p0, B.np, B.c0, B.c1, G.gemm.wg, nt, 0});
-        G.gemm.wg += nt * nt;
-        const double td = t, bw = B.c1 - B.c0;
-        G.gemm.flops += 2.0 * td * td * bw;
-        G.gemm.elems += 2.0 * td * td + 2.0 * td * bw;
+        const i32 L = h->fL[f];
+        if (gi % 2 == 0 && gi + 1 < fblocks[fi].size()) {
+          const Blk& N = fblocks[fi][gi + 1];
+          rect(f, B.c0, B.c1, B.c1, L, N.c0, N.c1);
+          rect(f, B.c0, B.c1, N.c0, N.c1, N.c1, L);
+        } else if (gi % 2 == 1) {
+          const Blk& P = fblocks[fi][gi - 1];
+          rect(f, P.c0, B.c1, B.c1, L, B.c1, L);
+        } else {
+          rect(f, B.c0, B.c1, B.c1, L, B.c1, L);
+        }
       }
-      G.gemm.cnt = (int)bdesc.size() - G.gemm.off;
+      G.gemm.cnt = (int)gdesc.size() - G.gemm.off;
       lp.groups.push_back(G);
     }
     // blocked solve steps over the same blocks
@@ -740,12 +756,8 @@ void run_factor(spkhip_handle* h) {
         if (G.gemm.cnt > 0) {
           h->cur_grid = G.gemm.wg;
           Prof pr(h, SPKHIP_K_UPDATE, G.gemm.flops, G.gemm.elems * sizeof(T));
-          if (h->gemm_valu)
-            hipLaunchKernelGGL(spk::blk_gemm_kernel<T>, dim3(G.gemm.wg), dim3(kWG), 0, st,
-                               h->ds, h->d_bdesc.p + G.gemm.off, G.gemm.cnt, arena.p);
-          else
-            hipLaunchKernelGGL(spk::blk_gemm_mfma_kernel<T>, dim3(G.gemm.wg), dim3(kWG), 0, st,
-                               h->ds, h->d_bdesc.p + G.gemm.off, G.gemm.cnt, arena.p);
+          hipLaunchKernelGGL(spk::blk_gemm_kernel<T>, dim3(G.gemm.wg), dim3(kWG), 0, st, h->ds,
+                             h->d_gdesc.p + G.gemm.off, G.gemm.cnt, arena.p);
           hipck(hipGetLastError());
         }
       }
@@ -940,7 +952,8 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   std::vector<spk::SolveDesc> sdesc;
   std::vector<i64> it_ea;
   std::vector<BlkDesc> bdesc;
-  build_plan(h, small, it_front, it_col, it_ea, ea_child, ea_t, bdesc, solve, largel, sdesc,
+  std::vector<spk::GemmDesc> gdesc;
+  build_plan(h, small, it_front, it_col, it_ea, ea_child, ea_t, bdesc, gdesc, solve, largel, sdesc,
              ut_front, ut_piece, ut_r0);
 
   if (largel.empty()) largel.push_back(0);
@@ -1000,7 +1013,9 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   h->d_ea_child.upload(ea_child, st);
   h->d_ea_t.upload(ea_t, st);
   if (bdesc.empty()) bdesc.push_back(BlkDesc{});
+  if (gdesc.empty()) gdesc.push_back(spk::GemmDesc{});
   h->d_bdesc.upload(bdesc, st);
+  h->d_gdesc.upload(gdesc, st);
   if (solve.empty()) solve.push_back(0);
   h->d_solve.upload(solve, st);
   h->d_large.upload(largel, st);
@@ -1364,9 +1379,6 @@ int64_t spkhip_factor(spkhip_t h, int64_t precision) {
     hipck(hipSetDevice(h->device));
     hipck(hipEventRecord(h->ev0, h->stream));
     const bool lu = h->kind == SPKHIP_KIND_LU;
-    // tuning knob: SPKHIP_GEMM_VALU=1 runs the trailing update on the
-    // vector ALUs instead of the matrix cores
-    h->gemm_valu = std::getenv("SPKHIP_GEMM_VALU") != nullptr;
     if (precision == SPKHIP_FP64) {
       run_assemble<double>(h);
       run_factor<double>(h);

@@ -77,6 +77,18 @@ struct BlkDesc {
   int32_t slot;      // stage slot of the (first) piece
 };
 
+// Per-launch descriptor of the trailing-update kernel: one rectangle of one
+// front, F[r0:r0+nr, j0:j0+nc] -= F[r0:r0+nr, k0:k1] F[k0:k1, j0:j0+nc],
+// in 128 x 128 tiles.
+struct GemmDesc {
+  int32_t front;
+  int32_t k0, k1;    // inner dimension: front columns / rows [k0, k1)
+  int32_t r0, nr;    // output rows
+  int32_t j0, nc;    // output columns
+  int32_t wg_begin;  // prefix sum of workgroups over the launch
+  int32_t ntr;       // row tiles
+};
+
 // Per-workgroup descriptor of the blocked solve kernels.
 struct SolveDesc {
   int32_t front;

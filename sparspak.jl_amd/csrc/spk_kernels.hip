@@ -374,7 +374,9 @@ large_writeback_kernel(DevSym s, const int32_t* __restrict__ it_front,
   for (int r = cp + threadIdx.x; r < L; r += kWG) lc[r] = F[r];
 }
 
-__device__ inline int find_blk(const BlkDesc* d, int nd, int b) {
+// Descriptor of workgroup b: the last one whose wg_begin <= b.
+template <typename Desc>
+__device__ inline int find_blk(const Desc* d, int nd, int b) {
   int lo = 0, hi = nd - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
@@ -407,8 +409,13 @@ __device__ inline int find_blk(const BlkDesc* d, int nd, int b) {
 //                      earlier pieces' contribution, the piece's interchanges
 //                      and its unit-lower solve — the left-looking order of
 //                      the reference, so every entry is the reference's.
-//  blk_gemm_*_kernel   trailing update A22 -= L21 U12 with inner dimension
-//                      bw instead of one piece width.
+//  blk_gemm_kernel     trailing update A22 -= L21 U12. Blocks are taken in
+//                      pairs: after the first block of a pair only the
+//                      border of the second (its columns, and its rows right
+//                      of it) is updated; after the second, the rest of the
+//                      trailing matrix receives both blocks at once, with
+//                      inner dimension up to 256 — half the read-modify-write
+//                      traffic of one update per block.
 // ---------------------------------------------------------------------------
 
 // Copy of an nr x nc tile (nr <= 128) from a column-major HBM array (leading
@@ -587,6 +594,101 @@ __device__ void pos_getrf(T* D, int nq, int ncols, int* piv, int* perm, T* dinv,
   }
 }
 
+// pos_getrf with the whole workgroup: every wave runs the same pivot search
+// (identical inputs, identical result) and updates the columns j = k + 1 +
+// wave + 4 i of step k, one barrier per step. The multipliers of column k
+// are written by wave 0 one step later, when no wave reads that column any
+// more. Must be called by all threads of the workgroup.
+template <typename T, int LD>
+__device__ void wg_getrf(T* D, int nq, int ncols, int* piv, int* perm, T* dinv, int* zero) {
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const bool mine = lane < nq;
+  int pos = lane;
+  T lprev = T(0);
+  bool wprev = false;
+  for (int k = 0; k < nq; ++k) {
+    const bool cand = mine && pos >= k;
+    const T v = mine ? D[lane + k * LD] : T(0);
+    if (wave == 0 && wprev) D[lane + (k - 1) * LD] = lprev;
+    const T a = cand ? fabs(v) : T(-1);
+    const T m = wave_max(a);
+    const bool tie = cand && a == m;
+    const unsigned long long msk = __ballot(tie);
+    int blane;
+    if (msk == 0ull) {  // no comparable candidate (NaN column): keep row k
+      blane = __ffsll((long long)__ballot(mine && pos == k)) - 1;
+    } else if (__popcll(msk) == 1) {
+      blane = __ffsll((long long)msk) - 1;
+    } else {
+      const int mp = wave_min(tie ? pos : (1 << 20));
+      blane = __ffsll((long long)__ballot(tie && pos == mp)) - 1;
+    }
+    blane = __builtin_amdgcn_readfirstlane(blane);
+    const int kp = __builtin_amdgcn_readlane(pos, blane);
+    const T pv = rdlane(v, blane);
+    const bool nonzero = pv != T(0);
+    if (nonzero && kp != k) {
+      if (mine && pos == k)
+        pos = kp;
+      else if (lane == blane)
+        pos = k;
+    }
+    const T inv = T(1) / pv;
+    if (threadIdx.x == 0) {
+      piv[k] = kp;
+      dinv[k] = inv;
+      if (!nonzero) *zero = 1;
+    }
+    const bool below = mine && pos > k;
+    const T l = nonzero ? v * inv : v;
+    lprev = l;
+    wprev = below && nonzero;
+    const int j1 = k + 1 + wave;
+    int j = j1;
+    for (; j + 28 < nq; j += 32) {
+      const T* pr = D + blane + j * LD;
+      T* xr = D + lane + j * LD;
+      T u[8], x[8];
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        u[jj] = pr[4 * jj * LD];
+        x[jj] = xr[4 * jj * LD];
+      }
+      if (below) {
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) xr[4 * jj * LD] = x[jj] - l * u[jj];
+      }
+    }
+    for (; j < nq; j += 4)
+      if (below) D[lane + j * LD] -= l * D[blane + j * LD];
+    __syncthreads();
+  }
+  if (wave == 0 && wprev) D[lane + (nq - 1) * LD] = lprev;
+  if (wave == 0 && mine) perm[pos] = lane;
+  __syncthreads();
+  // rows to their positions (position p takes LDS row perm[p]), the waves
+  // taking interleaved columns; all reads of a batch precede its writes
+  const T* srow = D + (mine ? perm[lane] : 0);
+  T* drow = D + lane;
+  for (int c0 = 0; c0 < ncols; c0 += 32) {
+    T x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int c = c0 + wave + 4 * u;
+      x[u] = (c < ncols) ? srow[c * LD] : T(0);
+    }
+    __syncthreads();
+    if (mine) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int c = c0 + wave + 4 * u;
+        if (c < ncols) drow[c * LD] = x[u];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // The same factorisation without pivoting for SPD pieces of up to kMaxNq
 // columns (lane = row, rows lane and lane + 64): a zero pivot sets *zero and
 // skips the scaling; piv[k] = k.
@@ -656,14 +758,13 @@ blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ aren
     const int o = (int)(pc0 - fc0) - c0;  // piece offset inside the region
     const int rest = bw - o - nq;         // region rows below / columns right
     T* D = R + o + o * LD;
-    if (wave == 0) {
-      if (tid == 0) s_zero = 0;
-      __builtin_amdgcn_wave_barrier();
-      // interchanges reach the region columns right of the piece too
-      if (s.pivot)
-        pos_getrf<T, LD>(D, nq, nq + rest, piv, perm, dinv, &s_zero);
-      else
-        nopiv_getrf<T, LD>(D, nq, piv, dinv, &s_zero);
+    if (tid == 0) s_zero = 0;
+    __syncthreads();
+    // interchanges reach the region columns right of the piece too
+    if (s.pivot && nq <= 64) {
+      wg_getrf<T, LD>(D, nq, nq + rest, piv, perm, dinv, &s_zero);
+    } else if (wave == 0) {
+      nopiv_getrf<T, LD>(D, nq, piv, dinv, &s_zero);
     }
     __syncthreads();
     if (tid < nq) ipvt[pc0 + tid] = piv[tid] + 1;
@@ -876,73 +977,8 @@ blk_offdiag_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
   }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kWG)
-blk_gemm_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
-                T* __restrict__ arena) {
-  constexpr int TM = 128, KC = 16;
-  __shared__ T As[KC][TM];
-  __shared__ T Bs[KC][TM];
-  const int tid = threadIdx.x;
-  const int di = find_blk(desc, ndesc, blockIdx.x);
-  const BlkDesc d = desc[di];
-  const int b = blockIdx.x - d.wg_begin;
-  const int f = d.front;
-  const int L = s.fL[f];
-  const int64_t fc0 = s.fc0[f];
-  const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
-  T* F = arena + s.foff[f];
-  const int nt = d.nl;
-  const int ti = b % nt, tj = b / nt;
-  const int r0 = c1 + ti * TM, j0 = c1 + tj * TM;
-  const int tx = tid % 16, ty = tid / 16;
-  T acc[8][8];
-#pragma unroll
-  for (int a = 0; a < 8; ++a)
-#pragma unroll
-    for (int c = 0; c < 8; ++c) acc[a][c] = T(0);
-  for (int k0 = 0; k0 < bw; k0 += KC) {
-    const int kc = min(KC, bw - k0);
-    for (int idx = tid; idx < KC * TM; idx += kWG) {
-      int i = idx % TM, k = idx / TM;
-      int r = r0 + i;
-      As[k][i] = (k < kc && r < L) ? F[r + (int64_t)(c0 + k0 + k) * L] : T(0);
-    }
-    for (int idx = tid; idx < KC * TM; idx += kWG) {
-      int k = idx % KC, j = idx / KC;
-      int c = j0 + j;
-      Bs[k][j] = (k < kc && c < L) ? F[(c0 + k0 + k) + (int64_t)c * L] : T(0);
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int k = 0; k < KC; ++k) {
-      T av[8], bv[8];
-#pragma unroll
-      for (int a = 0; a < 8; ++a) av[a] = As[k][tx * 8 + a];
-#pragma unroll
-      for (int c = 0; c < 8; ++c) bv[c] = Bs[k][ty * 8 + c];
-#pragma unroll
-      for (int a = 0; a < 8; ++a)
-#pragma unroll
-        for (int c = 0; c < 8; ++c) acc[a][c] += av[a] * bv[c];
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const int col = j0 + ty * 8 + c;
-    if (col >= L) continue;
-    T* fc = F + (int64_t)col * L;
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      const int r = r0 + tx * 8 + a;
-      if (r < L) fc[r] -= acc[a][c];
-    }
-  }
-}
-
-// The same trailing update on the matrix cores: 128 x 128 tiles, each of
-// the four waves owns a 64 x 64 quadrant as 4 x 4 MFMA 16x16x4 tiles. The
+// The trailing update on the matrix cores: 128 x 128 tiles, each of the
+// four waves owns a 64 x 64 quadrant as 4 x 4 MFMA 16x16x4 tiles. The
 // operands are passed as (U^T, L^T) so that the accumulator holds the tile
 // transposed: lane l keeps row l % 16 of four columns, and each group of 16
 // lanes writes 16 contiguous rows of the column-major front.
@@ -970,43 +1006,41 @@ struct Mfma16<float> {
 
 template <typename T>
 __global__ void __launch_bounds__(kWG, 2)
-blk_gemm_mfma_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
-                     T* __restrict__ arena) {
+blk_gemm_kernel(DevSym s, const GemmDesc* __restrict__ desc, int ndesc,
+                T* __restrict__ arena) {
   constexpr int TM = 128, KC = 16, KP = KC + 1, PER = TM * KC / kWG;  // 8 loads each
   typedef typename Mfma16<T>::v4 v4;
-  // double-buffered: As[buf][i * KP + k] = L(r0 + i, c0 + k0 + k),
-  //                  Bs[buf][j * KP + k] = U(c0 + k0 + k, j0 + j)
+  // double-buffered: As[buf][i * KP + k] = L(r0 + i, k0 + k),
+  //                  Bs[buf][j * KP + k] = U(k0 + k, j0 + j)
   __shared__ T As[2][TM * KP];
   __shared__ T Bs[2][TM * KP];
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const int di = find_blk(desc, ndesc, blockIdx.x);
-  const BlkDesc d = desc[di];
+  const GemmDesc d = desc[di];
   const int b = blockIdx.x - d.wg_begin;
-  const int f = d.front;
-  const int L = s.fL[f];
-  const int64_t fc0 = s.fc0[f];
-  const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
-  T* F = arena + s.foff[f];
-  const int nt = d.nl;
-  const int ti0 = b % nt, tj0 = b / nt;
-  const int r0 = c1 + ti0 * TM, j0 = c1 + tj0 * TM;
+  const int L = s.fL[d.front];
+  T* F = arena + s.foff[d.front];
+  const int bw = d.k1 - d.k0;
+  const int rend = d.r0 + d.nr, cend = d.j0 + d.nc;
+  const int r0 = d.r0 + (b % d.ntr) * TM, j0 = d.j0 + (b / d.ntr) * TM;
   const int wr = (wave % 2) * 64, wc = (wave / 2) * 64;
   const int l16 = lane % 16, lk = lane / 16;
   // this thread's share of a chunk: A element (ai, ak + 2 t), B element
   // (bk, bj + 16 t) for t < 8
   const int ai = tid % TM, ak = tid / TM;       // ak in {0, 1}
   const int bk = tid % KC, bj = tid / KC;       // bj in 0..15
-  const bool arow = r0 + ai < L;
-  const T* Ap = F + (r0 + ai) + (int64_t)c0 * L;
-  const T* Bp = F + c0 + bk;
+  const bool arow = r0 + ai < rend;
+  const T* Ap = F + (r0 + ai) + (int64_t)d.k0 * L;
+  const T* Bp = F + d.k0 + bk;
   T ra[PER], rb[PER];
   auto gload = [&](int k0) {
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
       const int kk = ak + 2 * t;
-      ra[t] = (arow && k0 + kk < bw) ? Ap[(int64_t)(k0 + kk) * L] : T(0);
+      // L21 enters negated: the accumulators start from A22 itself
+      ra[t] = (arow && k0 + kk < bw) ? -Ap[(int64_t)(k0 + kk) * L] : T(0);
       const int c = j0 + bj + 16 * t;
-      rb[t] = (k0 + bk < bw && c < L) ? Bp[k0 + (int64_t)c * L] : T(0);
+      rb[t] = (k0 + bk < bw && c < cend) ? Bp[k0 + (int64_t)c * L] : T(0);
     }
   };
   auto lstore = [&](int buf) {
@@ -1016,11 +1050,20 @@ blk_gemm_mfma_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
       Bs[buf][(bj + 16 * t) * KP + bk] = rb[t];
     }
   };
+  // accumulators start as this lane's A22 entries, loaded together with
+  // the first chunk, so the epilogue is stores only
   v4 acc[4][4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 4; ++a) {
+    const int r = r0 + wr + a * 16 + l16;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[a][c] = v4{T(0), T(0), T(0), T(0)};
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int col = j0 + wc + c * 16 + Mfma16<T>::row(lk, v);
+        acc[a][c][v] = (r < rend && col < cend) ? F[r + (int64_t)col * L] : T(0);
+      }
+  }
   const int nch = (bw + KC - 1) / KC;
   gload(0);
   lstore(0);
@@ -1048,13 +1091,13 @@ blk_gemm_mfma_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     const int r = r0 + wr + a * 16 + l16;
-    if (r >= L) continue;
+    if (r >= rend) continue;
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int col = j0 + wc + c * 16 + Mfma16<T>::row(lk, v);
-        if (col < L) F[r + (int64_t)col * L] -= acc[a][c][v];
+        if (col < cend) F[r + (int64_t)col * L] = acc[a][c][v];
       }
   }
 }
