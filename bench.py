@@ -180,6 +180,14 @@ def main():
                 "launches": dk["launches"], "avg_launch_ms": dk["ms"] / dk["launches"],
                 "work_per_launch": dk["bytes"] / dk["launches"]}
 
+    # measured HBM bytes per launch of the dominant kernel, from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload (scaled
+    # by a plain copy's counter reading); None when no summary covers it
+    tr = traffic_from_profile(roof["kernel"], prec_name)
+    if tr is not None:
+        roof["traffic"] = tr
+        roof["traffic_source"] = "profiles/" + TRAFFIC_FILE
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_grid)
@@ -206,6 +214,24 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+TRAFFIC_FILE = "large_update_traffic_convdiff3d64.md"
+
+
+def traffic_from_profile(kernel_class, prec_name):
+    path = os.path.join(ROOT, "profiles", TRAFFIC_FILE)
+    if prec_name != "fp64" or not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            if ":" in line and not line.startswith(("|", " ", "#")):
+                key, _, val = line.partition(":")
+                vals[key.strip()] = val.strip()
+    if vals.get("kernel_class") != kernel_class or "traffic_bytes_per_launch" not in vals:
+        return None
+    return float(vals["traffic_bytes_per_launch"])
 
 
 def cpu_baseline(grid):

@@ -77,6 +77,26 @@ def test_config2_convdiff3d_reduced(precision):
     assert np.max(np.abs(x - xt)) < 1e-10
 
 
+def test_convdiff3d_32_accuracy_matches_oracle():
+    # the device factor is at least as accurate as the reference's own
+    # algorithm (the oracle) on this operator, not merely under the 1e-14 bar
+    _say("case convdiff3d 32^3 fp64 accuracy")
+    A = convdiff3d(32)
+    n = A.shape[0]
+    xt = uniform(2, n, -1, 1)
+    b = A @ xt
+    s = _solver(A)
+    x = b.copy()
+    triangularsolve(s, x, refine=False)
+    eta = backward_error(A, x, b)
+    n_, cp, rv, nz = csc1(A)
+    ol, ou, op, rc, _ = orc.factor_csc(s.slvr.sym, cp, rv, nz)
+    xo = orc.solve(s.slvr.sym, ol, ou, op, b.copy())
+    eta_o = backward_error(A, xo, b)
+    _say(f"  eta {eta:.2e} oracle {eta_o:.2e}")
+    assert eta <= max(eta_o, 1e-16)
+
+
 def test_config3_elasticity_reduced():
     _say("case config3 elasticity 5^3")
     A = elasticity3d(5)
