@@ -24,13 +24,6 @@
 namespace spk {
 
 constexpr int kWG = 256;      // threads per workgroup (4 waves)
-constexpr int kTile = 64;     // update-kernel tile edge
-constexpr int kMaxPiece = 64; // widest piece of a small (LDS) front; the
-                              // large-front kernels take the width bound as
-                              // a template argument: 64 for LU, 96 for SPD
-                              // (the reference's split leaves a last piece
-                              // of up to 1.5x its 30 / 60 limit)
-constexpr int kUcols = 32;    // U12 columns per panel workgroup
 constexpr int kSmallLmax = 64; // fronts with L <= this are factored in LDS
 
 // Device view of the symbolic factor and the front schedule.
@@ -69,12 +62,12 @@ struct DevSym {
 // taking part in the launch).
 struct BlkDesc {
   int32_t front;
-  int32_t piece;     // 0-based supernode (panel) / first piece of the block
+  int32_t piece;     // first piece (0-based supernode) of the block
   int32_t np;        // pieces in the block
   int32_t c0, c1;    // block columns [c0, c1), front-local
   int32_t wg_begin;  // prefix sum of workgroups over the launch
-  int32_t nl;        // gemm: tiles per dimension
-  int32_t slot;      // stage slot of the (first) piece
+  int32_t nl;        // off-diagonal kernel: workgroups taking rows below
+  int32_t pad;
 };
 
 // Per-launch descriptor of the trailing-update kernel: one rectangle of one

@@ -164,7 +164,7 @@ small_front_kernel(DevSym s, const int32_t* __restrict__ list,
                    unsigned long long* __restrict__ first_fail) {
   extern __shared__ unsigned char smem_raw[];
   T* F = reinterpret_cast<T*>(smem_raw);
-  __shared__ int s_piv[kMaxPiece];
+  __shared__ int s_piv[kSmallLmax];  // a piece of a small front has <= L columns
   __shared__ int s_zero;
   const int tid = threadIdx.x;
   const int f = list[blockIdx.x];
