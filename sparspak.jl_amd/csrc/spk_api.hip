@@ -685,6 +685,10 @@ void set_lds_attrs() {
                             hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
   hipck(hipFuncSetAttribute((const void*)spk::blk_offdiag_kernel<T, kNqSPD>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+  hipck(hipFuncSetAttribute((const void*)spk::fwd_chain_kernel<T>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+  hipck(hipFuncSetAttribute((const void*)spk::bwd_chain_kernel<T>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
   done = true;
 }
 
@@ -801,6 +805,7 @@ void run_assemble(spkhip_handle* h) {
 template <typename T, int MAXW, int GW>
 void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bwd) {
   hipStream_t st = h->stream;
+  set_lds_attrs<T>();
   DevBuf<T>& vec = Bufs<T>::vec(h);
   vec.ensure((size_t)std::max<i64>(h->vec_elems * nrhs, 1));
   const T* lnz = Bufs<T>::lnz(h).p;
@@ -824,8 +829,8 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
       for (const LevelPlan::SolveStep& S : lp.fsteps) {
         {
           Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
-          hipLaunchKernelGGL((spk::fwd_chain_kernel<T, MAXW, GW>), dim3(S.chain_cnt), dim3(kWG),
-                             0, st, h->ds, h->d_sdesc.p + S.chain_off, lnz, h->d_ipvt.p, rhs, ldr,
+          hipLaunchKernelGGL((spk::fwd_chain_kernel<T>), dim3(S.chain_cnt), dim3(kWG),
+                             (size_t)128 * spk::kRLD * sizeof(T), st, h->ds, h->d_sdesc.p + S.chain_off, lnz, h->d_ipvt.p, rhs, ldr,
                              nrhs, vec.p);
           hipck(hipGetLastError());
         }
@@ -848,8 +853,8 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
           hipck(hipGetLastError());
         }
         Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);
-        hipLaunchKernelGGL((spk::bwd_chain_kernel<T, MAXW, GW>), dim3(S.chain_cnt), dim3(kWG), 0,
-                           st, h->ds, h->d_sdesc.p + S.chain_off, lnz, unz, rhs, ldr, nrhs, vec.p);
+        hipLaunchKernelGGL((spk::bwd_chain_kernel<T>), dim3(S.chain_cnt), dim3(kWG),
+                           (size_t)128 * spk::kRLD * sizeof(T), st, h->ds, h->d_sdesc.p + S.chain_off, lnz, unz, rhs, ldr, nrhs, vec.p);
         hipck(hipGetLastError());
       }
       if (lp.small_cnt > 0) {

@@ -1293,45 +1293,101 @@ fwd_assemble_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restr
   }
 }
 
-// Forward chain of one block (one workgroup per front of the step).
-template <typename T, int MAXW, int GW>
+// Value i (0 <= i < 128, wave-uniform) of a vector held two entries per lane
+// (entry i in lane i % 64 of y0 for i < 64, of y1 otherwise).
+template <typename T>
+__device__ inline T bcast2(T y0, T y1, int i) {
+  return rdlane(i < 64 ? y0 : y1, i & 63);
+}
+
+// Forward chain of one block (one workgroup per front of the step). The
+// block's strictly lower triangle is loaded into LDS once (leading dimension
+// kRLD, every column from its lnz panel); wave 0 then holds the block's
+// entries of y in registers, two per lane, and per piece applies the
+// interchanges and eliminates the piece's columns from every later row of
+// the block (the piece's unit-lower solve and its in-block L21 update).
+// Dynamic LDS: 128 * kRLD elements.
+template <typename T>
 __global__ void __launch_bounds__(kWG)
 fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ lnz,
                  const int64_t* __restrict__ ipvt, T* __restrict__ rhs, int64_t ldr, int nrhs,
                  T* __restrict__ vec) {
-  __shared__ T yb[GW];
-  __shared__ T D[MAXW * MAXW];
-  __shared__ int spiv[MAXW];
-  const int tid = threadIdx.x, wave = tid / 64;
+  constexpr int LD = kRLD;
+  extern __shared__ unsigned char smem_raw[];
+  T* R = reinterpret_cast<T*>(smem_raw);
+  __shared__ int spiv[128];     // block-local row swapped with row i
+  __shared__ int64_t lb[128];   // lnz offset of (front row c0, column g0 + c)
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
   const int L = s.fL[f];
   const int64_t fc0 = s.fc0[f];
   const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
+  const int64_t g0 = fc0 + c0;
+  for (int i = tid; i < bw; i += kWG) {
+    lb[i] = s.lcol[g0 + i] + c0;
+    const int ps = piece_start_of_col(s, fc0, c0 + i) - c0;  // block-local
+    spiv[i] = ps + (int)ipvt[g0 + i] - 1;
+  }
+  __syncthreads();
+  // strictly lower triangle: R[r + c LD] = L(c0 + r, g0 + c), r > c; each
+  // thread takes rows lane, lane + 64 of 16 columns, 32 loads in flight
+  for (int cb = wave; cb < bw; cb += 64) {
+    T v[16][2];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int c = cb + 4 * u;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = lane + 64 * h;
+        v[u][h] = (c < bw && r < bw && r > c) ? lnz[lb[c] + r] : T(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int c = cb + 4 * u;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = lane + 64 * h;
+        if (c < bw && r < bw && r > c) R[r + c * LD] = v[u][h];
+      }
+    }
+  }
+  __syncthreads();
+  if (wave != 0) return;
+  const bool in0 = lane < bw, in1 = lane + 64 < bw;
   for (int q = 0; q < nrhs; ++q) {
     T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
     T* x = rhs + (int64_t)q * ldr;
-    for (int i = tid; i < bw; i += kWG) yb[i] = yg[c0 + i];
-    __syncthreads();
+    T y0 = in0 ? yg[c0 + lane] : T(0);
+    T y1 = in1 ? yg[c0 + 64 + lane] : T(0);
     for (int pi = 0; pi < d.np; ++pi) {
       int64_t pc0, nq64;
       piece_cols(s, d.piece + pi, pc0, nq64);
-      const int nq = (int)nq64;
-      const int cq = (int)(pc0 - fc0);
-      stage_block<T, MAXW>(D, lnz + (s.xlnz[pc0] - 1), nq, L - cq);
-      for (int i = tid; i < nq; i += kWG) spiv[i] = (int)ipvt[pc0 + i] - 1;
-      __syncthreads();
-      if (wave == 0) wave_lower_solve<T, MAXW>(yb + (cq - c0), D, spiv, nq);
-      __syncthreads();
-      for (int r = cq + nq + tid; r < c1; r += kWG)
-        yb[r - c0] -= dot_lrow<T>(s, lnz, pc0, nq, r, yb + (cq - c0));
-      __syncthreads();
+      const int o = (int)(pc0 - g0), nq = (int)nq64;
+      for (int k = o; k < o + nq; ++k) {  // interchanges (glaswp!)
+        const int kp = spiv[k];
+        if (kp != k) {
+          const T a = bcast2(y0, y1, k), b = bcast2(y0, y1, kp);
+          if (k < 64) { if (lane == k) y0 = b; } else if (lane == k - 64) y1 = b;
+          if (kp < 64) { if (lane == kp) y0 = a; } else if (lane == kp - 64) y1 = a;
+        }
+      }
+      for (int k = o; k < o + nq; ++k) {  // eliminate column k below row k
+        const T yk = bcast2(y0, y1, k);
+        const T l0 = R[lane + k * LD], l1 = R[lane + 64 + k * LD];
+        if (in0 && lane > k) y0 -= l0 * yk;
+        if (in1 && lane + 64 > k) y1 -= l1 * yk;
+      }
     }
-    for (int i = tid; i < bw; i += kWG) {
-      yg[c0 + i] = yb[i];
-      x[fc0 + c0 + i] = yb[i];
+    if (in0) {
+      yg[c0 + lane] = y0;
+      x[fc0 + c0 + lane] = y0;
     }
-    __syncthreads();
+    if (in1) {
+      yg[c0 + 64 + lane] = y1;
+      x[fc0 + c0 + 64 + lane] = y1;
+    }
   }
 }
 
@@ -1399,45 +1455,84 @@ bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   }
 }
 
-// Backward chain of one block.
-template <typename T, int MAXW, int GW>
+// Backward chain of one block. The block's upper triangle is loaded into
+// LDS once: inside a piece's diagonal block from the packed LU in lnz, right
+// of it from the row's unz panel. With t = U[c0:c1, c1:L] x[c1:L] already
+// in the vector arena (bwd_bgemv_kernel), wave 0 holds y = x - t in
+// registers and back-substitutes over the whole block, last column first:
+// x_k = y_k / U(k, k), then y_i -= U(i, k) x_k for i < k — the in-block U12
+// products and the upper block solves of the reference, by columns.
+// Dynamic LDS: 128 * kRLD elements.
+template <typename T>
 __global__ void __launch_bounds__(kWG)
 bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ lnz,
                  const T* __restrict__ unz, T* __restrict__ rhs, int64_t ldr, int nrhs,
                  T* __restrict__ vec) {
-  __shared__ T yb[GW];
-  __shared__ T D[MAXW * MAXW];
+  constexpr int LD = kRLD;
+  extern __shared__ unsigned char smem_raw[];
+  T* R = reinterpret_cast<T*>(smem_raw);
+  __shared__ int pstart[128];   // block-local first column of each column's piece
+  __shared__ int64_t lb[128];   // lnz offset of (front row c0, column g0 + c)
+  __shared__ int64_t ub[128];   // unz offset of (row g0 + r, front column c0)
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
   const int L = s.fL[f];
   const int64_t fc0 = s.fc0[f];
   const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
+  const int64_t g0 = fc0 + c0;
+  for (int i = tid; i < bw; i += kWG) {
+    pstart[i] = piece_start_of_col(s, fc0, c0 + i) - c0;
+    lb[i] = s.lcol[g0 + i] + c0;
+    ub[i] = s.ucol[g0 + i] + c0;
+  }
+  __syncthreads();
+  // upper triangle: R[r + c LD] = U(g0 + r, c0 + c), r <= c; inside the
+  // column's piece from the packed LU, before it from the row's U panel
+  for (int cb = wave; cb < bw; cb += 64) {
+    T v[16][2];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int c = cb + 4 * u;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = lane + 64 * h;
+        v[u][h] = T(0);
+        if (c < bw && r <= c) v[u][h] = (r >= pstart[c]) ? lnz[lb[c] + r] : unz[ub[r] + c];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int c = cb + 4 * u;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = lane + 64 * h;
+        if (c < bw && r <= c) R[r + c * LD] = v[u][h];
+      }
+    }
+  }
+  __syncthreads();
+  if (wave != 0) return;
+  const bool in0 = lane < bw, in1 = lane + 64 < bw;
   for (int q = 0; q < nrhs; ++q) {
     T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
     T* x = rhs + (int64_t)q * ldr;
-    for (int i = tid; i < bw; i += kWG)
-      yb[i] = (c1 < L) ? x[fc0 + c0 + i] - yg[c0 + i] : x[fc0 + c0 + i];
-    __syncthreads();
-    for (int pi = d.np - 1; pi >= 0; --pi) {
-      int64_t pc0, nq64;
-      piece_cols(s, d.piece + pi, pc0, nq64);
-      const int nq = (int)nq64;
-      const int cq = (int)(pc0 - fc0);
-      stage_block<T, MAXW>(D, lnz + (s.xlnz[pc0] - 1), nq, L - cq);
-      for (int c = wave; c < nq; c += 4) {
-        const T* u = unz + s.ucol[pc0 + c];
-        T acc = T(0);
-        for (int j = cq + nq + lane; j < c1; j += 64) acc += u[j] * yb[j - c0];
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-        if (lane == 0) yb[cq - c0 + c] -= acc;
+    const bool sub = c1 < L;
+    T y0 = in0 ? x[fc0 + c0 + lane] - (sub ? yg[c0 + lane] : T(0)) : T(0);
+    T y1 = in1 ? x[fc0 + c0 + 64 + lane] - (sub ? yg[c0 + 64 + lane] : T(0)) : T(0);
+    for (int k = bw - 1; k >= 0; --k) {
+      const T xk = bcast2(y0, y1, k) / R[k + k * LD];
+      const T u0 = R[lane + k * LD], u1 = R[lane + 64 + k * LD];
+      if (k < 64) {
+        if (lane == k) y0 = xk;
+      } else if (lane == k - 64) {
+        y1 = xk;
       }
-      __syncthreads();
-      if (wave == 0) wave_upper_solve<T, MAXW>(yb + (cq - c0), D, nq);
-      __syncthreads();
+      if (lane < k) y0 -= u0 * xk;
+      if (lane + 64 < k) y1 -= u1 * xk;
     }
-    for (int i = tid; i < bw; i += kWG) x[fc0 + c0 + i] = yb[i];
-    __syncthreads();
+    if (in0) x[fc0 + c0 + lane] = y0;
+    if (in1) x[fc0 + c0 + 64 + lane] = y1;
   }
 }
 
