@@ -166,6 +166,7 @@ struct spkhip_handle {
     int level, grid;
   };
   int cur_level = -1, cur_grid = 0;   // tags of the next profiled launches
+  size_t upd_group = 8;                // blocks per full trailing update
   std::vector<Rec> recs;
   ~spkhip_handle() {
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
@@ -537,15 +538,16 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         const Blk& B = fblocks[fi][gi];
         const i32 f = large[fi];
         const i32 L = h->fL[f];
-        if (gi % 2 == 0 && gi + 1 < fblocks[fi].size()) {
-          const Blk& N = fblocks[fi][gi + 1];
-          rect(f, B.c0, B.c1, B.c1, L, N.c0, N.c1);
-          rect(f, B.c0, B.c1, N.c0, N.c1, N.c1, L);
-        } else if (gi % 2 == 1) {
-          const Blk& P = fblocks[fi][gi - 1];
-          rect(f, P.c0, B.c1, B.c1, L, B.c1, L);
+        const size_t gs = gi - gi % h->upd_group;
+        const size_t ge = std::min(gs + h->upd_group, fblocks[fi].size());
+        if (gi + 1 < ge) {
+          // border of the group's remaining blocks [B.c1, E) x ..., E = their end
+          const i32 E = fblocks[fi][ge - 1].c1;
+          rect(f, B.c0, B.c1, B.c1, L, B.c1, E);
+          rect(f, B.c0, B.c1, B.c1, E, E, L);
         } else {
-          rect(f, B.c0, B.c1, B.c1, L, B.c1, L);
+          // last block of the group: the rest with the whole group
+          rect(f, fblocks[fi][gs].c0, B.c1, B.c1, L, B.c1, L);
         }
       }
       G.gemm.cnt = (int)gdesc.size() - G.gemm.off;
@@ -951,6 +953,8 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   const i64 wmax = (kind == SPKHIP_KIND_LU) ? kNqLU : kNqSPD;
   for (i64 J = 0; J < nsuper; ++J)
     if (xsuper[J + 1] - xsuper[J] > wmax) throw ApiError{-2};
+  // tuning knob: SPKHIP_UPD_GROUP=G takes G blocks per full trailing update
+  if (const char* e = std::getenv("SPKHIP_UPD_GROUP")) h->upd_group = std::max(1, std::atoi(e));
   build_fronts(h);
   std::vector<i32> small, it_front, it_col, ea_child, ea_t, solve, largel, ut_front, ut_piece,
       ut_r0;

@@ -410,12 +410,13 @@ __device__ inline int find_blk(const Desc* d, int nd, int b) {
 //                      and its unit-lower solve — the left-looking order of
 //                      the reference, so every entry is the reference's.
 //  blk_gemm_kernel     trailing update A22 -= L21 U12. Blocks are taken in
-//                      pairs: after the first block of a pair only the
-//                      border of the second (its columns, and its rows right
-//                      of it) is updated; after the second, the rest of the
-//                      trailing matrix receives both blocks at once, with
-//                      inner dimension up to 256 — half the read-modify-write
-//                      traffic of one update per block.
+//                      groups (8 by default): after each block but the last
+//                      of a group only the border of the group's remaining
+//                      blocks (their columns, and their rows right of them)
+//                      is updated; after the last, the rest of the trailing
+//                      matrix receives the whole group at once, with inner
+//                      dimension up to 8 x 128 — an eighth of the
+//                      read-modify-write traffic of one update per block.
 // ---------------------------------------------------------------------------
 
 // Copy of an nr x nc tile (nr <= 128) from a column-major HBM array (leading
