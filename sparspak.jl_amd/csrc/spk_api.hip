@@ -136,7 +136,6 @@ struct spkhip_handle {
   DevBuf<i32> d_colind;
   DevBuf<i64> d_csc2csr;
   DevBuf<double> d_nzval, d_csrval, d_rowabs;
-  double anorm = 0;
   bool have_values = false;
   // numeric
   int precision = -1;  // of the current factor
@@ -149,7 +148,8 @@ struct spkhip_handle {
   // solve / refine work
   DevBuf<double> d_b, d_x, d_r, d_tmp64, d_norms;
   std::vector<i64> seg{0};            // row segments for the backward error
-  DevBuf<i64> d_seg;
+  DevBuf<i64> d_seg, d_clo;
+  DevBuf<i32> d_cseg;
   int refine_iters = 0;
   double backward_error = 0;
   // timing
@@ -1338,10 +1338,7 @@ int64_t inmatrix_common(spkhip_t h) {
                        h->d_nzval.p, h->d_csc2csr.p, h->d_csrval.p, h->nnz);
   hipLaunchKernelGGL(spk::rowabs_kernel, dim3(grid_for(h->n)), dim3(kWG), 0, st, h->d_rowptr.p,
                      h->d_csrval.p, h->d_rowabs.p, h->n);
-  hipLaunchKernelGGL(spk::colmaxabs_kernel, dim3(1), dim3(kWG), 0, st, h->d_rowabs.p, h->n, h->n,
-                     h->d_norms.p);
   hipck(hipGetLastError());
-  hipck(hipMemcpyAsync(&h->anorm, h->d_norms.p, sizeof(double), hipMemcpyDeviceToHost, st));
   sync(h);
   h->have_values = true;
   h->factored = false;
@@ -1487,10 +1484,27 @@ int64_t refine_impl(spkhip_t h, const double* d_bm, double* d_xo, int nrhs, doub
   double* nr_b = nr_x + (size_t)nseg * nrhs;
   double* nr_a = nr_b + (size_t)nseg * nrhs;
   std::vector<double> nrm((size_t)3 * nseg * nrhs + nseg);
-  hipLaunchKernelGGL(spk::segmaxabs_kernel, dim3(nseg * nrhs), dim3(kWG), 0, st, d_bm, n,
-                     h->d_seg.p, nseg, nr_b);
-  hipLaunchKernelGGL(spk::segmaxabs_kernel, dim3(nseg), dim3(kWG), 0, st, h->d_rowabs.p, n,
-                     h->d_seg.p, nseg, nr_a);
+  // row chunks of at most 8192 rows, none crossing a segment boundary
+  std::vector<i64> clo;
+  std::vector<i32> cseg;
+  for (int i = 0; i < nseg; ++i)
+    for (i64 r = h->seg[i]; r < h->seg[i + 1]; r += 8192) {
+      clo.push_back(r);
+      cseg.push_back(i);
+    }
+  const int nch = (int)cseg.size();
+  clo.push_back(n);
+  h->d_clo.upload(clo, st);
+  h->d_cseg.upload(cseg.empty() ? std::vector<i32>{0} : cseg, st);
+  auto segmax = [&](const double* v, int ncol, double* out) {
+    hipLaunchKernelGGL(spk::fill_kernel<double>, dim3(grid_for((i64)nseg * ncol)), dim3(kWG), 0,
+                       st, out, (int64_t)nseg * ncol, 0.0);
+    if (nch > 0)
+      hipLaunchKernelGGL(spk::segmax_chunk_kernel, dim3(nch * ncol), dim3(kWG), 0, st, v, n,
+                         h->d_clo.p, h->d_cseg.p, nch, nseg, (unsigned long long*)out);
+  };
+  segmax(d_bm, nrhs, nr_b);
+  segmax(h->d_rowabs.p, 1, nr_a);
   const double* src = d_bm;
   int iters = 0;
   double eta_max = 0;
@@ -1510,10 +1524,8 @@ int64_t refine_impl(spkhip_t h, const double* d_bm, double* d_xo, int nrhs, doub
       hipLaunchKernelGGL(spk::residual_kernel, dim3(grid_for(n)), dim3(kWG), 0, st, h->d_rowptr.p,
                          h->d_colind.p, h->d_csrval.p, d_xo, n, d_bm, n, h->d_r.p, n, n, nrhs);
     }
-    hipLaunchKernelGGL(spk::segmaxabs_kernel, dim3(nseg * nrhs), dim3(kWG), 0, st, h->d_r.p, n,
-                       h->d_seg.p, nseg, nr_r);
-    hipLaunchKernelGGL(spk::segmaxabs_kernel, dim3(nseg * nrhs), dim3(kWG), 0, st, d_xo, n,
-                       h->d_seg.p, nseg, nr_x);
+    segmax(h->d_r.p, nrhs, nr_r);
+    segmax(d_xo, nrhs, nr_x);
     hipck(hipGetLastError());
     hipck(hipMemcpyAsync(nrm.data(), h->d_norms.p, nrm.size() * sizeof(double),
                          hipMemcpyDeviceToHost, st));

@@ -644,24 +644,22 @@ __device__ void wg_getrf(T* D, int nq, int ncols, int* piv, int* perm, T* dinv, 
     const T l = nonzero ? v * inv : v;
     lprev = l;
     wprev = below && nonzero;
-    const int j1 = k + 1 + wave;
-    int j = j1;
-    for (; j + 28 < nq; j += 32) {
-      const T* pr = D + blane + j * LD;
-      T* xr = D + lane + j * LD;
+    // columns j = k + 1 + wave + 4 i in batches of eight: loads of a batch
+    // are issued together (indices clamped inside the block), stores guarded
+    for (int jb = k + 1 + wave; jb < nq; jb += 32) {
       T u[8], x[8];
 #pragma unroll
       for (int jj = 0; jj < 8; ++jj) {
-        u[jj] = pr[4 * jj * LD];
-        x[jj] = xr[4 * jj * LD];
+        const int j = min(jb + 4 * jj, nq - 1);
+        u[jj] = D[blane + j * LD];
+        x[jj] = D[lane + j * LD];
       }
       if (below) {
 #pragma unroll
-        for (int jj = 0; jj < 8; ++jj) xr[4 * jj * LD] = x[jj] - l * u[jj];
+        for (int jj = 0; jj < 8; ++jj)
+          if (jb + 4 * jj < nq) D[lane + (jb + 4 * jj) * LD] = x[jj] - l * u[jj];
       }
     }
-    for (; j < nq; j += 4)
-      if (below) D[lane + j * LD] -= l * D[blane + j * LD];
     __syncthreads();
   }
   if (wave == 0 && wprev) D[lane + (nq - 1) * LD] = lprev;
@@ -711,20 +709,18 @@ __device__ void nopiv_getrf(T* D, int nq, int* piv, T* dinv, int* zero) {
         l *= inv;
         D[i + k * LD] = l;
       }
-      int j0 = k + 1;
-      for (; j0 + 8 <= nq; j0 += 8) {
-        const T* pr = D + k + j0 * LD;
-        T* xr = D + i + j0 * LD;
+      for (int j0 = k + 1; j0 < nq; j0 += 8) {
         T u[8], x[8];
 #pragma unroll
         for (int jj = 0; jj < 8; ++jj) {
-          u[jj] = pr[jj * LD];
-          x[jj] = xr[jj * LD];
+          const int j = min(j0 + jj, nq - 1);
+          u[jj] = D[k + j * LD];
+          x[jj] = D[i + j * LD];
         }
 #pragma unroll
-        for (int jj = 0; jj < 8; ++jj) xr[jj * LD] = x[jj] - l * u[jj];
+        for (int jj = 0; jj < 8; ++jj)
+          if (j0 + jj < nq) D[i + (j0 + jj) * LD] = x[jj] - l * u[jj];
       }
-      for (; j0 < nq; ++j0) D[i + j0 * LD] -= l * D[k + j0 * LD];
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -1301,13 +1297,23 @@ __device__ inline T bcast2(T y0, T y1, int i) {
   return rdlane(i < 64 ? y0 : y1, i & 63);
 }
 
+// Entry src (0 <= src < 128, per lane) of the same two-per-lane vector.
+template <typename T>
+__device__ inline T gather2(T y0, T y1, int src) {
+  const T a = __shfl(y0, src & 63, 64), b = __shfl(y1, src & 63, 64);
+  return src < 64 ? a : b;
+}
+
 // Forward chain of one block (one workgroup per front of the step). The
 // block's strictly lower triangle is loaded into LDS once (leading dimension
-// kRLD, every column from its lnz panel); wave 0 then holds the block's
-// entries of y in registers, two per lane, and per piece applies the
-// interchanges and eliminates the piece's columns from every later row of
-// the block (the piece's unit-lower solve and its in-block L21 update).
-// Dynamic LDS: 128 * kRLD elements.
+// kRLD, every column from its lnz panel) and each piece's interchanges are
+// composed into one row permutation. Wave 0 then holds the block's entries
+// of y in registers, two per lane, and per piece applies the permutation
+// (cross-lane shuffles) and eliminates the piece's columns, four at a time,
+// from every later row of the block: the four new entries come from a small
+// unit-lower solve every lane does alike, then each row takes its four
+// products in column order — the same operations, in the same order, as
+// one column at a time. Dynamic LDS: 128 * kRLD elements.
 template <typename T>
 __global__ void __launch_bounds__(kWG)
 fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ lnz,
@@ -1316,8 +1322,10 @@ fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   constexpr int LD = kRLD;
   extern __shared__ unsigned char smem_raw[];
   T* R = reinterpret_cast<T*>(smem_raw);
-  __shared__ int spiv[128];     // block-local row swapped with row i
+  __shared__ int spiv[128];     // block-local row swapped with row i at its step
+  __shared__ int sperm[128];    // block-local row that position i takes
   __shared__ int64_t lb[128];   // lnz offset of (front row c0, column g0 + c)
+  __shared__ int pbeg[9];       // piece boundaries, block-local
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
@@ -1327,30 +1335,46 @@ fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   const int64_t g0 = fc0 + c0;
   for (int i = tid; i < bw; i += kWG) {
     lb[i] = s.lcol[g0 + i] + c0;
-    const int ps = piece_start_of_col(s, fc0, c0 + i) - c0;  // block-local
+    const int ps = piece_start_of_col(s, fc0, c0 + i) - c0;
     spiv[i] = ps + (int)ipvt[g0 + i] - 1;
+    sperm[i] = i;
   }
+  if (tid <= d.np) pbeg[tid] = (tid < d.np) ? (int)(s.xsuper[d.piece + tid] - 1 - g0) : bw;
   __syncthreads();
-  // strictly lower triangle: R[r + c LD] = L(c0 + r, g0 + c), r > c; each
-  // thread takes rows lane, lane + 64 of 16 columns, 32 loads in flight
-  for (int cb = wave; cb < bw; cb += 64) {
-    T v[16][2];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int c = cb + 4 * u;
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int r = lane + 64 * h;
-        v[u][h] = (c < bw && r < bw && r > c) ? lnz[lb[c] + r] : T(0);
+  if (wave == 0) {
+    // one lane per piece composes its interchanges (glaswp! order)
+    if (lane < d.np) {
+      for (int k = pbeg[lane]; k < pbeg[lane + 1]; ++k) {
+        const int kp = spiv[k];
+        if (kp != k) {
+          const int t = sperm[k];
+          sperm[k] = sperm[kp];
+          sperm[kp] = t;
+        }
       }
     }
+  } else {
+    // strictly lower triangle: R[r + c LD] = L(c0 + r, g0 + c), r > c;
+    // waves 1-3, rows lane and lane + 64, 8 columns per batch
+    for (int cb = wave - 1; cb < bw; cb += 24) {
+      T v[8][2];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int c = cb + 4 * u;
+      for (int u = 0; u < 8; ++u) {
+        const int c = cb + 3 * u;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int r = lane + 64 * h;
-        if (c < bw && r < bw && r > c) R[r + c * LD] = v[u][h];
+        for (int h = 0; h < 2; ++h) {
+          const int r = lane + 64 * h;
+          v[u][h] = (c < bw && r < bw && r > c) ? lnz[lb[c] + r] : T(0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int c = cb + 3 * u;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int r = lane + 64 * h;
+          if (c < bw && r < bw && r > c) R[r + c * LD] = v[u][h];
+        }
       }
     }
   }
@@ -1363,18 +1387,48 @@ fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
     T y0 = in0 ? yg[c0 + lane] : T(0);
     T y1 = in1 ? yg[c0 + 64 + lane] : T(0);
     for (int pi = 0; pi < d.np; ++pi) {
-      int64_t pc0, nq64;
-      piece_cols(s, d.piece + pi, pc0, nq64);
-      const int o = (int)(pc0 - g0), nq = (int)nq64;
-      for (int k = o; k < o + nq; ++k) {  // interchanges (glaswp!)
-        const int kp = spiv[k];
-        if (kp != k) {
-          const T a = bcast2(y0, y1, k), b = bcast2(y0, y1, kp);
-          if (k < 64) { if (lane == k) y0 = b; } else if (lane == k - 64) y1 = b;
-          if (kp < 64) { if (lane == kp) y0 = a; } else if (lane == kp - 64) y1 = a;
+      const int o = pbeg[pi], e = pbeg[pi + 1];
+      {  // the piece's rows to their positions
+        const int s0 = (lane >= o && lane < e) ? sperm[lane] : lane;
+        const int s1 = (lane + 64 >= o && lane + 64 < e) ? sperm[lane + 64] : lane + 64;
+        const T n0 = gather2(y0, y1, s0), n1 = gather2(y0, y1, s1);
+        y0 = n0;
+        y1 = n1;
+      }
+      int k = o;
+      for (; k + 4 <= e; k += 4) {
+        const T b0 = bcast2(y0, y1, k), b1 = bcast2(y0, y1, k + 1);
+        const T b2 = bcast2(y0, y1, k + 2), b3 = bcast2(y0, y1, k + 3);
+        const T* Rk = R + k * LD;
+        const T z0 = b0;
+        const T z1 = b1 - Rk[k + 1] * z0;
+        const T z2 = (b2 - Rk[k + 2] * z0) - Rk[LD + k + 2] * z1;
+        const T z3 = ((b3 - Rk[k + 3] * z0) - Rk[LD + k + 3] * z1) - Rk[2 * LD + k + 3] * z2;
+        const T* r0 = Rk + lane;
+        const T* r1 = Rk + lane + 64;
+        if (in0) {
+          if (lane > k + 3)
+            y0 = (((y0 - r0[0] * z0) - r0[LD] * z1) - r0[2 * LD] * z2) - r0[3 * LD] * z3;
+          else if (lane == k + 1)
+            y0 = z1;
+          else if (lane == k + 2)
+            y0 = z2;
+          else if (lane == k + 3)
+            y0 = z3;
+        }
+        if (in1) {
+          const int r = lane + 64;
+          if (r > k + 3)
+            y1 = (((y1 - r1[0] * z0) - r1[LD] * z1) - r1[2 * LD] * z2) - r1[3 * LD] * z3;
+          else if (r == k + 1)
+            y1 = z1;
+          else if (r == k + 2)
+            y1 = z2;
+          else if (r == k + 3)
+            y1 = z3;
         }
       }
-      for (int k = o; k < o + nq; ++k) {  // eliminate column k below row k
+      for (; k < e; ++k) {
         const T yk = bcast2(y0, y1, k);
         const T l0 = R[lane + k * LD], l1 = R[lane + 64 + k * LD];
         if (in0 && lane > k) y0 -= l0 * yk;
@@ -1460,10 +1514,11 @@ bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
 // LDS once: inside a piece's diagonal block from the packed LU in lnz, right
 // of it from the row's unz panel. With t = U[c0:c1, c1:L] x[c1:L] already
 // in the vector arena (bwd_bgemv_kernel), wave 0 holds y = x - t in
-// registers and back-substitutes over the whole block, last column first:
-// x_k = y_k / U(k, k), then y_i -= U(i, k) x_k for i < k — the in-block U12
-// products and the upper block solves of the reference, by columns.
-// Dynamic LDS: 128 * kRLD elements.
+// registers and back-substitutes over the whole block, last column first,
+// four columns per step: the four new entries x_k = (y_k - ...) / U(k, k)
+// every lane computes alike, then each earlier row takes its four products
+// in column order — the in-block U12 products and the upper block solves
+// of the reference, by columns. Dynamic LDS: 128 * kRLD elements.
 template <typename T>
 __global__ void __launch_bounds__(kWG)
 bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ lnz,
@@ -1490,10 +1545,10 @@ bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   __syncthreads();
   // upper triangle: R[r + c LD] = U(g0 + r, c0 + c), r <= c; inside the
   // column's piece from the packed LU, before it from the row's U panel
-  for (int cb = wave; cb < bw; cb += 64) {
-    T v[16][2];
+  for (int cb = wave; cb < bw; cb += 32) {
+    T v[8][2];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
+    for (int u = 0; u < 8; ++u) {
       const int c = cb + 4 * u;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
@@ -1503,7 +1558,7 @@ bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
       }
     }
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
+    for (int u = 0; u < 8; ++u) {
       const int c = cb + 4 * u;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
@@ -1521,7 +1576,46 @@ bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
     const bool sub = c1 < L;
     T y0 = in0 ? x[fc0 + c0 + lane] - (sub ? yg[c0 + lane] : T(0)) : T(0);
     T y1 = in1 ? x[fc0 + c0 + 64 + lane] - (sub ? yg[c0 + 64 + lane] : T(0)) : T(0);
-    for (int k = bw - 1; k >= 0; --k) {
+    int k = bw - 1;
+    for (; k >= 3; k -= 4) {
+      // columns k, k-1, k-2, k-3
+      const T b0 = bcast2(y0, y1, k), b1 = bcast2(y0, y1, k - 1);
+      const T b2 = bcast2(y0, y1, k - 2), b3 = bcast2(y0, y1, k - 3);
+      const T* R0 = R + k * LD;          // column k
+      const T* R1 = R0 - LD;             // column k - 1
+      const T* R2 = R0 - 2 * LD;
+      const T* R3 = R0 - 3 * LD;
+      const T x0 = b0 / R0[k];
+      const T x1 = (b1 - R0[k - 1] * x0) / R1[k - 1];
+      const T x2 = ((b2 - R0[k - 2] * x0) - R1[k - 2] * x1) / R2[k - 2];
+      const T x3 = (((b3 - R0[k - 3] * x0) - R1[k - 3] * x1) - R2[k - 3] * x2) / R3[k - 3];
+      if (in0) {
+        if (lane < k - 3)
+          y0 = (((y0 - R0[lane] * x0) - R1[lane] * x1) - R2[lane] * x2) - R3[lane] * x3;
+        else if (lane == k)
+          y0 = x0;
+        else if (lane == k - 1)
+          y0 = x1;
+        else if (lane == k - 2)
+          y0 = x2;
+        else if (lane == k - 3)
+          y0 = x3;
+      }
+      if (in1) {
+        const int r = lane + 64;
+        if (r < k - 3)
+          y1 = (((y1 - R0[r] * x0) - R1[r] * x1) - R2[r] * x2) - R3[r] * x3;
+        else if (r == k)
+          y1 = x0;
+        else if (r == k - 1)
+          y1 = x1;
+        else if (r == k - 2)
+          y1 = x2;
+        else if (r == k - 3)
+          y1 = x3;
+      }
+    }
+    for (; k >= 0; --k) {
       const T xk = bcast2(y0, y1, k) / R[k + k * LD];
       const T u0 = R[lane + k * LD], u1 = R[lane + 64 + k * LD];
       if (k < 64) {
@@ -1582,38 +1676,36 @@ __global__ void rowabs_kernel(const int64_t* __restrict__ rowptr,
   out[i] = acc;
 }
 
-// Column-wise max |v| over n rows of nrhs columns; out[q] (one block per q).
-__global__ void colmaxabs_kernel(const double* __restrict__ v, int64_t ld,
-                                 int64_t n, double* __restrict__ out) {
+// Max |v| over row chunks [clo[c], clo[c+1]) of each column, folded into the
+// per-(segment, column) maxima out[cseg[c] + q * nseg] by an integer
+// atomicMax on the bit patterns (a non-negative double orders as its bits);
+// out starts at +0.0. One block per (chunk, column).
+__global__ void segmax_chunk_kernel(const double* __restrict__ v, int64_t ld,
+                                    const int64_t* __restrict__ clo,
+                                    const int32_t* __restrict__ cseg, int nchunks, int nseg,
+                                    unsigned long long* __restrict__ out) {
   __shared__ double red[kWG];
-  const int q = blockIdx.x;
+  const int c = blockIdx.x % nchunks, q = blockIdx.x / nchunks;
+  const double* vq = v + (int64_t)q * ld;
+  const int64_t lo = clo[c], hi = clo[c + 1];
   double m = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += kWG) m = fmax(m, fabs(v[i + q * ld]));
+  int64_t r = lo + threadIdx.x;
+  for (; r + 7 * kWG < hi; r += 8 * kWG) {
+    double a[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = fabs(vq[r + u * kWG]);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m = fmax(m, a[u]);
+  }
+  for (; r < hi; r += kWG) m = fmax(m, fabs(vq[r]));
   red[threadIdx.x] = m;
   __syncthreads();
   for (int off = kWG / 2; off > 0; off >>= 1) {
     if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[q] = red[0];
-}
-
-// Max |v| over each segment of rows [seg[i], seg[i+1]) of each column:
-// out[i + q * nseg]; one block per (segment, column).
-__global__ void segmaxabs_kernel(const double* __restrict__ v, int64_t ld,
-                                 const int64_t* __restrict__ seg, int nseg,
-                                 double* __restrict__ out) {
-  __shared__ double red[kWG];
-  const int i = blockIdx.x % nseg, q = blockIdx.x / nseg;
-  double m = 0.0;
-  for (int64_t r = seg[i] + threadIdx.x; r < seg[i + 1]; r += kWG) m = fmax(m, fabs(v[r + q * ld]));
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int off = kWG / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[i + (int64_t)q * nseg] = red[0];
+  if (threadIdx.x == 0)
+    atomicMax(out + cseg[c] + (int64_t)q * nseg, (unsigned long long)__double_as_longlong(red[0]));
 }
 
 // x += d (d in the factor precision, already in original numbering).
