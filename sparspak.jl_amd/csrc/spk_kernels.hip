@@ -338,7 +338,15 @@ large_init_kernel(DevSym s, const int32_t* __restrict__ it_front,
         if (r >= j) F[r] = lc[r];
         else F[r] = lnz[s.lcol[fc0 + r] + j];
     } else {
-      for (int r = cp + threadIdx.x; r < L; r += kWG) F[r] = lc[r];
+      // eight loads in flight per thread
+      for (int r0 = cp + threadIdx.x; r0 < L; r0 += 8 * kWG) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = (r0 + u * kWG < L) ? lc[r0 + u * kWG] : T(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (r0 + u * kWG < L) F[r0 + u * kWG] = v[u];
+      }
     }
   } else {
     for (int r = W + threadIdx.x; r < L; r += kWG) F[r] = T(0);
@@ -350,7 +358,21 @@ large_init_kernel(DevSym s, const int32_t* __restrict__ it_front,
     const int Rc = s.fL[c] - s.fW[c];
     const int32_t* rel = s.relmap + s.frel[c];
     const T* col = arena + s.fcb[c] + (int64_t)t * s.fcbld[c];
-    for (int tp = threadIdx.x; tp < Rc; tp += kWG) F[rel[tp]] += col[tp];
+    for (int tp0 = threadIdx.x; tp0 < Rc; tp0 += 4 * kWG) {
+      int ri[4];
+      T cv[4], fv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int tp = tp0 + u * kWG;
+        ri[u] = (tp < Rc) ? rel[tp] : 0;
+        cv[u] = (tp < Rc) ? col[tp] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) fv[u] = F[ri[u]];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (tp0 + u * kWG < Rc) F[ri[u]] = fv[u] + cv[u];
+    }
     __syncthreads();
   }
 }
@@ -371,7 +393,14 @@ large_writeback_kernel(DevSym s, const int32_t* __restrict__ it_front,
   const T* F = arena + s.foff[f] + (int64_t)j * L;
   const int cp = piece_start_of_col(s, fc0, j);
   T* lc = lnz + s.lcol[fc0 + j];
-  for (int r = cp + threadIdx.x; r < L; r += kWG) lc[r] = F[r];
+  for (int r0 = cp + threadIdx.x; r0 < L; r0 += 8 * kWG) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = (r0 + u * kWG < L) ? F[r0 + u * kWG] : T(0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (r0 + u * kWG < L) lc[r0 + u * kWG] = v[u];
+  }
 }
 
 // Descriptor of workgroup b: the last one whose wg_begin <= b.
