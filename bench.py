@@ -2,11 +2,13 @@
 """Benchmark: numeric factor + solve of the 3D upwind convection-diffusion
 7-point stencil on a 64^3 grid (n = 262,144; diffusivity 1, velocity
 (1, 0.5, 0.25), cell Peclet 0.5), LU through SparseSolver's numeric phase on
-one MI355X, in two precision paths:
+one MI355X, in three precision paths:
 
-  fp64     factor and solve in double (the reference's Float64 path)
-  fp32+ir  factor and solve in float, residual and update in double,
-           refined until the normwise backward error is <= 1e-14
+  fp64       factor and solve in double (the reference's Float64 path)
+  fp32       factor and solve in float, residual and update in double,
+             refined until the normwise backward error is <= 1e-14
+  fp32_bf16  as fp32, with bf16 operands (float accumulation) in the
+             trailing updates of fronts wider than 256 columns
 
 A step is: inmatrix (values already resident in HBM) -> factor -> solve of
 one right-hand side b = A x_true (x_true ~ U[-1, 1], splitmix64 seed 2) with
@@ -30,7 +32,9 @@ for p in (ROOT, os.path.join(ROOT, "sparspak.jl_amd")):
 
 import numpy as np  # noqa: E402
 
-PEAK = {"fp64": 78.6, "fp32": 157.3}   # MI355X dense matrix TFLOP/s (datasheet)
+# MI355X dense matrix TFLOP/s (datasheet); the bf16 path's large-front
+# updates run on the bf16 matrix cores
+PEAK = {"fp64": 78.6, "fp32": 157.3, "fp32_bf16": 2516.6}
 HBM_PEAK = 8000.0                       # GB/s
 
 
@@ -46,7 +50,7 @@ def main():
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--cpu-grid", type=int, default=36)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--paths", default="fp64,fp32")
+    ap.add_argument("--paths", default="fp64,fp32,fp32_bf16")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (three times), the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -98,7 +102,7 @@ def main():
     d_x = torch.empty_like(d_b)
     torch.cuda.synchronize()
 
-    prec = {"fp64": 0, "fp32": 1}
+    prec = {"fp64": 0, "fp32": 1, "fp32_bf16": 2}
 
     def step(path):
         _lib.check(L.spkhip_inmatrix_device(h, ctypes.c_void_p(d_nz.data_ptr())), "inmatrix")
@@ -165,7 +169,7 @@ def main():
     dom = max(kst, key=lambda k: kst[k]["ms"])
     dk = kst[dom]
     compute_bound = dom in ("large_update", "small_front", "large_diag", "large_offdiag")
-    prec_name = "fp32" if head == "fp32" else "fp64"
+    prec_name = head
     if compute_bound and dk["flops"] > 0:
         achieved = dk["flops"] / (dk["ms"] / 1e3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK[prec_name],
@@ -202,7 +206,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64ir" if head == "fp32" else "f64",
+        "dtype": {"fp64": "f64", "fp32": "f32+f64ir", "fp32_bf16": "f32+bf16upd+f64ir"}[head],
         "data": "synthetic (generated matrix and rhs, splitmix64 seeds)",
         "config": {"workload": f"3D upwind convection-diffusion 7-point {g}^3 LU "
                                "(SparseSolver numeric phase: inmatrix+factor+solve)",

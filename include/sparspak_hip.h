@@ -87,6 +87,9 @@ typedef struct spkhip_handle* spkhip_t;
 /* Factor precisions. */
 #define SPKHIP_FP64 0     /* factor and solve in double */
 #define SPKHIP_FP32 1     /* factor and solve in float (refine in double) */
+#define SPKHIP_FP32_BF16 2 /* float factor whose trailing updates in fronts of
+                              more than 256 own columns take bf16 operands
+                              (float accumulation); solves as SPKHIP_FP32 */
 
 typedef struct spkhip_stats {
   int64_t n, nsuper, nfront, nlevel;
@@ -128,9 +131,9 @@ int64_t spkhip_inmatrix(spkhip_t h, const double* nzval);
 int64_t spkhip_inmatrix_device(spkhip_t h, const double* d_nzval);
 
 /* _factor! -> _lufactor! (SpkSparseBase.jl:378-391, SpkLUFactor.jl:60-255)
- * or _ldltfactor! (SpkSparseSpdBase.jl:313-332). precision: SPKHIP_FP64 or
- * SPKHIP_FP32. Returns -1 if any diagonal block met a zero pivot (the first
- * failing supernode is in stats). */
+ * or _ldltfactor! (SpkSparseSpdBase.jl:313-332). precision: SPKHIP_FP64,
+ * SPKHIP_FP32 or SPKHIP_FP32_BF16. Returns -1 if any diagonal block met a
+ * zero pivot (the first failing supernode is in stats). */
 int64_t spkhip_factor(spkhip_t h, int64_t precision);
 
 /* _triangularsolve! (SpkSparseBase.jl:400-416) for nrhs right-hand sides

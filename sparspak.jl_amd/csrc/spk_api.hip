@@ -167,6 +167,8 @@ struct spkhip_handle {
   };
   int cur_level = -1, cur_grid = 0;   // tags of the next profiled launches
   size_t upd_group = 8;                // blocks per full trailing update
+  bool lowp = false;                   // bf16 operands in large-front updates
+  int mode = 0;                        // precision mode of the last factor
   std::vector<Rec> recs;
   ~spkhip_handle() {
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
@@ -527,7 +529,8 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
       auto rect = [&](i32 f, i32 k0, i32 k1, i32 r0, i32 r1, i32 j0, i32 j1) {
         if (r1 <= r0 || j1 <= j0 || k1 <= k0) return;
         const i32 ntr = (r1 - r0 + 127) / 128, ntc = (j1 - j0 + 127) / 128;
-        gdesc.push_back(spk::GemmDesc{f, k0, k1, r0, r1 - r0, j0, j1 - j0, G.gemm.wg, ntr});
+        gdesc.push_back(spk::GemmDesc{f, k0, k1, r0, r1 - r0, j0, j1 - j0, G.gemm.wg, ntr,
+                                      h->fW[f] > 256 ? 1 : 0});
         G.gemm.wg += ntr * ntc;
         const double m = r1 - r0, nn = j1 - j0, kk = k1 - k0;
         G.gemm.flops += 2.0 * m * nn * kk;
@@ -763,7 +766,7 @@ void run_factor(spkhip_handle* h) {
           h->cur_grid = G.gemm.wg;
           Prof pr(h, SPKHIP_K_UPDATE, G.gemm.flops, G.gemm.elems * sizeof(T));
           hipLaunchKernelGGL(spk::blk_gemm_kernel<T>, dim3(G.gemm.wg), dim3(kWG), 0, st, h->ds,
-                             h->d_gdesc.p + G.gemm.off, G.gemm.cnt, arena.p);
+                             h->d_gdesc.p + G.gemm.off, G.gemm.cnt, arena.p, h->lowp ? 1 : 0);
           hipck(hipGetLastError());
         }
       }
@@ -1380,11 +1383,12 @@ int64_t spkhip_inmatrix_device(spkhip_t h, const double* d_nzval) {
 int64_t spkhip_factor(spkhip_t h, int64_t precision) {
   if (guard(h)) return -4;
   if (!h->have_values) return -3;
-  if (precision != SPKHIP_FP64 && precision != SPKHIP_FP32) return -4;
+  if (precision != SPKHIP_FP64 && precision != SPKHIP_FP32 && precision != SPKHIP_FP32_BF16)
+    return -4;
   return wrap([&]() -> int64_t {
     hipck(hipSetDevice(h->device));
     hipck(hipEventRecord(h->ev0, h->stream));
-    const bool lu = h->kind == SPKHIP_KIND_LU;
+    h->lowp = precision == SPKHIP_FP32_BF16;
     if (precision == SPKHIP_FP64) {
       run_assemble<double>(h);
       run_factor<double>(h);
@@ -1392,13 +1396,16 @@ int64_t spkhip_factor(spkhip_t h, int64_t precision) {
       run_assemble<float>(h);
       run_factor<float>(h);
     }
+    h->lowp = false;
     hipck(hipEventRecord(h->ev1, h->stream));
     hipck(hipEventSynchronize(h->ev1));
     float ms = 0;
     hipck(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->t_factor = ms;
     prof_collect(h);
-    h->precision = (int)precision;
+    // the factor is stored in float after either float mode
+    h->precision = precision == SPKHIP_FP64 ? SPKHIP_FP64 : SPKHIP_FP32;
+    h->mode = (int)precision;
     h->factored = true;
     return h->first_fail ? -1 : 0;
   });

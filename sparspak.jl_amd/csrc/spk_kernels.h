@@ -20,6 +20,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace spk {
 
@@ -80,6 +81,7 @@ struct GemmDesc {
   int32_t j0, nc;    // output columns
   int32_t wg_begin;  // prefix sum of workgroups over the launch
   int32_t ntr;       // row tiles
+  int32_t big;       // the front has more than 256 own columns
 };
 
 // Per-workgroup descriptor of the blocked solve kernels.

@@ -1030,10 +1030,22 @@ struct Mfma16<float> {
   static __device__ inline int row(int lgrp, int v) { return 4 * lgrp + v; }
 };
 
+// float -> bf16, round to nearest even (finite inputs).
+__device__ inline unsigned short f2bf(float x) {
+  const unsigned u = __float_as_uint(x);
+  return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+
+// With lowp set, a float update of a front wider than 256 columns takes its
+// L21 / U12 operands rounded to bf16 and accumulates in float on
+// v_mfma_f32_16x16x16bf16_1k (operand k = 4 (l / 16) .. + 3 per lane, the
+// accumulator layout of the f32 form); refinement in double recovers the
+// accuracy.
 template <typename T>
 __global__ void __launch_bounds__(kWG, 2)
 blk_gemm_kernel(DevSym s, const GemmDesc* __restrict__ desc, int ndesc,
-                T* __restrict__ arena) {
+                T* __restrict__ arena, int lowp) {
   constexpr int TM = 128, KC = 16, KP = KC + 1, PER = TM * KC / kWG;  // 8 loads each
   typedef typename Mfma16<T>::v4 v4;
   // double-buffered: As[buf][i * KP + k] = L(r0 + i, k0 + k),
@@ -1085,6 +1097,45 @@ blk_gemm_kernel(DevSym s, const GemmDesc* __restrict__ desc, int ndesc,
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[a][c] = v4{T(0), T(0), T(0), T(0)};
   const int nch = (bw + KC - 1) / KC;
+  bool done = false;
+  if constexpr (std::is_same<T, float>::value) {
+    if (lowp && d.big) {
+      // bf16 operands in the same LDS: rows of KB = 20 halves (8-byte
+      // aligned groups of four k)
+      constexpr int KB = 20;
+      unsigned short* Ab = reinterpret_cast<unsigned short*>(&As[0][0]);
+      unsigned short* Bb = reinterpret_cast<unsigned short*>(&Bs[0][0]);
+      auto bstore = [&](int buf) {
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+          Ab[buf * TM * KB + ai * KB + ak + 2 * t] = f2bf(ra[t]);
+          Bb[buf * TM * KB + (bj + 16 * t) * KB + bk] = f2bf(rb[t]);
+        }
+      };
+      gload(0);
+      bstore(0);
+      __syncthreads();
+      for (int ch = 0; ch < nch; ++ch) {
+        const int cur = ch & 1;
+        if (ch + 1 < nch) gload((ch + 1) * KC);
+        bf16x4 av[4], bv[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          av[t] = *reinterpret_cast<const bf16x4*>(Ab + cur * TM * KB + (wr + t * 16 + l16) * KB + 4 * lk);
+          bv[t] = *reinterpret_cast<const bf16x4*>(Bb + cur * TM * KB + (wc + t * 16 + l16) * KB + 4 * lk);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bv[c], av[a], acc[a][c], 0, 0, 0);
+        if (ch + 1 < nch) bstore(cur ^ 1);
+        __syncthreads();
+      }
+      done = true;
+    }
+  }
+  if (!done) {
   gload(0);
   lstore(0);
   __syncthreads();
@@ -1107,6 +1158,7 @@ blk_gemm_kernel(DevSym s, const GemmDesc* __restrict__ desc, int ndesc,
     }
     if (ch + 1 < nch) lstore(cur ^ 1);
     __syncthreads();
+  }
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a) {

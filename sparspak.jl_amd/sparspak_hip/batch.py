@@ -142,7 +142,7 @@ class BatchedSolver:
         vec = B.ndim == 1
         B = np.asfortranarray(B.reshape(self.n, -1))
         if refine is None:
-            refine = self.precision == FP32
+            refine = self.precision != FP64
         L = self._dev.L
         if refine:
             rc = L.spkhip_solve_refine(self._dev.h, B.shape[1], pd(B), self.n, float(tol),

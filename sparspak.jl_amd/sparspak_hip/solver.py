@@ -23,8 +23,10 @@ from .symbolic import Graph, SymbolicFactor
 
 FP64 = 0
 FP32 = 1
+FP32_BF16 = 2  # float factor, bf16 operands in the trailing updates of large fronts
 _PREC = {"fp64": FP64, "float64": FP64, FP64: FP64,
-         "fp32": FP32, "float32": FP32, FP32: FP32}
+         "fp32": FP32, "float32": FP32, FP32: FP32,
+         "fp32_bf16": FP32_BF16, FP32_BF16: FP32_BF16}
 
 
 def _csc_arrays(m):
@@ -329,7 +331,7 @@ def triangularsolve(s, rhs=None, refine=None, tol=1e-14, max_iter=10):
     solve is in place and always runs. refine defaults to True for a float
     factor (double-precision iterative refinement)."""
     if refine is None:
-        refine = s.precision == FP32
+        refine = s.precision != FP64
     if rhs is None:
         if s._trisolvedone:
             return True

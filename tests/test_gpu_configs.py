@@ -162,3 +162,19 @@ def test_config5_value_sets_one_pattern():
     worst = max(float(np.max(backward_error(A, x, b))) for A, b, x in zip(mats, rhs, xs))
     _say(f"  worst eta {worst:.2e} iters {bs.stats().refine_iters}")
     assert worst <= 1e-14
+
+
+def test_bf16_trailing_updates_refine_to_1e14():
+    # float factor with bf16 operands in the trailing updates of fronts wider
+    # than 256 columns (the 24^3 top separator is), FP64 refinement
+    _say("case convdiff3d 24^3 fp32_bf16")
+    A = convdiff3d(24)
+    n = A.shape[0]
+    xt = uniform(2, n, -1, 1)
+    b = A @ xt
+    s = _solver(A, precision="fp32_bf16")
+    x = b.copy()
+    triangularsolve(s, x, refine=True, tol=1e-14, max_iter=30)
+    eta = backward_error(A, x, b)
+    _say(f"  eta {eta:.2e} iters {s.slvr.stats().refine_iters}")
+    assert eta <= 1e-14
