@@ -128,6 +128,15 @@ struct spkhip_handle {
   DevBuf<spk::GemmDesc> d_gdesc;
   DevBuf<i32> d_large, d_ut_front, d_ut_piece, d_ut_r0;
   DevBuf<spk::SolveDesc> d_sdesc;
+  // per-block solve data (packed triangles, composed interchanges), built
+  // once per factorisation at the first solve
+  std::vector<spk::SolveDesc> pdesc;
+  i64 prep_units = 0;  // solve data size in units of 4 elements
+  DevBuf<spk::SolveDesc> d_pdesc;
+  DevBuf<double> d_prep64;
+  DevBuf<float> d_prep32;
+  DevBuf<i32> d_prepi;
+  bool prepped = false;
   DevSym ds{};
   // matrix
   i64 nnz = 0;
@@ -368,6 +377,9 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
                 std::vector<spk::SolveDesc>& sdesc, std::vector<i32>& ut_front,
                 std::vector<i32>& ut_piece, std::vector<i32>& ut_r0) {
   const i32 nf = h->nfront;
+  h->pdesc.clear();
+  h->prep_units = 0;
+  h->prepped = false;
   std::vector<std::vector<i32>> bylev(h->nlevel);
   for (i32 f = 0; f < nf; ++f) bylev[h->flev[f]].push_back(f);
   h->levels.assign(h->nlevel, LevelPlan{});
@@ -556,15 +568,23 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
       G.gemm.cnt = (int)gdesc.size() - G.gemm.off;
       lp.groups.push_back(G);
     }
-    // blocked solve steps over the same blocks
+    // blocked solve steps over the same blocks; each block's solve data
+    // (index r0, offset 4 aux) is built once per factorisation
+    std::vector<std::vector<spk::SolveDesc>> bdata(large.size());
+    for (size_t fi = 0; fi < large.size(); ++fi)
+      for (const Blk& B : fblocks[fi]) {
+        const spk::SolveDesc D{large[fi], B.p0, B.np, B.c0, B.c1, (i32)h->pdesc.size(),
+                               (i32)h->prep_units};
+        h->pdesc.push_back(D);
+        bdata[fi].push_back(D);
+        const i64 ng = (B.c1 - B.c0 + 3) / 4;
+        h->prep_units += 4 * ng * (ng + 1) + ng;
+      }
     for (size_t gi = 0; gi < maxg; ++gi) {
       LevelPlan::SolveStep S;
       S.chain_off = (int)sdesc.size();
       for (size_t fi = 0; fi < large.size(); ++fi)
-        if (gi < fblocks[fi].size()) {
-          const Blk& B = fblocks[fi][gi];
-          sdesc.push_back(spk::SolveDesc{large[fi], B.p0, B.np, B.c0, B.c1, 0});
-        }
+        if (gi < fblocks[fi].size()) sdesc.push_back(bdata[fi][gi]);
       S.chain_cnt = (int)sdesc.size() - S.chain_off;
       S.gemv_off = (int)sdesc.size();
       for (size_t fi = 0; fi < large.size(); ++fi)
@@ -572,7 +592,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
           const Blk& B = fblocks[fi][gi];
           const i32 t = h->fL[large[fi]] - B.c1;
           for (i32 r0 = 0; r0 < t; r0 += 64)
-            sdesc.push_back(spk::SolveDesc{large[fi], B.p0, B.np, B.c0, B.c1, r0});
+            sdesc.push_back(spk::SolveDesc{large[fi], B.p0, B.np, B.c0, B.c1, r0, 0});
           S.elems += (double)(B.c1 - B.c0) * (h->fL[large[fi]] - B.c0);
         }
       S.gemv_cnt = (int)sdesc.size() - S.gemv_off;
@@ -587,7 +607,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         const Blk& B = fblocks[fi][bi];
         if (B.c1 >= h->fL[large[fi]]) continue;
         for (i32 r0 = 0; r0 < B.c1 - B.c0; r0 += 4)
-          sdesc.push_back(spk::SolveDesc{large[fi], B.p0, B.np, B.c0, B.c1, r0});
+          sdesc.push_back(spk::SolveDesc{large[fi], B.p0, B.np, B.c0, B.c1, r0, 0});
       }
       S.gemv_cnt = (int)sdesc.size() - S.gemv_off;
       S.chain_off = (int)sdesc.size();
@@ -595,7 +615,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         const i64 bi = (i64)fblocks[fi].size() - 1 - (i64)si;
         if (bi < 0) continue;
         const Blk& B = fblocks[fi][bi];
-        sdesc.push_back(spk::SolveDesc{large[fi], B.p0, B.np, B.c0, B.c1, 0});
+        sdesc.push_back(bdata[fi][bi]);
         S.elems += (double)(B.c1 - B.c0) * (h->fL[large[fi]] - B.c0);
       }
       S.chain_cnt = (int)sdesc.size() - S.chain_off;
@@ -665,6 +685,7 @@ struct Bufs<double> {
   static DevBuf<double>& arena(spkhip_handle* h) { return h->d_arena64; }
   static DevBuf<double>& vec(spkhip_handle* h) { return h->d_vec64; }
   static DevBuf<double>& rhs(spkhip_handle* h) { return h->d_rhs64; }
+  static DevBuf<double>& prep(spkhip_handle* h) { return h->d_prep64; }
 };
 template <>
 struct Bufs<float> {
@@ -673,6 +694,7 @@ struct Bufs<float> {
   static DevBuf<float>& arena(spkhip_handle* h) { return h->d_arena32; }
   static DevBuf<float>& vec(spkhip_handle* h) { return h->d_vec32; }
   static DevBuf<float>& rhs(spkhip_handle* h) { return h->d_rhs32; }
+  static DevBuf<float>& prep(spkhip_handle* h) { return h->d_prep32; }
 };
 
 template <typename T>
@@ -691,9 +713,11 @@ void set_lds_attrs() {
   hipck(hipFuncSetAttribute((const void*)spk::blk_offdiag_kernel<T, kNqSPD>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
   hipck(hipFuncSetAttribute((const void*)spk::fwd_chain_kernel<T>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            spk::kPanelElems * (int)sizeof(T)));
   hipck(hipFuncSetAttribute((const void*)spk::bwd_chain_kernel<T>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (spk::kPanelElems + 128) * (int)sizeof(T)));
   done = true;
 }
 
@@ -815,8 +839,20 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
   vec.ensure((size_t)std::max<i64>(h->vec_elems * nrhs, 1));
   const T* lnz = Bufs<T>::lnz(h).p;
   const T* unz = Bufs<T>::unz(h).p;
+  DevBuf<T>& prep = Bufs<T>::prep(h);
+  if (!h->prepped && !h->pdesc.empty()) {
+    prep.ensure((size_t)h->prep_units * 4);
+    h->d_prepi.ensure(h->pdesc.size() * spk::kPrepInts);
+    Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
+    hipLaunchKernelGGL(spk::solve_prep_kernel<T>, dim3((unsigned)h->pdesc.size()), dim3(kWG), 0,
+                       st, h->ds, h->d_pdesc.p, lnz, unz, h->d_ipvt.p, prep.p, h->d_prepi.p);
+    hipck(hipGetLastError());
+    h->prepped = true;
+  }
   if (fwd)
     for (const LevelPlan& lp : h->levels) {
+      h->cur_level = (int)(&lp - h->levels.data());
+      h->cur_grid = lp.small_cnt;
       if (lp.small_cnt > 0) {
         Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
         hipLaunchKernelGGL((spk::fwd_small_kernel<T, MAXW>), dim3(lp.small_cnt), dim3(kWG),
@@ -835,8 +871,9 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
         {
           Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
           hipLaunchKernelGGL((spk::fwd_chain_kernel<T>), dim3(S.chain_cnt), dim3(kWG),
-                             (size_t)128 * spk::kRLD * sizeof(T), st, h->ds, h->d_sdesc.p + S.chain_off, lnz, h->d_ipvt.p, rhs, ldr,
-                             nrhs, vec.p);
+                             (size_t)spk::kPanelElems * sizeof(T), st, h->ds,
+                             h->d_sdesc.p + S.chain_off, prep.p, h->d_prepi.p, rhs, ldr, nrhs,
+                             vec.p);
           hipck(hipGetLastError());
         }
         if (S.gemv_cnt > 0) {
@@ -850,6 +887,14 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
   if (bwd)
     for (int l = (int)h->levels.size() - 1; l >= 0; --l) {
       const LevelPlan& lp = h->levels[l];
+      h->cur_level = l;
+      h->cur_grid = lp.large_cnt;
+      if (lp.large_cnt > 0) {
+        Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);
+        hipLaunchKernelGGL(spk::bwd_gather_kernel<T>, dim3(lp.large_cnt), dim3(kWG), 0, st,
+                           h->ds, h->d_large.p + lp.large_off, rhs, ldr, nrhs, vec.p);
+        hipck(hipGetLastError());
+      }
       for (const LevelPlan::SolveStep& S : lp.bsteps) {
         if (S.gemv_cnt > 0) {
           Prof pr(h, SPKHIP_K_BWD_GEMV, 2.0 * S.elems * nrhs, S.elems * sizeof(T));
@@ -859,7 +904,8 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
         }
         Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);
         hipLaunchKernelGGL((spk::bwd_chain_kernel<T>), dim3(S.chain_cnt), dim3(kWG),
-                           (size_t)128 * spk::kRLD * sizeof(T), st, h->ds, h->d_sdesc.p + S.chain_off, lnz, unz, rhs, ldr, nrhs, vec.p);
+                           (size_t)(spk::kPanelElems + 128) * sizeof(T), st, h->ds,
+                           h->d_sdesc.p + S.chain_off, prep.p, rhs, ldr, nrhs, vec.p);
         hipck(hipGetLastError());
       }
       if (lp.small_cnt > 0) {
@@ -1040,6 +1086,7 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   h->d_ut_piece.upload(ut_piece, st);
   h->d_ut_r0.upload(ut_r0, st);
   h->d_sdesc.upload(sdesc, st);
+  if (!h->pdesc.empty()) h->d_pdesc.upload(h->pdesc, st);
   DevSym& d = h->ds;
   d.n = n;
   d.nsuper = nsuper;
@@ -1184,6 +1231,7 @@ int64_t lufactor_stateless(i64 n, i64 nsuper, const i64* xsuper, const i64* snod
     run_factor<T>(h);
     h->precision = prec;
     h->factored = true;
+    h->prepped = false;
     download_impl<T>(h, lnz, kind == SPKHIP_KIND_LU ? unz : nullptr, ipvt);
     return h->first_fail ? -1 : 0;
   });
@@ -1407,6 +1455,7 @@ int64_t spkhip_factor(spkhip_t h, int64_t precision) {
     h->precision = precision == SPKHIP_FP64 ? SPKHIP_FP64 : SPKHIP_FP32;
     h->mode = (int)precision;
     h->factored = true;
+    h->prepped = false;
     return h->first_fail ? -1 : 0;
   });
 }

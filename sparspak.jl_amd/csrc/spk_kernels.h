@@ -84,13 +84,19 @@ struct GemmDesc {
   int32_t big;       // the front has more than 256 own columns
 };
 
+// Integers of a block's solve data: the composed row permutation.
+constexpr int kPrepInts = 128;
+
 // Per-workgroup descriptor of the blocked solve kernels.
 struct SolveDesc {
   int32_t front;
   int32_t piece;   // first piece of the block
   int32_t np;      // pieces in the block
   int32_t c0, c1;  // block columns [c0, c1), front-local
-  int32_t r0;      // gemv: first row of the workgroup's tile (relative)
+  int32_t r0;      // gemv: first row of the workgroup's tile (relative);
+                   // chain and prep: the block's index in the solve data
+  int32_t aux;     // chain and prep: the block's solve data starts at
+                   // element 4 * aux
 };
 
 }  // namespace spk

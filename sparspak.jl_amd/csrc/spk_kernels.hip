@@ -1371,6 +1371,114 @@ fwd_assemble_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restr
   }
 }
 
+// Per-block solve data, built once per factorisation and read by every
+// solve (refinement solves many times). A block of bw columns is padded to
+// bwp = 4 ng columns, ng = ceil(bw / 4), in ng panels of four columns.
+//  - Lower part (8 ng (ng + 1) elements): panel g holds rows 4g .. bwp-1 of
+//    columns 4g .. 4g+3 of the unit-lower factor, four values per row, at
+//    lo_panel(g, ng); entries on or above the diagonal and padding are 0.
+//    The rows of every piece carry that piece's interchanges also in the
+//    columns of earlier pieces: the reference interchanges y's rows when it
+//    reaches a piece, after earlier pieces have updated them; interchanging
+//    y once at the start and the stored L rows instead does the same
+//    operations on the same values.
+//  - Upper part (8 ng (ng + 1) elements): panel g holds rows 0 .. 4g+3 of
+//    columns 4g .. 4g+3 of U, at up_panel(g); entries below the diagonal are
+//    0, padding columns are unit columns.
+//  - The bwp reciprocals of U's diagonal (1 for padding).
+// In all 16 ng (ng + 1) + 4 ng elements, from element 4 desc.aux. The
+// integer part (kPrepInts per block, from desc.r0 kPrepInts) is the row each
+// position takes after its piece's interchanges (composed, glaswp! order).
+__device__ inline int lo_panel(int g, int ng) { return 4 * (4 * ng * g - 2 * g * (g - 1)); }
+__device__ inline int up_panel(int g) { return 8 * g * (g + 1); }
+constexpr int kPanelElems = 8 * 32 * 33;  // one part of a 128-column block
+
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<double> {
+  typedef double __attribute__((ext_vector_type(4))) type;
+  typedef double __attribute__((ext_vector_type(2))) half;
+};
+template <>
+struct Vec4<float> {
+  typedef float __attribute__((ext_vector_type(4))) type;
+  typedef float __attribute__((ext_vector_type(2))) half;
+};
+
+// One workgroup per block.
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+solve_prep_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ lnz,
+                  const T* __restrict__ unz, const int64_t* __restrict__ ipvt,
+                  T* __restrict__ prep, int32_t* __restrict__ prepi) {
+  __shared__ int spiv[128];
+  __shared__ int sperm[128];
+  __shared__ int pstart[128];
+  __shared__ int pbeg[9];
+  __shared__ int64_t lb[128], ub[128];
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const SolveDesc d = desc[blockIdx.x];
+  const int64_t fc0 = s.fc0[d.front];
+  const int c0 = d.c0, bw = d.c1 - d.c0;
+  const int ng = (bw + 3) / 4, bwp = 4 * ng;
+  const int64_t g0 = fc0 + c0;
+  T* lo = prep + 4 * (int64_t)d.aux;
+  T* up = lo + 8 * ng * (ng + 1);
+  T* rinv = up + 8 * ng * (ng + 1);
+  for (int i = tid; i < bw; i += kWG) {
+    const int ps = piece_start_of_col(s, fc0, c0 + i) - c0;
+    pstart[i] = ps;
+    spiv[i] = ps + (int)ipvt[g0 + i] - 1;
+    sperm[i] = i;
+    lb[i] = s.lcol[g0 + i] + c0;
+    ub[i] = s.ucol[g0 + i] + c0;
+  }
+  if (tid <= d.np) pbeg[tid] = (tid < d.np) ? (int)(s.xsuper[d.piece + tid] - 1 - g0) : bw;
+  __syncthreads();
+  if (wave == 0 && lane < d.np) {
+    for (int k = pbeg[lane]; k < pbeg[lane + 1]; ++k) {
+      const int kp = spiv[k];
+      if (kp != k) {
+        const int t = sperm[k];
+        sperm[k] = sperm[kp];
+        sperm[kp] = t;
+      }
+    }
+  }
+  __syncthreads();
+  for (int g = wave; g < ng; g += 4) {
+    T* P = lo + lo_panel(g, ng);
+    const int ne = (bwp - 4 * g) * 4;
+#pragma unroll 4
+    for (int i = lane; i < ne; i += 64) {
+      const int r = 4 * g + i / 4, c = 4 * g + i % 4;
+      T v = T(0);
+      if (r > c && r < bw && c < bw) v = lnz[lb[c] + (pstart[r] == pstart[c] ? r : sperm[r])];
+      P[i] = v;
+    }
+  }
+  for (int g = wave; g < ng; g += 4) {
+    T* P = up + up_panel(g);
+    const int ne = (4 * g + 4) * 4;
+#pragma unroll 4
+    for (int i = lane; i < ne; i += 64) {
+      const int r = i / 4, c = 4 * g + i % 4;
+      T v = T(0);
+      if (r <= c) {
+        if (c < bw)
+          v = (r >= pstart[c]) ? lnz[lb[c] + r] : unz[ub[r] + c];
+        else if (r == c)
+          v = T(1);
+      }
+      P[i] = v;
+    }
+  }
+  for (int c = tid; c < bwp; c += kWG) rinv[c] = (c < bw) ? T(1) / lnz[lb[c] + c] : T(1);
+  int32_t* pi = prepi + (int64_t)d.r0 * kPrepInts;
+  for (int i = tid; i < bw; i += kWG) pi[i] = sperm[i];
+}
+
 // Value i (0 <= i < 128, wave-uniform) of a vector held two entries per lane
 // (entry i in lane i % 64 of y0 for i < 64, of y1 otherwise).
 template <typename T>
@@ -1385,135 +1493,148 @@ __device__ inline T gather2(T y0, T y1, int src) {
   return src < 64 ? a : b;
 }
 
+// v, computed where it stands for every lane (keeps the compiler from
+// turning a select of it into a branch around its computation).
+__device__ inline double pin(double v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+__device__ inline float pin(float v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// The solve data of a block (nv four-element vectors) into LDS, all loads in
+// flight at once; at most kPrepV vectors per thread.
+constexpr int kPrepV = 9;
+template <typename V>
+__device__ inline void load_panels(const V* __restrict__ src, V* dst, int nv) {
+  V buf[kPrepV];
+#pragma unroll
+  for (int u = 0; u < kPrepV; ++u) {
+    const int i = threadIdx.x + u * kWG;
+    if (i < nv) buf[u] = src[i];
+  }
+#pragma unroll
+  for (int u = 0; u < kPrepV; ++u) {
+    const int i = threadIdx.x + u * kWG;
+    if (i < nv) dst[i] = buf[u];
+  }
+}
+
 // Forward chain of one block (one workgroup per front of the step). The
-// block's strictly lower triangle is loaded into LDS once (leading dimension
-// kRLD, every column from its lnz panel) and each piece's interchanges are
-// composed into one row permutation. Wave 0 then holds the block's entries
-// of y in registers, two per lane, and per piece applies the permutation
-// (cross-lane shuffles) and eliminates the piece's columns, four at a time,
-// from every later row of the block: the four new entries come from a small
-// unit-lower solve every lane does alike, then each row takes its four
+// block's lower panels come from the solve data in one coalesced load. Wave
+// 0 holds y in registers, two entries per lane (rows lane, lane + 64),
+// takes the block's interchanges at once (cross-lane shuffles) and
+// eliminates four columns per step: the four new entries from a small
+// unit-lower solve every lane does alike, then every row takes its four
 // products in column order — the same operations, in the same order, as
-// one column at a time. Dynamic LDS: 128 * kRLD elements.
+// the reference's column-by-column piece solves and in-block L21 updates.
+// A row at or above the step's first column reads the panel's zero row
+// (its own four products are then exact no-ops), so the three rows below
+// the first come out as the small solve's entries and no lane branches.
+// The next step's panel values are read from LDS while the current step
+// computes; once the first 64 rows are final only the second half updates.
+template <typename T>
+struct FwdStep {
+  T t10, t32;                        // L(k+1, k), L(k+3, k+2)
+  typename Vec4<T>::half t2, t3;     // L(k+2, k:k+2), L(k+3, k:k+2)
+  typename Vec4<T>::type a, b;       // the lane's two rows
+};
+
 template <typename T>
 __global__ void __launch_bounds__(kWG)
-fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ lnz,
-                 const int64_t* __restrict__ ipvt, T* __restrict__ rhs, int64_t ldr, int nrhs,
+fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ prep,
+                 const int32_t* __restrict__ prepi, T* __restrict__ rhs, int64_t ldr, int nrhs,
                  T* __restrict__ vec) {
-  constexpr int LD = kRLD;
-  extern __shared__ unsigned char smem_raw[];
-  T* R = reinterpret_cast<T*>(smem_raw);
-  __shared__ int spiv[128];     // block-local row swapped with row i at its step
-  __shared__ int sperm[128];    // block-local row that position i takes
-  __shared__ int64_t lb[128];   // lnz offset of (front row c0, column g0 + c)
-  __shared__ int pbeg[9];       // piece boundaries, block-local
+  using V = typename Vec4<T>::type;
+  extern __shared__ __attribute__((aligned(32))) unsigned char smem_a32[];
+  V* smem_v4 = reinterpret_cast<V*>(smem_a32);
+  const T* R = reinterpret_cast<const T*>(smem_a32);
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
   const int L = s.fL[f];
   const int64_t fc0 = s.fc0[f];
-  const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
-  const int64_t g0 = fc0 + c0;
-  for (int i = tid; i < bw; i += kWG) {
-    lb[i] = s.lcol[g0 + i] + c0;
-    const int ps = piece_start_of_col(s, fc0, c0 + i) - c0;
-    spiv[i] = ps + (int)ipvt[g0 + i] - 1;
-    sperm[i] = i;
-  }
-  if (tid <= d.np) pbeg[tid] = (tid < d.np) ? (int)(s.xsuper[d.piece + tid] - 1 - g0) : bw;
-  __syncthreads();
+  const int c0 = d.c0, bw = d.c1 - d.c0;
+  const int ng = (bw + 3) / 4, bwp = 4 * ng;
+  const bool in0 = lane < bw, in1 = lane + 64 < bw;
+  const int32_t* pi = prepi + (int64_t)d.r0 * kPrepInts;
+  // wave 0: permutation and first right-hand side, in flight with the panels
+  int s0 = lane, s1 = lane + 64;
+  T y0r = T(0), y1r = T(0);
+  const T* yv = vec + s.fvec[f] * nrhs;
   if (wave == 0) {
-    // one lane per piece composes its interchanges (glaswp! order)
-    if (lane < d.np) {
-      for (int k = pbeg[lane]; k < pbeg[lane + 1]; ++k) {
-        const int kp = spiv[k];
-        if (kp != k) {
-          const int t = sperm[k];
-          sperm[k] = sperm[kp];
-          sperm[kp] = t;
-        }
-      }
+    if (in0) {
+      s0 = pi[lane];
+      y0r = yv[c0 + lane];
     }
-  } else {
-    // strictly lower triangle: R[r + c LD] = L(c0 + r, g0 + c), r > c;
-    // waves 1-3, rows lane and lane + 64, 8 columns per batch
-    for (int cb = wave - 1; cb < bw; cb += 24) {
-      T v[8][2];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int c = cb + 3 * u;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int r = lane + 64 * h;
-          v[u][h] = (c < bw && r < bw && r > c) ? lnz[lb[c] + r] : T(0);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int c = cb + 3 * u;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int r = lane + 64 * h;
-          if (c < bw && r < bw && r > c) R[r + c * LD] = v[u][h];
-        }
-      }
+    if (in1) {
+      s1 = pi[lane + 64];
+      y1r = yv[c0 + 64 + lane];
     }
   }
+  load_panels(reinterpret_cast<const V*>(prep + 4 * (int64_t)d.aux), smem_v4,
+              2 * ng * (ng + 1));
   __syncthreads();
   if (wave != 0) return;
-  const bool in0 = lane < bw, in1 = lane + 64 < bw;
+  const int gA = min(ng, 16);
+  // panel values of step g (rows clamped into the panel)
+  auto load = [&](int g, FwdStep<T>& o, bool both) {
+    const int k = 4 * g;
+    const T* P = R + lo_panel(g, ng);
+    using H = typename Vec4<T>::half;
+    o.t10 = P[4];
+    o.t2 = *(const H*)(P + 8);
+    o.t3 = *(const H*)(P + 12);
+    o.t32 = P[14];
+    if (both) o.a = *(const V*)(P + 4 * (min(max(lane, k), bwp - 1) - k));
+    o.b = *(const V*)(P + 4 * (min(max(lane + 64, k), bwp - 1) - k));
+  };
   for (int q = 0; q < nrhs; ++q) {
     T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
     T* x = rhs + (int64_t)q * ldr;
-    T y0 = in0 ? yg[c0 + lane] : T(0);
-    T y1 = in1 ? yg[c0 + 64 + lane] : T(0);
-    for (int pi = 0; pi < d.np; ++pi) {
-      const int o = pbeg[pi], e = pbeg[pi + 1];
-      {  // the piece's rows to their positions
-        const int s0 = (lane >= o && lane < e) ? sperm[lane] : lane;
-        const int s1 = (lane + 64 >= o && lane + 64 < e) ? sperm[lane + 64] : lane + 64;
-        const T n0 = gather2(y0, y1, s0), n1 = gather2(y0, y1, s1);
-        y0 = n0;
-        y1 = n1;
-      }
-      int k = o;
-      for (; k + 4 <= e; k += 4) {
-        const T b0 = bcast2(y0, y1, k), b1 = bcast2(y0, y1, k + 1);
-        const T b2 = bcast2(y0, y1, k + 2), b3 = bcast2(y0, y1, k + 3);
-        const T* Rk = R + k * LD;
-        const T z0 = b0;
-        const T z1 = b1 - Rk[k + 1] * z0;
-        const T z2 = (b2 - Rk[k + 2] * z0) - Rk[LD + k + 2] * z1;
-        const T z3 = ((b3 - Rk[k + 3] * z0) - Rk[LD + k + 3] * z1) - Rk[2 * LD + k + 3] * z2;
-        const T* r0 = Rk + lane;
-        const T* r1 = Rk + lane + 64;
-        if (in0) {
-          if (lane > k + 3)
-            y0 = (((y0 - r0[0] * z0) - r0[LD] * z1) - r0[2 * LD] * z2) - r0[3 * LD] * z3;
-          else if (lane == k + 1)
-            y0 = z1;
-          else if (lane == k + 2)
-            y0 = z2;
-          else if (lane == k + 3)
-            y0 = z3;
-        }
-        if (in1) {
-          const int r = lane + 64;
-          if (r > k + 3)
-            y1 = (((y1 - r1[0] * z0) - r1[LD] * z1) - r1[2 * LD] * z2) - r1[3 * LD] * z3;
-          else if (r == k + 1)
-            y1 = z1;
-          else if (r == k + 2)
-            y1 = z2;
-          else if (r == k + 3)
-            y1 = z3;
-        }
-      }
-      for (; k < e; ++k) {
-        const T yk = bcast2(y0, y1, k);
-        const T l0 = R[lane + k * LD], l1 = R[lane + 64 + k * LD];
-        if (in0 && lane > k) y0 -= l0 * yk;
-        if (in1 && lane + 64 > k) y1 -= l1 * yk;
+    if (q > 0) {
+      y0r = in0 ? yg[c0 + lane] : T(0);
+      y1r = in1 ? yg[c0 + 64 + lane] : T(0);
+    }
+    T y0 = gather2(y0r, y1r, s0), y1 = gather2(y0r, y1r, s1);
+    FwdStep<T> A, B;
+    // pivots in rows 0..63: both halves update
+    auto stepA = [&](int g, const FwdStep<T>& c, FwdStep<T>& n) {
+      const int k = 4 * g;
+      load(min(g + 1, ng - 1), n, true);
+      const T b0 = rdlane(y0, k), b1 = rdlane(y0, k + 1);
+      const T b2 = rdlane(y0, k + 2), b3 = rdlane(y0, k + 3);
+      const T z0 = b0;
+      const T z1 = b1 - c.t10 * z0;
+      const T z2 = (b2 - c.t2.x * z0) - c.t2.y * z1;
+      const T z3 = ((b3 - c.t3.x * z0) - c.t3.y * z1) - c.t32 * z2;
+      y0 = (((y0 - c.a.x * z0) - c.a.y * z1) - c.a.z * z2) - c.a.w * z3;
+      y1 = (((y1 - c.b.x * z0) - c.b.y * z1) - c.b.z * z2) - c.b.w * z3;
+    };
+    load(0, A, true);
+    for (int g = 0; g < gA; g += 2) {
+      stepA(g, A, B);
+      if (g + 1 < gA) stepA(g + 1, B, A);
+    }
+    // pivots in rows 64..127: only the second half updates
+    auto stepB = [&](int g, const FwdStep<T>& c, FwdStep<T>& n) {
+      const int k = 4 * g - 64;
+      load(min(g + 1, ng - 1), n, false);
+      const T b0 = rdlane(y1, k), b1 = rdlane(y1, k + 1);
+      const T b2 = rdlane(y1, k + 2), b3 = rdlane(y1, k + 3);
+      const T z0 = b0;
+      const T z1 = b1 - c.t10 * z0;
+      const T z2 = (b2 - c.t2.x * z0) - c.t2.y * z1;
+      const T z3 = ((b3 - c.t3.x * z0) - c.t3.y * z1) - c.t32 * z2;
+      y1 = (((y1 - c.b.x * z0) - c.b.y * z1) - c.b.z * z2) - c.b.w * z3;
+    };
+    if (ng > 16) {
+      load(16, A, false);
+      for (int g = 16; g < ng; g += 2) {
+        stepB(g, A, B);
+        if (g + 1 < ng) stepB(g + 1, B, A);
       }
     }
     if (in0) {
@@ -1554,8 +1675,36 @@ fwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   }
 }
 
-// Backward: t[c] = U[c, c1:L] x[I[c1:L]] for rows c = c0 + r0 + wave of
-// the block (one wave per row), stored in y[c].
+// Backward: x of a large front's update rows, x[I[W:L]], gathered into the
+// front's vector (rows W..L-1; the forward pass no longer needs them), so
+// that the backward GEMVs read it contiguously. One workgroup per front.
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+bwd_gather_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict__ rhs,
+                  int64_t ldr, int nrhs, T* __restrict__ vec) {
+  const int f = list[blockIdx.x];
+  const int L = s.fL[f], W = s.fW[f];
+  const int32_t* I = s.lindx + (s.xlindx[s.fsp0[f]] - 1);
+  for (int q = 0; q < nrhs; ++q) {
+    const T* x = rhs + (int64_t)q * ldr;
+    T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
+    for (int j0 = W + threadIdx.x; j0 < L; j0 += 4 * kWG) {
+      int g[4];
+      T v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) g[u] = (j0 + u * kWG < L) ? I[j0 + u * kWG] - 1 : 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = (j0 + u * kWG < L) ? x[g[u]] : T(0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (j0 + u * kWG < L) yg[j0 + u * kWG] = v[u];
+    }
+  }
+}
+
+// Backward: t[c] = U[c, c1:L] x[c1:L] for rows c = c0 + r0 + wave of the
+// block (one wave per row), stored in y[c]; x of the front's own columns
+// from rhs, of its update rows from the front's vector (bwd_gather_kernel).
 template <typename T>
 __global__ void __launch_bounds__(kWG)
 bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ unz,
@@ -1565,147 +1714,168 @@ bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   const int f = d.front;
   const int c = d.c0 + d.r0 + wave;
   if (c >= d.c1) return;
-  const int L = s.fL[f];
+  const int L = s.fL[f], W = s.fW[f];
   const int64_t fc0 = s.fc0[f];
-  const int32_t* I = s.lindx + (s.xlindx[s.fsp0[f]] - 1);
   const T* u = unz + s.ucol[fc0 + c];
   for (int q = 0; q < nrhs; ++q) {
-    const T* x = rhs + (int64_t)q * ldr;
+    const T* xo = rhs + (int64_t)q * ldr + fc0;  // own columns, front-local
     T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
     T acc = T(0);
-    int j = d.c1 + lane;
-    for (; j + 192 < L; j += 256) {
-      int g[4];
-      T uv[4];
+    for (int j0 = d.c1 + lane; j0 < L; j0 += 8 * 64) {
+      T uv[8], xv[8];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        g[t] = I[j + 64 * t] - 1;
-        uv[t] = u[j + 64 * t];
+      for (int t = 0; t < 8; ++t) {
+        const int j = j0 + 64 * t;
+        uv[t] = (j < L) ? u[j] : T(0);
+        xv[t] = (j < L) ? (j < W ? xo[j] : yg[j]) : T(0);
       }
 #pragma unroll
-      for (int t = 0; t < 4; ++t) acc += uv[t] * x[g[t]];
+      for (int t = 0; t < 8; ++t) acc += uv[t] * xv[t];
     }
-    for (; j < L; j += 64) acc += u[j] * x[I[j] - 1];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
     if (lane == 0) yg[c] = acc;
   }
 }
 
-// Backward chain of one block. The block's upper triangle is loaded into
-// LDS once: inside a piece's diagonal block from the packed LU in lnz, right
-// of it from the row's unz panel. With t = U[c0:c1, c1:L] x[c1:L] already
-// in the vector arena (bwd_bgemv_kernel), wave 0 holds y = x - t in
-// registers and back-substitutes over the whole block, last column first,
-// four columns per step: the four new entries x_k = (y_k - ...) / U(k, k)
-// every lane computes alike, then each earlier row takes its four products
-// in column order — the in-block U12 products and the upper block solves
-// of the reference, by columns. Dynamic LDS: 128 * kRLD elements.
+// b / u from r = 1 / u (correctly rounded): q0 = b r, then one correction
+// with the exact remainder b - q0 u (an FMA) gives the correctly rounded
+// quotient b / u, as the division itself (Markstein's theorem; no overflow
+// or underflow assumed), in three dependent operations.
+__device__ inline double div_r(double b, double u, double r) {
+  const double q0 = b * r;
+  return __builtin_fma(__builtin_fma(-q0, u, b), r, q0);
+}
+__device__ inline float div_r(float b, float u, float r) {
+  const float q0 = b * r;
+  return __builtin_fmaf(__builtin_fmaf(-q0, u, b), r, q0);
+}
+
+// Backward chain of one block. The block's upper panels and diagonal
+// reciprocals come from the solve data in one coalesced load. With t =
+// U[c0:c1, c1:L] x[c1:L] already in the vector arena (bwd_bgemv_kernel),
+// wave 0 holds y = x - t in registers and back-substitutes, last column
+// first, four columns per step: the four new entries x_k = (y_k - ...) /
+// U(k, k) every lane computes alike, then each earlier row takes its four
+// products in column order — the in-block U12 products and the upper block
+// solves of the reference, by columns. The next step's panel values are
+// read from LDS while the current step computes.
+template <typename T>
+struct BwdStep {
+  typename Vec4<T>::type d0, ri, a, b;  // U(k, k:k+4), 1 / U(j, j), the lane's rows
+  typename Vec4<T>::half d1, d2;        // U(k+1, k+2:k+4), U(k+2, k+2:k+4)
+  T d11, d33;                           // U(k+1, k+1), U(k+3, k+3)
+};
+
 template <typename T>
 __global__ void __launch_bounds__(kWG)
-bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ lnz,
-                 const T* __restrict__ unz, T* __restrict__ rhs, int64_t ldr, int nrhs,
-                 T* __restrict__ vec) {
-  constexpr int LD = kRLD;
-  extern __shared__ unsigned char smem_raw[];
-  T* R = reinterpret_cast<T*>(smem_raw);
-  __shared__ int pstart[128];   // block-local first column of each column's piece
-  __shared__ int64_t lb[128];   // lnz offset of (front row c0, column g0 + c)
-  __shared__ int64_t ub[128];   // unz offset of (row g0 + r, front column c0)
+bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ prep,
+                 T* __restrict__ rhs, int64_t ldr, int nrhs, T* __restrict__ vec) {
+  using V = typename Vec4<T>::type;
+  extern __shared__ __attribute__((aligned(32))) unsigned char smem_a32[];
+  V* smem_v4 = reinterpret_cast<V*>(smem_a32);
+  const T* R = reinterpret_cast<const T*>(smem_a32);
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
   const int L = s.fL[f];
   const int64_t fc0 = s.fc0[f];
   const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
-  const int64_t g0 = fc0 + c0;
-  for (int i = tid; i < bw; i += kWG) {
-    pstart[i] = piece_start_of_col(s, fc0, c0 + i) - c0;
-    lb[i] = s.lcol[g0 + i] + c0;
-    ub[i] = s.ucol[g0 + i] + c0;
-  }
-  __syncthreads();
-  // upper triangle: R[r + c LD] = U(g0 + r, c0 + c), r <= c; inside the
-  // column's piece from the packed LU, before it from the row's U panel
-  for (int cb = wave; cb < bw; cb += 32) {
-    T v[8][2];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int c = cb + 4 * u;
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int r = lane + 64 * h;
-        v[u][h] = T(0);
-        if (c < bw && r <= c) v[u][h] = (r >= pstart[c]) ? lnz[lb[c] + r] : unz[ub[r] + c];
-      }
+  const int ng = (bw + 3) / 4;
+  const bool in0 = lane < bw, in1 = lane + 64 < bw, sub = c1 < L;
+  T x0r = T(0), x1r = T(0), t0r = T(0), t1r = T(0);
+  if (wave == 0) {
+    const T* x = rhs + fc0 + c0;
+    const T* yg = vec + s.fvec[f] * nrhs + c0;
+    if (in0) {
+      x0r = x[lane];
+      if (sub) t0r = yg[lane];
     }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int c = cb + 4 * u;
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int r = lane + 64 * h;
-        if (c < bw && r <= c) R[r + c * LD] = v[u][h];
-      }
+    if (in1) {
+      x1r = x[lane + 64];
+      if (sub) t1r = yg[lane + 64];
     }
   }
+  const int npan = 8 * ng * (ng + 1);
+  load_panels(reinterpret_cast<const V*>(prep + 4 * (int64_t)d.aux + npan), smem_v4,
+              2 * ng * (ng + 1) + ng);
   __syncthreads();
   if (wave != 0) return;
-  const bool in0 = lane < bw, in1 = lane + 64 < bw;
+  const T* Rinv = R + npan;
+  const int gA = min(ng, 16);
+  auto load = [&](int g, BwdStep<T>& o, bool both) {
+    const int k = 4 * g;
+    const T* P = R + up_panel(g);
+    using H = typename Vec4<T>::half;
+    o.d0 = *(const V*)(P + 4 * k);
+    o.d11 = P[4 * k + 5];
+    o.d1 = *(const H*)(P + 4 * k + 6);
+    o.d2 = *(const H*)(P + 4 * k + 10);
+    o.d33 = P[4 * k + 15];
+    o.ri = *(const V*)(Rinv + k);
+    o.a = *(const V*)(P + 4 * min(lane, k + 3));
+    if (both) o.b = *(const V*)(P + 4 * min(lane + 64, k + 3));
+  };
   for (int q = 0; q < nrhs; ++q) {
     T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
     T* x = rhs + (int64_t)q * ldr;
-    const bool sub = c1 < L;
-    T y0 = in0 ? x[fc0 + c0 + lane] - (sub ? yg[c0 + lane] : T(0)) : T(0);
-    T y1 = in1 ? x[fc0 + c0 + 64 + lane] - (sub ? yg[c0 + 64 + lane] : T(0)) : T(0);
-    int k = bw - 1;
-    for (; k >= 3; k -= 4) {
-      // columns k, k-1, k-2, k-3
-      const T b0 = bcast2(y0, y1, k), b1 = bcast2(y0, y1, k - 1);
-      const T b2 = bcast2(y0, y1, k - 2), b3 = bcast2(y0, y1, k - 3);
-      const T* R0 = R + k * LD;          // column k
-      const T* R1 = R0 - LD;             // column k - 1
-      const T* R2 = R0 - 2 * LD;
-      const T* R3 = R0 - 3 * LD;
-      const T x0 = b0 / R0[k];
-      const T x1 = (b1 - R0[k - 1] * x0) / R1[k - 1];
-      const T x2 = ((b2 - R0[k - 2] * x0) - R1[k - 2] * x1) / R2[k - 2];
-      const T x3 = (((b3 - R0[k - 3] * x0) - R1[k - 3] * x1) - R2[k - 3] * x2) / R3[k - 3];
-      if (in0) {
-        if (lane < k - 3)
-          y0 = (((y0 - R0[lane] * x0) - R1[lane] * x1) - R2[lane] * x2) - R3[lane] * x3;
-        else if (lane == k)
-          y0 = x0;
-        else if (lane == k - 1)
-          y0 = x1;
-        else if (lane == k - 2)
-          y0 = x2;
-        else if (lane == k - 3)
-          y0 = x3;
-      }
-      if (in1) {
-        const int r = lane + 64;
-        if (r < k - 3)
-          y1 = (((y1 - R0[r] * x0) - R1[r] * x1) - R2[r] * x2) - R3[r] * x3;
-        else if (r == k)
-          y1 = x0;
-        else if (r == k - 1)
-          y1 = x1;
-        else if (r == k - 2)
-          y1 = x2;
-        else if (r == k - 3)
-          y1 = x3;
+    if (q > 0) {
+      x0r = in0 ? x[fc0 + c0 + lane] : T(0);
+      x1r = in1 ? x[fc0 + c0 + 64 + lane] : T(0);
+      t0r = (in0 && sub) ? yg[c0 + lane] : T(0);
+      t1r = (in1 && sub) ? yg[c0 + 64 + lane] : T(0);
+    }
+    T y0 = in0 ? x0r - t0r : T(0);
+    T y1 = in1 ? x1r - t1r : T(0);
+    BwdStep<T> A, B;
+    // the step's four new entries from its pivot values b0..b3
+    auto solve4 = [&](const BwdStep<T>& c, T b0, T b1, T b2, T b3, T& x0, T& x1, T& x2,
+                      T& x3) {
+      x3 = div_r(b3, c.d33, c.ri.w);
+      x2 = div_r(b2 - c.d2.y * x3, c.d2.x, c.ri.z);
+      x1 = div_r((b1 - c.d1.y * x3) - c.d1.x * x2, c.d11, c.ri.y);
+      x0 = div_r(((b0 - c.d0.w * x3) - c.d0.z * x2) - c.d0.y * x1, c.d0.x, c.ri.x);
+    };
+    // row update of y at offset dl from the step's first column: rows above
+    // take the four products, the step's rows their new entries
+    auto upd = [&](T y, const V& a, int dl, T x0, T x1, T x2, T x3) {
+      T r = pin((((y - a.w * x3) - a.z * x2) - a.y * x1) - a.x * x0);
+      r = (dl == 0) ? x0 : r;
+      r = (dl == 1) ? x1 : r;
+      r = (dl == 2) ? x2 : r;
+      r = (dl == 3) ? x3 : r;
+      r = (dl >= 4) ? y : r;
+      return r;
+    };
+    // pivots in rows 64..127: every row of the first half updates
+    auto stepB = [&](int g, const BwdStep<T>& c, BwdStep<T>& n) {
+      const int k = 4 * g;
+      load(max(g - 1, 0), n, true);
+      T x0, x1, x2, x3;
+      solve4(c, rdlane(y1, k - 64), rdlane(y1, k - 63), rdlane(y1, k - 62), rdlane(y1, k - 61),
+             x0, x1, x2, x3);
+      y0 = (((y0 - c.a.w * x3) - c.a.z * x2) - c.a.y * x1) - c.a.x * x0;
+      y1 = upd(y1, c.b, lane + 64 - k, x0, x1, x2, x3);
+    };
+    if (ng > 16) {
+      load(ng - 1, A, true);
+      for (int g = ng - 1; g >= 16; g -= 2) {
+        stepB(g, A, B);
+        if (g - 1 >= 16) stepB(g - 1, B, A);
       }
     }
-    for (; k >= 0; --k) {
-      const T xk = bcast2(y0, y1, k) / R[k + k * LD];
-      const T u0 = R[lane + k * LD], u1 = R[lane + 64 + k * LD];
-      if (k < 64) {
-        if (lane == k) y0 = xk;
-      } else if (lane == k - 64) {
-        y1 = xk;
-      }
-      if (lane < k) y0 -= u0 * xk;
-      if (lane + 64 < k) y1 -= u1 * xk;
+    // pivots in rows 0..63: only the first half updates
+    auto stepA = [&](int g, const BwdStep<T>& c, BwdStep<T>& n) {
+      const int k = 4 * g;
+      load(max(g - 1, 0), n, false);
+      T x0, x1, x2, x3;
+      solve4(c, rdlane(y0, k), rdlane(y0, k + 1), rdlane(y0, k + 2), rdlane(y0, k + 3), x0, x1,
+             x2, x3);
+      y0 = upd(y0, c.a, lane - k, x0, x1, x2, x3);
+    };
+    load(gA - 1, A, false);
+    for (int g = gA - 1; g >= 0; g -= 2) {
+      stepA(g, A, B);
+      if (g - 1 >= 0) stepA(g - 1, B, A);
     }
     if (in0) x[fc0 + c0 + lane] = y0;
     if (in1) x[fc0 + c0 + 64 + lane] = y1;
