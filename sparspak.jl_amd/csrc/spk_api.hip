@@ -606,7 +606,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         if (bi < 0) continue;
         const Blk& B = fblocks[fi][bi];
         if (B.c1 >= h->fL[large[fi]]) continue;
-        for (i32 r0 = 0; r0 < B.c1 - B.c0; r0 += 4)
+        for (i32 r0 = 0; r0 < B.c1 - B.c0; ++r0)
           sdesc.push_back(spk::SolveDesc{large[fi], B.p0, B.np, B.c0, B.c1, r0, 0});
       }
       S.gemv_cnt = (int)sdesc.size() - S.gemv_off;
@@ -899,7 +899,7 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
         if (S.gemv_cnt > 0) {
           Prof pr(h, SPKHIP_K_BWD_GEMV, 2.0 * S.elems * nrhs, S.elems * sizeof(T));
           hipLaunchKernelGGL(spk::bwd_bgemv_kernel<T>, dim3(S.gemv_cnt), dim3(kWG), 0, st, h->ds,
-                             h->d_sdesc.p + S.gemv_off, unz, rhs, ldr, nrhs, vec.p);
+                             h->d_sdesc.p + S.gemv_off, unz, nrhs, vec.p);
           hipck(hipGetLastError());
         }
         Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);

@@ -1649,12 +1649,16 @@ fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
 }
 
 // Forward: y[c1 + r0 + lane] -= L[., c0:c1] y[c0:c1]; wave w takes a
-// quarter of the block's columns, partial sums added in a fixed order.
+// quarter of the block's columns, partial sums added in a fixed order. The
+// block's column offsets and y[c0:c1] are staged in LDS first, then every
+// thread issues all of its L loads at once.
 template <typename T>
 __global__ void __launch_bounds__(kWG)
 fwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ lnz,
                  int nrhs, T* __restrict__ vec) {
   __shared__ T red[4][64];
+  __shared__ int64_t lc[128];
+  __shared__ T zs[128];
   const int tid = threadIdx.x, lane = tid % 64;
   const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
   const SolveDesc d = desc[blockIdx.x];
@@ -1663,11 +1667,21 @@ fwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   const int64_t fc0 = s.fc0[f];
   const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
   const int r = c1 + d.r0 + lane;
-  const int cb = c0 + bw * wave / 4, ce = c0 + bw * (wave + 1) / 4;
+  const int cb = bw * wave / 4, ce = bw * (wave + 1) / 4;  // at most 32 columns
+  if (tid < bw) lc[tid] = s.lcol[fc0 + c0 + tid];
   for (int q = 0; q < nrhs; ++q) {
     T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
+    if (tid < bw) zs[tid] = yg[c0 + tid];
+    __syncthreads();
     T acc = T(0);
-    if (r < L) acc = dot_lrow<T>(s, lnz, fc0 + cb, ce - cb, r, yg + cb);
+    if (r < L) {
+      T lv[32];
+#pragma unroll
+      for (int j = 0; j < 32; ++j) lv[j] = (cb + j < ce) ? lnz[lc[cb + j] + r] : T(0);
+#pragma unroll
+      for (int j = 0; j < 32; ++j)
+        if (cb + j < ce) acc += lv[j] * zs[cb + j];
+    }
     red[wave][lane] = acc;
     __syncthreads();
     if (wave == 0 && r < L) yg[r] -= ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
@@ -1702,38 +1716,43 @@ bwd_gather_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restric
   }
 }
 
-// Backward: t[c] = U[c, c1:L] x[c1:L] for rows c = c0 + r0 + wave of the
-// block (one wave per row), stored in y[c]; x of the front's own columns
-// from rhs, of its update rows from the front's vector (bwd_gather_kernel).
+// Backward: t[c] = U[c, c1:L] x[c1:L] for row c = c0 + r0 of the block, one
+// workgroup per row, stored in y[c]. x[c1:L] is read from the front's vector:
+// its update rows from bwd_gather_kernel, its own columns as the backward
+// chains of the later blocks left them. Every thread takes columns c1 + tid
+// + 256 m, sixteen loads in flight; per-thread sums are added over the wave,
+// then the four waves in a fixed order.
 template <typename T>
 __global__ void __launch_bounds__(kWG)
 bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ unz,
-                 const T* __restrict__ rhs, int64_t ldr, int nrhs, T* __restrict__ vec) {
+                 int nrhs, T* __restrict__ vec) {
+  __shared__ T red[4];
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
-  const int c = d.c0 + d.r0 + wave;
-  if (c >= d.c1) return;
-  const int L = s.fL[f], W = s.fW[f];
+  const int c = d.c0 + d.r0;
+  const int L = s.fL[f];
   const int64_t fc0 = s.fc0[f];
   const T* u = unz + s.ucol[fc0 + c];
   for (int q = 0; q < nrhs; ++q) {
-    const T* xo = rhs + (int64_t)q * ldr + fc0;  // own columns, front-local
     T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
     T acc = T(0);
-    for (int j0 = d.c1 + lane; j0 < L; j0 += 8 * 64) {
-      T uv[8], xv[8];
+    for (int j0 = d.c1 + tid; j0 < L; j0 += 16 * kWG) {
+      T uv[16], xv[16];
 #pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        const int j = j0 + 64 * t;
+      for (int t = 0; t < 16; ++t) {
+        const int j = j0 + kWG * t;
         uv[t] = (j < L) ? u[j] : T(0);
-        xv[t] = (j < L) ? (j < W ? xo[j] : yg[j]) : T(0);
+        xv[t] = (j < L) ? yg[j] : T(0);
       }
 #pragma unroll
-      for (int t = 0; t < 8; ++t) acc += uv[t] * xv[t];
+      for (int t = 0; t < 16; ++t) acc += uv[t] * xv[t];
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane == 0) yg[c] = acc;
+    if (lane == 0) red[wave] = acc;
+    __syncthreads();
+    if (tid == 0) yg[c] = ((red[0] + red[1]) + red[2]) + red[3];
+    __syncthreads();
   }
 }
 
@@ -1877,8 +1896,14 @@ bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
       stepA(g, A, B);
       if (g - 1 >= 0) stepA(g - 1, B, A);
     }
-    if (in0) x[fc0 + c0 + lane] = y0;
-    if (in1) x[fc0 + c0 + 64 + lane] = y1;
+    if (in0) {
+      x[fc0 + c0 + lane] = y0;
+      yg[c0 + lane] = y0;
+    }
+    if (in1) {
+      x[fc0 + c0 + 64 + lane] = y1;
+      yg[c0 + 64 + lane] = y1;
+    }
   }
 }
 
