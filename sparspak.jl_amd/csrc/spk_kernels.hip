@@ -3,6 +3,10 @@
 
 namespace spk {
 
+// Index of the calling thread's wave in its workgroup, as a wave-uniform
+// (scalar) value so that per-wave index arithmetic stays on the scalar unit.
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / 64); }
+
 __device__ inline int64_t ld_i64(const int64_t* p, int64_t i) { return p[i]; }
 
 // Column range of piece p (0-based supernode): first column (0-based) and
@@ -624,38 +628,63 @@ __device__ void pos_getrf(T* D, int nq, int ncols, int* piv, int* perm, T* dinv,
   }
 }
 
-// pos_getrf with the whole workgroup: every wave runs the same pivot search
-// (identical inputs, identical result) and updates the columns j = k + 1 +
-// wave + 4 i of step k, one barrier per step. The multipliers of column k
-// are written by wave 0 one step later, when no wave reads that column any
-// more. Must be called by all threads of the workgroup.
+// Pivot of column k among the rows whose position is k or later (v: this
+// lane's entry of the column, pos: its row's position): largest |v|, ties
+// to the smallest position; a column without a comparable candidate (NaN)
+// keeps the row at position k. Returns the lane of the pivot row.
+template <typename T>
+__device__ inline int pivot_lane(T v, int pos, bool mine, int k) {
+  const bool cand = mine && pos >= k;
+  const T a = cand ? fabs(v) : T(-1);
+  const T m = wave_max(a);
+  const bool tie = cand && a == m;
+  const unsigned long long msk = __ballot(tie);
+  int blane;
+  if (msk == 0ull) {
+    blane = __ffsll((long long)__ballot(mine && pos == k)) - 1;
+  } else if (__popcll(msk) == 1) {
+    blane = __ffsll((long long)msk) - 1;
+  } else {
+    const int mp = wave_min(tie ? pos : (1 << 20));
+    blane = __ffsll((long long)__ballot(tie && pos == mp)) - 1;
+  }
+  return __builtin_amdgcn_readfirstlane(blane);
+}
+
+// pos_getrf with the whole workgroup, one barrier per step. Wave 0 updates
+// the next pivot column first and runs the next step's pivot search on it
+// (pivot_lane), publishing the pivot row and value through LDS (two slots,
+// by step parity); waves 1-3 update the columns j = k + 2 + (wave - 1) + 3 i
+// meanwhile. The multipliers of column k are written by wave 0 one step
+// later, when no wave reads that column any more. Must be called by all
+// threads of the workgroup.
 template <typename T, int LD>
 __device__ void wg_getrf(T* D, int nq, int ncols, int* piv, int* perm, T* dinv, int* zero) {
-  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  __shared__ int s_blane[2];
+  __shared__ T s_pv[2];
+  const int lane = threadIdx.x % 64, wave = wave_id();
   const bool mine = lane < nq;
   int pos = lane;
   T lprev = T(0);
   bool wprev = false;
+  if (wave == 0) {
+    const T v = mine ? D[lane] : T(0);
+    const int b = pivot_lane(v, pos, mine, 0);
+    const T pv = rdlane(v, b);
+    if (lane == 0) {
+      s_blane[0] = b;
+      s_pv[0] = pv;
+    }
+  }
+  __syncthreads();
   for (int k = 0; k < nq; ++k) {
-    const bool cand = mine && pos >= k;
+    const int par = k & 1;
+    const int blane = s_blane[par];
+    const T pv = s_pv[par];
     const T v = mine ? D[lane + k * LD] : T(0);
     if (wave == 0 && wprev) D[lane + (k - 1) * LD] = lprev;
-    const T a = cand ? fabs(v) : T(-1);
-    const T m = wave_max(a);
-    const bool tie = cand && a == m;
-    const unsigned long long msk = __ballot(tie);
-    int blane;
-    if (msk == 0ull) {  // no comparable candidate (NaN column): keep row k
-      blane = __ffsll((long long)__ballot(mine && pos == k)) - 1;
-    } else if (__popcll(msk) == 1) {
-      blane = __ffsll((long long)msk) - 1;
-    } else {
-      const int mp = wave_min(tie ? pos : (1 << 20));
-      blane = __ffsll((long long)__ballot(tie && pos == mp)) - 1;
-    }
-    blane = __builtin_amdgcn_readfirstlane(blane);
+    // the pivot row's position before this step's interchange
     const int kp = __builtin_amdgcn_readlane(pos, blane);
-    const T pv = rdlane(v, blane);
     const bool nonzero = pv != T(0);
     if (nonzero && kp != k) {
       if (mine && pos == k)
@@ -673,20 +702,39 @@ __device__ void wg_getrf(T* D, int nq, int ncols, int* piv, int* perm, T* dinv, 
     const T l = nonzero ? v * inv : v;
     lprev = l;
     wprev = below && nonzero;
-    // columns j = k + 1 + wave + 4 i in batches of eight: loads of a batch
-    // are issued together (indices clamped inside the block), stores guarded
-    for (int jb = k + 1 + wave; jb < nq; jb += 32) {
-      T u[8], x[8];
-#pragma unroll
-      for (int jj = 0; jj < 8; ++jj) {
-        const int j = min(jb + 4 * jj, nq - 1);
-        u[jj] = D[blane + j * LD];
-        x[jj] = D[lane + j * LD];
+    if (wave == 0) {
+      if (k + 1 < nq) {
+        const int j = k + 1;
+        const T u = D[blane + j * LD];
+        T x = D[lane + j * LD];
+        if (below) {
+          x -= l * u;
+          D[lane + j * LD] = x;
+        }
+        const int b = pivot_lane(x, pos, mine, j);
+        const T pn = rdlane(x, b);
+        if (lane == 0) {
+          s_blane[par ^ 1] = b;
+          s_pv[par ^ 1] = pn;
+        }
       }
-      if (below) {
+    } else {
+      // columns j = k + 2 + (wave - 1) + 3 i in batches of eight: loads of a
+      // batch are issued together (indices clamped inside the block), stores
+      // guarded
+      for (int jb = k + 1 + wave; jb < nq; jb += 24) {
+        T u[8], x[8];
 #pragma unroll
-        for (int jj = 0; jj < 8; ++jj)
-          if (jb + 4 * jj < nq) D[lane + (jb + 4 * jj) * LD] = x[jj] - l * u[jj];
+        for (int jj = 0; jj < 8; ++jj) {
+          const int j = min(jb + 3 * jj, nq - 1);
+          u[jj] = D[blane + j * LD];
+          x[jj] = D[lane + j * LD];
+        }
+        if (below) {
+#pragma unroll
+          for (int jj = 0; jj < 8; ++jj)
+            if (jb + 3 * jj < nq) D[lane + (jb + 3 * jj) * LD] = x[jj] - l * u[jj];
+        }
       }
     }
     __syncthreads();
@@ -755,6 +803,28 @@ __device__ void nopiv_getrf(T* D, int nq, int* piv, T* dinv, int* zero) {
   }
 }
 
+// v_mfma_*_16x16x4: both forms take A(i, k) and B(k, j) from lane i + 16 k
+// (j + 16 k); accumulator element v of lane l is D(i, l % 16) with i = 4
+// (l / 16) + v for the f32 form and i = 4 v + l / 16 for the f64 form.
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lgrp, int v) { return 4 * v + lgrp; }
+};
+template <>
+struct Mfma16<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ inline v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int lgrp, int v) { return 4 * lgrp + v; }
+};
+
 // Diagonal region of one block per workgroup (see the section comment);
 // NQ bounds the piece width. Dynamic LDS: 128 * kRLD elements.
 template <typename T, int NQ>
@@ -768,7 +838,7 @@ blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ aren
   __shared__ T dinv[NQ];
   __shared__ int s_zero;
   T* R = reinterpret_cast<T*>(smem_raw);
-  const int tid = threadIdx.x, wave = tid / 64;
+  const int tid = threadIdx.x, wave = wave_id();
   const BlkDesc d = desc[blockIdx.x];
   const int f = d.front;
   const int L = s.fL[f];
@@ -797,15 +867,35 @@ blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ aren
     if (tid == 0 && s_zero) atomicMin(first_fail, (unsigned long long)(d.piece + pi + 1));
     if (tid >= 128) {
       // U12 of the region (rows already interchanged): unit-lower solve,
-      // u_i = a_i - sum_k<i L(i, k) u_k
+      // u_i = a_i - sum_k<i L(i, k) u_k; for pieces of up to 48 columns the
+      // thread's column is held in registers (loops unrolled to NQ, cut at
+      // nq by uniform branches)
       for (int c = tid - 128; c < rest; c += 128) {
-        T* u = D + (nq + c) * LD;
-        for (int i = 1; i < nq; ++i) {
-          T v = u[i];
-          const T* li = D + i;
+        T* uc = D + (nq + c) * LD;
+        if constexpr (NQ <= 48) {
+          T u[NQ];
+#pragma unroll
+          for (int i = 0; i < NQ; ++i) u[i] = (i < nq) ? uc[i] : T(0);
+#pragma unroll
+          for (int i = 1; i < NQ; ++i) {
+            if (i < nq) {
+              T v = u[i];
+#pragma unroll
+              for (int kk = 0; kk < i; ++kk) v -= D[i + kk * LD] * u[kk];
+              u[i] = v;
+            }
+          }
+#pragma unroll
+          for (int i = 1; i < NQ; ++i)
+            if (i < nq) uc[i] = u[i];
+        } else {
+          for (int i = 1; i < nq; ++i) {
+            T v = uc[i];
+            const T* li = D + i;
 #pragma unroll 8
-          for (int kk = 0; kk < i; ++kk) v -= li[kk * LD] * u[kk];
-          u[i] = v;
+            for (int kk = 0; kk < i; ++kk) v -= li[kk * LD] * uc[kk];
+            uc[i] = v;
+          }
         }
       }
     } else {
@@ -816,55 +906,47 @@ blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ aren
           T v = x[j * LD];
           const T* uj = D + j * LD;
 #pragma unroll 8
-          for (int k = 0; k < j; ++k) v -= x[k * LD] * uj[k];
+          for (int kk = 0; kk < j; ++kk) v -= x[kk * LD] * uj[kk];
           x[j * LD] = v * dinv[j];
         }
       }
     }
     __syncthreads();
     if (rest > 0) {
-      // rest x rest update, 4 x 4 entries per thread and pass
-      const int nb = (rest + 3) / 4;
-      const T* Lb = D + nq;       // L21 rows
-      const T* Ub = D + nq * LD;  // U12 columns
+      // rest x rest update on the matrix cores: 16 x 16 tiles, every fourth
+      // to a wave; per tile all operand reads are issued before the MFMAs
+      // (k in steps of four, cut at nq); products summed from zero and
+      // subtracted once, as in blk_gemm_kernel
+      typedef typename Mfma16<T>::v4 v4;
+      constexpr int KS = (NQ + 3) / 4;
+      const int lane = tid % 64, l16 = lane % 16, lk = lane / 16;
+      const int nt = (rest + 15) / 16;
+      const T* Lb = D + nq;       // L21(i, k) = Lb[i + k LD]
+      const T* Ub = D + nq * LD;  // U12(k, j) = Ub[k + j LD]
       T* C = D + nq + nq * LD;
-      for (int idx = tid; idx < nb * nb; idx += kWG) {
-        // tiles at the edge shift back to stay inside (rest >= 4) and then
-        // write only their own entries; smaller regions clamp their reads
-        const int ib = (idx % nb) * 4, jb = (idx / nb) * 4;
-        const int i0 = rest >= 4 ? min(ib, rest - 4) : 0;
-        const int j0 = rest >= 4 ? min(jb, rest - 4) : 0;
-        const T* lp[4];
-        const T* up[4];
+      for (int t = wave; t < nt * nt; t += 4) {
+        const int tr = (t % nt) * 16, tc = (t / nt) * 16;
+        const int ri = tr + l16, cj = tc + l16;
+        const bool rin = ri < rest, cin = cj < rest;
+        T av[KS], bv[KS];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          lp[a] = Lb + min(i0 + a, rest - 1);
-          up[a] = Ub + min(j0 + a, rest - 1) * LD;
+        for (int ks = 0; ks < KS; ++ks) {
+          const int kk = 4 * ks + lk;
+          const bool kin = kk < nq;
+          av[ks] = (kin && cin) ? Ub[kk + cj * LD] : T(0);
+          bv[ks] = (kin && rin) ? Lb[ri + kk * LD] : T(0);
         }
-        T acc[4][4];
+        v4 acc = v4{T(0), T(0), T(0), T(0)};
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int ks = 0; ks < KS; ++ks)
+          if (4 * ks < nq) acc = Mfma16<T>::op(av[ks], bv[ks], acc);
+        if (rin) {
 #pragma unroll
-          for (int c = 0; c < 4; ++c) acc[a][c] = T(0);
-        for (int k = 0; k < nq; ++k) {
-          T lv[4], uv[4];
-#pragma unroll
-          for (int a = 0; a < 4; ++a) {
-            lv[a] = lp[a][k * LD];
-            uv[a] = up[a][k];
+          for (int v = 0; v < 4; ++v) {
+            const int j = tc + Mfma16<T>::row(lk, v);
+            if (j < rest) C[ri + j * LD] -= acc[v];
           }
-#pragma unroll
-          for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) acc[a][c] += lv[a] * uv[c];
         }
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-#pragma unroll
-          for (int a = 0; a < 4; ++a) {
-            const int i = i0 + a, j = j0 + c;
-            if (i >= ib && j >= jb && i < rest && j < rest) C[i + j * LD] -= acc[a][c];
-          }
       }
     }
     __syncthreads();
@@ -883,7 +965,7 @@ blk_offdiag_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
   extern __shared__ unsigned char smem_raw[];
   __shared__ int spiv[NQ];
   T* S = reinterpret_cast<T*>(smem_raw);
-  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
   const int di = find_blk(desc, ndesc, blockIdx.x);
   const BlkDesc d = desc[di];
   const int b = blockIdx.x - d.wg_begin;
@@ -1008,27 +1090,6 @@ blk_offdiag_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
 // operands are passed as (U^T, L^T) so that the accumulator holds the tile
 // transposed: lane l keeps row l % 16 of four columns, and each group of 16
 // lanes writes 16 contiguous rows of the column-major front.
-// Both instructions take A(i, k) and B(k, j) from lane i + 16 k (j + 16 k);
-// accumulator element v of lane l is D(i, l % 16) with i = 4 (l / 16) + v
-// for the f32 form and i = 4 v + l / 16 for the f64 form.
-template <typename T>
-struct Mfma16;
-template <>
-struct Mfma16<double> {
-  typedef double v4 __attribute__((ext_vector_type(4)));
-  static __device__ inline v4 op(double a, double b, v4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ inline int row(int lgrp, int v) { return 4 * v + lgrp; }
-};
-template <>
-struct Mfma16<float> {
-  typedef float v4 __attribute__((ext_vector_type(4)));
-  static __device__ inline v4 op(float a, float b, v4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ inline int row(int lgrp, int v) { return 4 * lgrp + v; }
-};
 
 // float -> bf16, round to nearest even (finite inputs).
 __device__ inline unsigned short f2bf(float x) {
@@ -1052,7 +1113,7 @@ blk_gemm_kernel(DevSym s, const GemmDesc* __restrict__ desc, int ndesc,
   //                  Bs[buf][j * KP + k] = U(k0 + k, j0 + j)
   __shared__ T As[2][TM * KP];
   __shared__ T Bs[2][TM * KP];
-  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
   const int di = find_blk(desc, ndesc, blockIdx.x);
   const GemmDesc d = desc[di];
   const int b = blockIdx.x - d.wg_begin;
@@ -1280,7 +1341,7 @@ fwd_small_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict
   T* yW = reinterpret_cast<T*>(smem_raw);
   __shared__ T D[MAXW * MAXW];
   __shared__ int spiv[MAXW];
-  const int tid = threadIdx.x, wave = tid / 64;
+  const int tid = threadIdx.x, wave = wave_id();
   const int f = list[blockIdx.x];
   const int L = s.fL[f], W = s.fW[f];
   const int64_t fc0 = s.fc0[f];
@@ -1321,7 +1382,7 @@ bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict
   extern __shared__ unsigned char smem_raw[];
   T* yW = reinterpret_cast<T*>(smem_raw);
   __shared__ T D[MAXW * MAXW];
-  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
   const int f = list[blockIdx.x];
   const int L = s.fL[f], W = s.fW[f];
   const int64_t fc0 = s.fc0[f];
@@ -1417,7 +1478,7 @@ solve_prep_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restr
   __shared__ int pstart[128];
   __shared__ int pbeg[9];
   __shared__ int64_t lb[128], ub[128];
-  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
   const SolveDesc d = desc[blockIdx.x];
   const int64_t fc0 = s.fc0[d.front];
   const int c0 = d.c0, bw = d.c1 - d.c0;
@@ -1551,7 +1612,7 @@ fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   extern __shared__ __attribute__((aligned(32))) unsigned char smem_a32[];
   V* smem_v4 = reinterpret_cast<V*>(smem_a32);
   const T* R = reinterpret_cast<const T*>(smem_a32);
-  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
   const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
   const int L = s.fL[f];
@@ -1727,7 +1788,7 @@ __global__ void __launch_bounds__(kWG)
 bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ unz,
                  int nrhs, T* __restrict__ vec) {
   __shared__ T red[4];
-  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
   const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
   const int c = d.c0 + d.r0;
@@ -1793,7 +1854,7 @@ bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   extern __shared__ __attribute__((aligned(32))) unsigned char smem_a32[];
   V* smem_v4 = reinterpret_cast<V*>(smem_a32);
   const T* R = reinterpret_cast<const T*>(smem_a32);
-  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
   const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
   const int L = s.fL[f];
