@@ -570,11 +570,16 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
     }
     // blocked solve steps over the same blocks; each block's solve data
     // (index r0, offset 4 aux) is built once per factorisation
+    // (the front's sizes and offsets ride in every descriptor, so that the
+    // solve kernels need no dependent load of them)
+    auto sd = [&](i32 f, const Blk& B, i32 r0, i32 aux) {
+      return spk::SolveDesc{f, B.p0, B.np, B.c0, B.c1, r0, aux, h->fL[f], h->fW[f],
+                            (int64_t)h->fc0[f], (int64_t)h->fvec[f], 0};
+    };
     std::vector<std::vector<spk::SolveDesc>> bdata(large.size());
     for (size_t fi = 0; fi < large.size(); ++fi)
       for (const Blk& B : fblocks[fi]) {
-        const spk::SolveDesc D{large[fi], B.p0, B.np, B.c0, B.c1, (i32)h->pdesc.size(),
-                               (i32)h->prep_units};
+        const spk::SolveDesc D = sd(large[fi], B, (i32)h->pdesc.size(), (i32)h->prep_units);
         h->pdesc.push_back(D);
         bdata[fi].push_back(D);
         const i64 ng = (B.c1 - B.c0 + 3) / 4;
@@ -592,7 +597,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
           const Blk& B = fblocks[fi][gi];
           const i32 t = h->fL[large[fi]] - B.c1;
           for (i32 r0 = 0; r0 < t; r0 += 64)
-            sdesc.push_back(spk::SolveDesc{large[fi], B.p0, B.np, B.c0, B.c1, r0, 0});
+            sdesc.push_back(sd(large[fi], B, r0, 0));
           S.elems += (double)(B.c1 - B.c0) * (h->fL[large[fi]] - B.c0);
         }
       S.gemv_cnt = (int)sdesc.size() - S.gemv_off;
@@ -607,7 +612,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         const Blk& B = fblocks[fi][bi];
         if (B.c1 >= h->fL[large[fi]]) continue;
         for (i32 r0 = 0; r0 < B.c1 - B.c0; ++r0)
-          sdesc.push_back(spk::SolveDesc{large[fi], B.p0, B.np, B.c0, B.c1, r0, 0});
+          sdesc.push_back(sd(large[fi], B, r0, 0));
       }
       S.gemv_cnt = (int)sdesc.size() - S.gemv_off;
       S.chain_off = (int)sdesc.size();

@@ -97,6 +97,10 @@ struct SolveDesc {
                    // chain and prep: the block's index in the solve data
   int32_t aux;     // chain and prep: the block's solve data starts at
                    // element 4 * aux
+  int32_t L, W;    // the front's index-list length and own columns
+  int64_t fc0;     // the front's first column
+  int64_t fvec;    // the front's offset in the solve vector arena
+  int64_t pad;
 };
 
 }  // namespace spk

@@ -1615,8 +1615,8 @@ fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
   const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
-  const int L = s.fL[f];
-  const int64_t fc0 = s.fc0[f];
+  const int L = d.L;
+  const int64_t fc0 = d.fc0;
   const int c0 = d.c0, bw = d.c1 - d.c0;
   const int ng = (bw + 3) / 4, bwp = 4 * ng;
   const bool in0 = lane < bw, in1 = lane + 64 < bw;
@@ -1624,7 +1624,7 @@ fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   // wave 0: permutation and first right-hand side, in flight with the panels
   int s0 = lane, s1 = lane + 64;
   T y0r = T(0), y1r = T(0);
-  const T* yv = vec + s.fvec[f] * nrhs;
+  const T* yv = vec + d.fvec * nrhs;
   if (wave == 0) {
     if (in0) {
       s0 = pi[lane];
@@ -1653,7 +1653,7 @@ fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
     o.b = *(const V*)(P + 4 * (min(max(lane + 64, k), bwp - 1) - k));
   };
   for (int q = 0; q < nrhs; ++q) {
-    T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
+    T* yg = vec + d.fvec * nrhs + (int64_t)q * L;
     T* x = rhs + (int64_t)q * ldr;
     if (q > 0) {
       y0r = in0 ? yg[c0 + lane] : T(0);
@@ -1724,14 +1724,14 @@ fwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
   const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
-  const int L = s.fL[f];
-  const int64_t fc0 = s.fc0[f];
+  const int L = d.L;
+  const int64_t fc0 = d.fc0;
   const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
   const int r = c1 + d.r0 + lane;
   const int cb = bw * wave / 4, ce = bw * (wave + 1) / 4;  // at most 32 columns
   if (tid < bw) lc[tid] = s.lcol[fc0 + c0 + tid];
   for (int q = 0; q < nrhs; ++q) {
-    T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
+    T* yg = vec + d.fvec * nrhs + (int64_t)q * L;
     if (tid < bw) zs[tid] = yg[c0 + tid];
     __syncthreads();
     T acc = T(0);
@@ -1792,11 +1792,11 @@ bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
   const int c = d.c0 + d.r0;
-  const int L = s.fL[f];
-  const int64_t fc0 = s.fc0[f];
+  const int L = d.L;
+  const int64_t fc0 = d.fc0;
   const T* u = unz + s.ucol[fc0 + c];
   for (int q = 0; q < nrhs; ++q) {
-    T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
+    T* yg = vec + d.fvec * nrhs + (int64_t)q * L;
     T acc = T(0);
     for (int j0 = d.c1 + tid; j0 < L; j0 += 16 * kWG) {
       T uv[16], xv[16];
@@ -1857,15 +1857,15 @@ bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
   const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
-  const int L = s.fL[f];
-  const int64_t fc0 = s.fc0[f];
+  const int L = d.L;
+  const int64_t fc0 = d.fc0;
   const int c0 = d.c0, c1 = d.c1, bw = c1 - c0;
   const int ng = (bw + 3) / 4;
   const bool in0 = lane < bw, in1 = lane + 64 < bw, sub = c1 < L;
   T x0r = T(0), x1r = T(0), t0r = T(0), t1r = T(0);
   if (wave == 0) {
     const T* x = rhs + fc0 + c0;
-    const T* yg = vec + s.fvec[f] * nrhs + c0;
+    const T* yg = vec + d.fvec * nrhs + c0;
     if (in0) {
       x0r = x[lane];
       if (sub) t0r = yg[lane];
@@ -1896,7 +1896,7 @@ bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
     if (both) o.b = *(const V*)(P + 4 * min(lane + 64, k + 3));
   };
   for (int q = 0; q < nrhs; ++q) {
-    T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
+    T* yg = vec + d.fvec * nrhs + (int64_t)q * L;
     T* x = rhs + (int64_t)q * ldr;
     if (q > 0) {
       x0r = in0 ? x[fc0 + c0 + lane] : T(0);
