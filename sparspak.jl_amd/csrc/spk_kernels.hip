@@ -1221,16 +1221,27 @@ blk_gemm_kernel(DevSym s, const GemmDesc* __restrict__ desc, int ndesc,
     __syncthreads();
   }
   }
+  // A22 -= the products, sixteen entries at a time: all reads of a group
+  // are issued before its writes (one memory round trip per group instead
+  // of one per entry)
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     const int r = r0 + wr + a * 16 + l16;
-    if (r >= rend) continue;
+    const bool rok = r < rend;
+    T cv[4][4];
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int col = j0 + wc + c * 16 + Mfma16<T>::row(lk, v);
-        if (col < cend) F[r + (int64_t)col * L] -= acc[a][c][v];
+        cv[c][v] = (rok && col < cend) ? F[r + (int64_t)col * L] : T(0);
+      }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int col = j0 + wc + c * 16 + Mfma16<T>::row(lk, v);
+        if (rok && col < cend) F[r + (int64_t)col * L] = cv[c][v] - acc[a][c][v];
       }
   }
 }
