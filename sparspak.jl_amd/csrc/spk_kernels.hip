@@ -1221,28 +1221,37 @@ blk_gemm_kernel(DevSym s, const GemmDesc* __restrict__ desc, int ndesc,
     __syncthreads();
   }
   }
-  // A22 -= the products, sixteen entries at a time: all reads of a group
+  // A22 -= the products, AG x 16 entries at a time: all reads of a group
   // are issued before its writes (one memory round trip per group instead
-  // of one per entry)
+  // of one per entry; all 64 floats at once, doubles in two groups of 32 to
+  // stay within the register budget of two workgroups per CU)
+  constexpr int AG = std::is_same<T, float>::value ? 4 : 2;
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int r = r0 + wr + a * 16 + l16;
-    const bool rok = r < rend;
-    T cv[4][4];
+  for (int a0 = 0; a0 < 4; a0 += AG) {
+    T cv[AG][4][4];
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+    for (int ag = 0; ag < AG; ++ag) {
+      const int r = r0 + wr + (a0 + ag) * 16 + l16;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int col = j0 + wc + c * 16 + Mfma16<T>::row(lk, v);
-        cv[c][v] = (rok && col < cend) ? F[r + (int64_t)col * L] : T(0);
-      }
+      for (int c = 0; c < 4; ++c)
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+        for (int v = 0; v < 4; ++v) {
+          const int col = j0 + wc + c * 16 + Mfma16<T>::row(lk, v);
+          cv[ag][c][v] = (r < rend && col < cend) ? F[r + (int64_t)col * L] : T(0);
+        }
+    }
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int col = j0 + wc + c * 16 + Mfma16<T>::row(lk, v);
-        if (rok && col < cend) F[r + (int64_t)col * L] = cv[c][v] - acc[a][c][v];
-      }
+    for (int ag = 0; ag < AG; ++ag) {
+      const int r = r0 + wr + (a0 + ag) * 16 + l16;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int col = j0 + wc + c * 16 + Mfma16<T>::row(lk, v);
+          if (r < rend && col < cend)
+            F[r + (int64_t)col * L] = cv[ag][c][v] - acc[a0 + ag][c][v];
+        }
+    }
   }
 }
 
