@@ -1014,11 +1014,30 @@ blk_offdiag_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
       if (wave == 0) {  // X_q U_qq^-1, one row per lane
         T* xr = X + lane + o * 64;
         const T* uq = Us + o * NQ;
-        for (int j = 0; j < nq; ++j) {
-          T v = xr[j * 64];
+        if constexpr (NQ <= 48) {
+          // the row in registers (loops unrolled to NQ, cut at nq)
+          T x[NQ];
+#pragma unroll
+          for (int j = 0; j < NQ; ++j) x[j] = (j < nq) ? xr[j * 64] : T(0);
+#pragma unroll
+          for (int j = 0; j < NQ; ++j) {
+            if (j < nq) {
+              T v = x[j];
+#pragma unroll
+              for (int k = 0; k < j; ++k) v -= x[k] * uq[k * NQ + j];
+              x[j] = v * (T(1) / uq[j * NQ + j]);
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < NQ; ++j)
+            if (j < nq) xr[j * 64] = x[j];
+        } else {
+          for (int j = 0; j < nq; ++j) {
+            T v = xr[j * 64];
 #pragma unroll 8
-          for (int k = 0; k < j; ++k) v -= xr[k * 64] * uq[k * NQ + j];
-          xr[j * 64] = v * (T(1) / uq[j * NQ + j]);
+            for (int k = 0; k < j; ++k) v -= xr[k * 64] * uq[k * NQ + j];
+            xr[j * 64] = v * (T(1) / uq[j * NQ + j]);
+          }
         }
       }
       __syncthreads();
@@ -1072,11 +1091,30 @@ blk_offdiag_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
           }
         }
         const T* lq = Ls + o * NQ;
-        for (int i = 1; i < nq; ++i) {
-          T v = a[i];
+        if constexpr (NQ <= 48) {
+          // the column in registers (loops unrolled to NQ, cut at nq)
+          T u[NQ];
+#pragma unroll
+          for (int i = 0; i < NQ; ++i) u[i] = (i < nq) ? a[i] : T(0);
+#pragma unroll
+          for (int i = 1; i < NQ; ++i) {
+            if (i < nq) {
+              T v = u[i];
+#pragma unroll
+              for (int kk = 0; kk < i; ++kk) v -= lq[i + kk * NQ] * u[kk];
+              u[i] = v;
+            }
+          }
+#pragma unroll
+          for (int i = 1; i < NQ; ++i)
+            if (i < nq) a[i] = u[i];
+        } else {
+          for (int i = 1; i < nq; ++i) {
+            T v = a[i];
 #pragma unroll 8
-          for (int kk = 0; kk < i; ++kk) v -= lq[i + kk * NQ] * a[kk];
-          a[i] = v;
+            for (int kk = 0; kk < i; ++kk) v -= lq[i + kk * NQ] * a[kk];
+            a[i] = v;
+          }
         }
       }
       __syncthreads();
