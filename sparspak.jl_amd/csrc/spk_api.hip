@@ -79,6 +79,7 @@ struct DevBuf {
 struct LevelPlan {
   int small_off = 0, small_cnt = 0, small_maxL = 0;
   int items_off = 0, items_cnt = 0;
+  int large_maxL = 0;               // longest index list of the level's large fronts
   struct Launch {
     int off = 0, cnt = 0, wg = 0;
     int lds = 0;                  // dynamic LDS elements per workgroup
@@ -446,6 +447,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
     lp.items_off = (int)it_front.size();
     for (i32 f : large) {
       const i32 L = h->fL[f];
+      lp.large_maxL = std::max(lp.large_maxL, L);
       // extend-add entries per destination column
       colcnt.assign(L + 1, 0);
       for (i32 ci = h->fchp[f]; ci < h->fchp[f + 1]; ++ci) {
@@ -758,10 +760,13 @@ void run_factor(spkhip_handle* h) {
         hipLaunchKernelGGL((spk::large_utile_kernel<T, false>), dim3(lp.ut_cnt), dim3(kWG), 0, st,
                            h->ds, h->d_ut_front.p + lp.ut_off, h->d_ut_piece.p + lp.ut_off,
                            h->d_ut_r0.p + lp.ut_off, lnz.p, unz.p, arena.p);
-      hipLaunchKernelGGL(spk::large_init_kernel<T>, dim3(lp.items_cnt), dim3(kWG), 0, st, h->ds,
-                         h->d_it_front.p + lp.items_off, h->d_it_col.p + lp.items_off,
-                         h->d_it_ea.p + lp.items_off, h->d_ea_child.p, h->d_ea_t.p, lnz.p,
-                         arena.p);
+      // the column is assembled in LDS when the longest one fits in 64 KB
+      const size_t clds = (size_t)lp.large_maxL * sizeof(T);
+      const bool staged = clds <= 64 * 1024;
+      hipLaunchKernelGGL(spk::large_init_kernel<T>, dim3(lp.items_cnt), dim3(kWG),
+                         staged ? clds : 0, st, h->ds, h->d_it_front.p + lp.items_off,
+                         h->d_it_col.p + lp.items_off, h->d_it_ea.p + lp.items_off,
+                         h->d_ea_child.p, h->d_ea_t.p, lnz.p, arena.p, (int)staged);
       hipck(hipGetLastError());
       }
       for (const LevelPlan::Group& G : lp.groups) {

@@ -319,7 +319,10 @@ large_utile_kernel(DevSym s, const int32_t* __restrict__ it_front,
 // from its piece's first row), zeros in the update block, then the
 // children's update-matrix columns that map to j, one child after the other
 // (rows of different children may coincide). Runs after large_utile_kernel
-// has placed the U part.
+// has placed the U part. With staged set the column is assembled in LDS
+// (dynamic, L elements): read once (the U part from the front, the rest from
+// lnz or zero), the children's columns added there, written once; otherwise
+// the children's entries are added in the front in place.
 template <typename T>
 __global__ void __launch_bounds__(kWG)
 large_init_kernel(DevSym s, const int32_t* __restrict__ it_front,
@@ -327,13 +330,64 @@ large_init_kernel(DevSym s, const int32_t* __restrict__ it_front,
                   const int64_t* __restrict__ it_ea,
                   const int32_t* __restrict__ ea_child,
                   const int32_t* __restrict__ ea_t, const T* __restrict__ lnz,
-                  T* __restrict__ arena) {
+                  T* __restrict__ arena, int staged) {
+  extern __shared__ unsigned char smem_raw[];
   const int item = blockIdx.x;
   const int f = it_front[item];
   const int j = it_col[item];
   const int L = s.fL[f], W = s.fW[f];
   const int64_t fc0 = s.fc0[f];
   T* F = arena + s.foff[f] + (int64_t)j * L;
+  const int64_t e0 = it_ea[item], e1 = it_ea[item + 1];
+  if (staged && !s.sym) {
+    T* Fs = reinterpret_cast<T*>(smem_raw);
+    // rows below r1 come from lnz (own column) or are zero (update column);
+    // rows above it are the U part already in the front
+    const int r1 = (j < W) ? piece_start_of_col(s, fc0, j) : W;
+    const T* lc = (j < W) ? lnz + s.lcol[fc0 + j] : nullptr;
+    for (int r0 = threadIdx.x; r0 < L; r0 += 8 * kWG) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int r = r0 + u * kWG;
+        v[u] = (r >= L) ? T(0) : (r < r1) ? F[r] : (lc ? lc[r] : T(0));
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (r0 + u * kWG < L) Fs[r0 + u * kWG] = v[u];
+    }
+    __syncthreads();
+    for (int64_t e = e0; e < e1; ++e) {
+      const int c = ea_child[e];
+      const int t = ea_t[e];
+      const int Rc = s.fL[c] - s.fW[c];
+      const int32_t* rel = s.relmap + s.frel[c];
+      const T* col = arena + s.fcb[c] + (int64_t)t * s.fcbld[c];
+      for (int tp0 = threadIdx.x; tp0 < Rc; tp0 += 8 * kWG) {
+        int ri[8];
+        T cv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int tp = tp0 + u * kWG;
+          ri[u] = (tp < Rc) ? rel[tp] : 0;
+          cv[u] = (tp < Rc) ? col[tp] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (tp0 + u * kWG < Rc) Fs[ri[u]] += cv[u];
+      }
+      __syncthreads();
+    }
+    for (int r0 = threadIdx.x; r0 < L; r0 += 8 * kWG) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = (r0 + u * kWG < L) ? Fs[r0 + u * kWG] : T(0);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (r0 + u * kWG < L) F[r0 + u * kWG] = v[u];
+    }
+    return;
+  }
   if (j < W) {
     const int cp = piece_start_of_col(s, fc0, j);
     const T* lc = lnz + s.lcol[fc0 + j];
@@ -356,7 +410,7 @@ large_init_kernel(DevSym s, const int32_t* __restrict__ it_front,
     for (int r = W + threadIdx.x; r < L; r += kWG) F[r] = T(0);
   }
   __syncthreads();
-  for (int64_t e = it_ea[item]; e < it_ea[item + 1]; ++e) {
+  for (int64_t e = e0; e < e1; ++e) {
     const int c = ea_child[e];
     const int t = ea_t[e];
     const int Rc = s.fL[c] - s.fW[c];
