@@ -32,10 +32,20 @@ for p in (ROOT, os.path.join(ROOT, "sparspak.jl_amd")):
 
 import numpy as np  # noqa: E402
 
-# MI355X dense matrix TFLOP/s (datasheet); the bf16 path's large-front
-# updates run on the bf16 matrix cores
-PEAK = {"fp64": 78.6, "fp32": 157.3, "fp32_bf16": 2516.6}
+# MI355X dense matrix TFLOP/s (datasheet) of the arithmetic each kernel class
+# runs in: everything in double on the fp64 path, in float on the fp32 path;
+# on the fp32_bf16 path the trailing updates of the large fronts run on the
+# bf16 matrix cores and the rest in float
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2516.6}
 HBM_PEAK = 8000.0                       # GB/s
+
+
+def kernel_peak(path, kclass):
+    if path == "fp64":
+        return PEAK["fp64"]
+    if path == "fp32_bf16" and kclass == "large_update":
+        return PEAK["bf16"]
+    return PEAK["fp32"]
 
 
 def log(msg):
@@ -172,8 +182,9 @@ def main():
     prec_name = head
     if compute_bound and dk["flops"] > 0:
         achieved = dk["flops"] / (dk["ms"] / 1e3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK[prec_name],
-                "unit": "TFLOP/s", "frac": achieved / PEAK[prec_name], "traffic": None,
+        peak = kernel_peak(head, dom)
+        roof = {"bound": "mfma", "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                 "kernel": dom, "launches": dk["launches"],
                 "avg_launch_ms": dk["ms"] / dk["launches"],
                 "work_per_launch": dk["flops"] / dk["launches"]}
@@ -220,22 +231,22 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-TRAFFIC_FILE = "large_update_traffic_convdiff3d64.md"
+TRAFFIC_FILE = "hbm_traffic_convdiff3d64.md"
 
 
 def traffic_from_profile(kernel_class, prec_name):
+    """Measured HBM bytes per launch of kernel_class on path prec_name from
+    the committed FETCH_SIZE / WRITE_SIZE summary (lines 'traffic <path>
+    <class>: <bytes>'), or None when it does not cover them."""
     path = os.path.join(ROOT, "profiles", TRAFFIC_FILE)
-    if prec_name != "fp64" or not os.path.exists(path):
+    if not os.path.exists(path):
         return None
-    vals = {}
+    key = f"traffic {prec_name} {kernel_class}"
     with open(path) as f:
         for line in f:
-            if ":" in line and not line.startswith(("|", " ", "#")):
-                key, _, val = line.partition(":")
-                vals[key.strip()] = val.strip()
-    if vals.get("kernel_class") != kernel_class or "traffic_bytes_per_launch" not in vals:
-        return None
-    return float(vals["traffic_bytes_per_launch"])
+            if line.startswith(key + ":"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(grid):
