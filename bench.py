@@ -9,6 +9,9 @@ one MI355X, in three precision paths:
              refined until the normwise backward error is <= 1e-14
   fp32_bf16  as fp32, with bf16 operands (float accumulation) in the
              trailing updates of fronts wider than 256 columns
+  fp32_fp16  as fp32_bf16 with fp16 operands under diagonal scaling (not in
+             the default set: at 64^3 it needs as many refinement steps as
+             fp32_bf16 and its scaled operand staging costs more)
 
 A step is: inmatrix (values already resident in HBM) -> factor -> solve of
 one right-hand side b = A x_true (x_true ~ U[-1, 1], splitmix64 seed 2) with
@@ -34,8 +37,8 @@ import numpy as np  # noqa: E402
 
 # MI355X dense matrix TFLOP/s (datasheet) of the arithmetic each kernel class
 # runs in: everything in double on the fp64 path, in float on the fp32 path;
-# on the fp32_bf16 path the trailing updates of the large fronts run on the
-# bf16 matrix cores and the rest in float
+# on the fp32_bf16 (fp32_fp16) path the trailing updates of the large fronts
+# run on the bf16 (fp16) matrix cores, same dense peak, and the rest in float
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2516.6}
 HBM_PEAK = 8000.0                       # GB/s
 
@@ -43,7 +46,7 @@ HBM_PEAK = 8000.0                       # GB/s
 def kernel_peak(path, kclass):
     if path == "fp64":
         return PEAK["fp64"]
-    if path == "fp32_bf16" and kclass == "large_update":
+    if path in ("fp32_bf16", "fp32_fp16") and kclass == "large_update":
         return PEAK["bf16"]
     return PEAK["fp32"]
 
@@ -112,7 +115,7 @@ def main():
     d_x = torch.empty_like(d_b)
     torch.cuda.synchronize()
 
-    prec = {"fp64": 0, "fp32": 1, "fp32_bf16": 2}
+    prec = {"fp64": 0, "fp32": 1, "fp32_bf16": 2, "fp32_fp16": 3}
 
     def step(path):
         _lib.check(L.spkhip_inmatrix_device(h, ctypes.c_void_p(d_nz.data_ptr())), "inmatrix")
@@ -217,7 +220,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32+f64ir", "fp32_bf16": "f32+bf16upd+f64ir"}[head],
+        "dtype": {"fp64": "f64", "fp32": "f32+f64ir", "fp32_bf16": "f32+bf16upd+f64ir",
+                  "fp32_fp16": "f32+fp16upd+f64ir"}[head],
         "data": "synthetic (generated matrix and rhs, splitmix64 seeds)",
         "config": {"workload": f"3D upwind convection-diffusion 7-point {g}^3 LU "
                                "(SparseSolver numeric phase: inmatrix+factor+solve)",

@@ -90,6 +90,12 @@ typedef struct spkhip_handle* spkhip_t;
 #define SPKHIP_FP32_BF16 2 /* float factor whose trailing updates in fronts of
                               more than 256 own columns take bf16 operands
                               (float accumulation); solves as SPKHIP_FP32 */
+#define SPKHIP_FP32_FP16 3 /* as SPKHIP_FP32_BF16 with fp16 operands under
+                              diagonal scaling: column k of L21 is multiplied
+                              and row k of U12 divided by the power of two
+                              nearest below |U(k,k)| (the product is
+                              unchanged), which keeps both in fp16 range for
+                              diagonally scaled problems */
 
 typedef struct spkhip_stats {
   int64_t n, nsuper, nfront, nlevel;
@@ -132,7 +138,7 @@ int64_t spkhip_inmatrix_device(spkhip_t h, const double* d_nzval);
 
 /* _factor! -> _lufactor! (SpkSparseBase.jl:378-391, SpkLUFactor.jl:60-255)
  * or _ldltfactor! (SpkSparseSpdBase.jl:313-332). precision: SPKHIP_FP64,
- * SPKHIP_FP32 or SPKHIP_FP32_BF16. Returns -1 if any diagonal block met a
+ * SPKHIP_FP32, SPKHIP_FP32_BF16 or SPKHIP_FP32_FP16. Returns -1 if any diagonal block met a
  * zero pivot (the first failing supernode is in stats). */
 int64_t spkhip_factor(spkhip_t h, int64_t precision);
 

@@ -177,7 +177,7 @@ struct spkhip_handle {
   };
   int cur_level = -1, cur_grid = 0;   // tags of the next profiled launches
   size_t upd_group = 8;                // blocks per full trailing update
-  bool lowp = false;                   // bf16 operands in large-front updates
+  int lowp = 0;                        // large-front updates: 1 bf16, 2 scaled fp16 operands
   int mode = 0;                        // precision mode of the last factor
   std::vector<Rec> recs;
   ~spkhip_handle() {
@@ -800,7 +800,7 @@ void run_factor(spkhip_handle* h) {
           h->cur_grid = G.gemm.wg;
           Prof pr(h, SPKHIP_K_UPDATE, G.gemm.flops, G.gemm.elems * sizeof(T));
           hipLaunchKernelGGL(spk::blk_gemm_kernel<T>, dim3(G.gemm.wg), dim3(kWG), 0, st, h->ds,
-                             h->d_gdesc.p + G.gemm.off, G.gemm.cnt, arena.p, h->lowp ? 1 : 0);
+                             h->d_gdesc.p + G.gemm.off, G.gemm.cnt, arena.p, h->lowp);
           hipck(hipGetLastError());
         }
       }
@@ -1441,12 +1441,13 @@ int64_t spkhip_inmatrix_device(spkhip_t h, const double* d_nzval) {
 int64_t spkhip_factor(spkhip_t h, int64_t precision) {
   if (guard(h)) return -4;
   if (!h->have_values) return -3;
-  if (precision != SPKHIP_FP64 && precision != SPKHIP_FP32 && precision != SPKHIP_FP32_BF16)
+  if (precision != SPKHIP_FP64 && precision != SPKHIP_FP32 && precision != SPKHIP_FP32_BF16 &&
+      precision != SPKHIP_FP32_FP16)
     return -4;
   return wrap([&]() -> int64_t {
     hipck(hipSetDevice(h->device));
     hipck(hipEventRecord(h->ev0, h->stream));
-    h->lowp = precision == SPKHIP_FP32_BF16;
+    h->lowp = precision == SPKHIP_FP32_BF16 ? 1 : precision == SPKHIP_FP32_FP16 ? 2 : 0;
     if (precision == SPKHIP_FP64) {
       run_assemble<double>(h);
       run_factor<double>(h);
@@ -1454,7 +1455,7 @@ int64_t spkhip_factor(spkhip_t h, int64_t precision) {
       run_assemble<float>(h);
       run_factor<float>(h);
     }
-    h->lowp = false;
+    h->lowp = 0;
     hipck(hipEventRecord(h->ev1, h->stream));
     hipck(hipEventSynchronize(h->ev1));
     float ms = 0;

@@ -1189,10 +1189,13 @@ __device__ inline unsigned short f2bf(float x) {
   return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
 // With lowp set, a float update of a front wider than 256 columns takes its
-// L21 / U12 operands rounded to bf16 and accumulates in float on
-// v_mfma_f32_16x16x16bf16_1k (operand k = 4 (l / 16) .. + 3 per lane, the
+// L21 / U12 operands in 16 bits and accumulates in float: lowp 1 rounds them
+// to bf16 for v_mfma_f32_16x16x16bf16_1k, lowp 2 scales column k of L21 up
+// and row k of U12 down by the power of two of |U(k, k)| and rounds to fp16
+// for v_mfma_f32_16x16x16f16 (operand k = 4 (l / 16) .. + 3 per lane, the
 // accumulator layout of the f32 form); refinement in double recovers the
 // accuracy.
 template <typename T>
@@ -1252,7 +1255,66 @@ blk_gemm_kernel(DevSym s, const GemmDesc* __restrict__ desc, int ndesc,
   const int nch = (bw + KC - 1) / KC;
   bool done = false;
   if constexpr (std::is_same<T, float>::value) {
-    if (lowp && d.big) {
+    if (lowp == 2 && d.big) {
+      // fp16 operands under diagonal scaling: A(i, k) times and B(k, j)
+      // divided by 2^e_k, 2^e_k the power of two nearest below |U(k, k)|
+      // (the product is unchanged; both stay in fp16 range when A is
+      // diagonally scaled); rows of KB = 20 halves as in the bf16 form
+      constexpr int KB = 20;
+      _Float16* Ah = reinterpret_cast<_Float16*>(&As[0][0]);
+      _Float16* Bh = reinterpret_cast<_Float16*>(&Bs[0][0]);
+      const int akw = __builtin_amdgcn_readfirstlane(ak);
+      float sa[PER], sb = 1.0f;
+      // 2^e and 2^-e of the diagonal entry of front column k (1 when the
+      // entry is zero, subnormal or not finite)
+      auto pow2 = [&](int k, bool inv) {
+        const float dg = F[(int64_t)k * (L + 1)];
+        const unsigned be = (__float_as_uint(dg) >> 23) & 0xffu;
+        if (be == 0u || be >= 254u) return 1.0f;
+        return __uint_as_float((inv ? 254u - be : be) << 23);
+      };
+      auto scales = [&](int k0) {
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+          const int kk = akw + 2 * t;
+          sa[t] = (k0 + kk < bw) ? pow2(d.k0 + k0 + kk, false) : 1.0f;
+        }
+        sb = (k0 + bk < bw) ? pow2(d.k0 + k0 + bk, true) : 1.0f;
+      };
+      auto hstore = [&](int buf) {
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+          Ah[buf * TM * KB + ai * KB + ak + 2 * t] = (_Float16)(ra[t] * sa[t]);
+          Bh[buf * TM * KB + (bj + 16 * t) * KB + bk] = (_Float16)(rb[t] * sb);
+        }
+      };
+      gload(0);
+      scales(0);
+      hstore(0);
+      __syncthreads();
+      for (int ch = 0; ch < nch; ++ch) {
+        const int cur = ch & 1;
+        if (ch + 1 < nch) {
+          gload((ch + 1) * KC);
+          scales((ch + 1) * KC);
+        }
+        f16x4 av[4], bv[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          av[t] = *reinterpret_cast<const f16x4*>(Ah + cur * TM * KB + (wr + t * 16 + l16) * KB + 4 * lk);
+          bv[t] = *reinterpret_cast<const f16x4*>(Bh + cur * TM * KB + (wc + t * 16 + l16) * KB + 4 * lk);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16f16(bv[c], av[a], acc[a][c], 0, 0, 0);
+        if (ch + 1 < nch) hstore(cur ^ 1);
+        __syncthreads();
+      }
+      done = true;
+    }
+    if (lowp == 1 && d.big) {
       // bf16 operands in the same LDS: rows of KB = 20 halves (8-byte
       // aligned groups of four k)
       constexpr int KB = 20;

@@ -24,9 +24,11 @@ from .symbolic import Graph, SymbolicFactor
 FP64 = 0
 FP32 = 1
 FP32_BF16 = 2  # float factor, bf16 operands in the trailing updates of large fronts
+FP32_FP16 = 3  # the same with diagonally scaled fp16 operands
 _PREC = {"fp64": FP64, "float64": FP64, FP64: FP64,
          "fp32": FP32, "float32": FP32, FP32: FP32,
-         "fp32_bf16": FP32_BF16, FP32_BF16: FP32_BF16}
+         "fp32_bf16": FP32_BF16, FP32_BF16: FP32_BF16,
+         "fp32_fp16": FP32_FP16, FP32_FP16: FP32_FP16}
 
 
 def _csc_arrays(m):
