@@ -2,13 +2,15 @@
 """Benchmark: numeric factor + solve of the 3D upwind convection-diffusion
 7-point stencil on a 64^3 grid (n = 262,144; diffusivity 1, velocity
 (1, 0.5, 0.25), cell Peclet 0.5), LU through SparseSolver's numeric phase on
-one MI355X, in three precision paths:
+one MI355X, in four precision paths (a fifth on request):
 
   fp64       factor and solve in double (the reference's Float64 path)
   fp32       factor and solve in float, residual and update in double,
              refined until the normwise backward error is <= 1e-14
   fp32_bf16  as fp32, with bf16 operands (float accumulation) in the
              trailing updates of fronts wider than 256 columns
+  fp32_bf16x3  as fp32_bf16 with split-bf16 operands (x = hi + lo, three bf16
+             products per pair, about 16 significant bits)
   fp32_fp16  as fp32_bf16 with fp16 operands under diagonal scaling (not in
              the default set: at 64^3 it needs as many refinement steps as
              fp32_bf16 and its scaled operand staging costs more)
@@ -46,7 +48,7 @@ HBM_PEAK = 8000.0                       # GB/s
 def kernel_peak(path, kclass):
     if path == "fp64":
         return PEAK["fp64"]
-    if path in ("fp32_bf16", "fp32_fp16") and kclass == "large_update":
+    if path in ("fp32_bf16", "fp32_fp16", "fp32_bf16x3") and kclass == "large_update":
         return PEAK["bf16"]
     return PEAK["fp32"]
 
@@ -63,7 +65,7 @@ def main():
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--cpu-grid", type=int, default=36)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--paths", default="fp64,fp32,fp32_bf16")
+    ap.add_argument("--paths", default="fp64,fp32,fp32_bf16,fp32_bf16x3")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (three times), the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -115,7 +117,7 @@ def main():
     d_x = torch.empty_like(d_b)
     torch.cuda.synchronize()
 
-    prec = {"fp64": 0, "fp32": 1, "fp32_bf16": 2, "fp32_fp16": 3}
+    prec = {"fp64": 0, "fp32": 1, "fp32_bf16": 2, "fp32_fp16": 3, "fp32_bf16x3": 4}
 
     def step(path):
         _lib.check(L.spkhip_inmatrix_device(h, ctypes.c_void_p(d_nz.data_ptr())), "inmatrix")
@@ -221,7 +223,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32+f64ir", "fp32_bf16": "f32+bf16upd+f64ir",
-                  "fp32_fp16": "f32+fp16upd+f64ir"}[head],
+                  "fp32_fp16": "f32+fp16upd+f64ir", "fp32_bf16x3": "f32+bf16x3upd+f64ir"}[head],
         "data": "synthetic (generated matrix and rhs, splitmix64 seeds)",
         "config": {"workload": f"3D upwind convection-diffusion 7-point {g}^3 LU "
                                "(SparseSolver numeric phase: inmatrix+factor+solve)",

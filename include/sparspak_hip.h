@@ -96,6 +96,9 @@ typedef struct spkhip_handle* spkhip_t;
                               nearest below |U(k,k)| (the product is
                               unchanged), which keeps both in fp16 range for
                               diagonally scaled problems */
+#define SPKHIP_FP32_BF16X3 4 /* as SPKHIP_FP32_BF16 with split-bf16 operands
+                              (x = hi + lo, three bf16 products per pair,
+                              about 16 significant bits) */
 
 typedef struct spkhip_stats {
   int64_t n, nsuper, nfront, nlevel;
@@ -138,7 +141,8 @@ int64_t spkhip_inmatrix_device(spkhip_t h, const double* d_nzval);
 
 /* _factor! -> _lufactor! (SpkSparseBase.jl:378-391, SpkLUFactor.jl:60-255)
  * or _ldltfactor! (SpkSparseSpdBase.jl:313-332). precision: SPKHIP_FP64,
- * SPKHIP_FP32, SPKHIP_FP32_BF16 or SPKHIP_FP32_FP16. Returns -1 if any diagonal block met a
+ * SPKHIP_FP32, SPKHIP_FP32_BF16, SPKHIP_FP32_FP16 or SPKHIP_FP32_BF16X3.
+ * Returns -1 if any diagonal block met a
  * zero pivot (the first failing supernode is in stats). */
 int64_t spkhip_factor(spkhip_t h, int64_t precision);
 

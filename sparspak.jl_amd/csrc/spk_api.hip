@@ -177,7 +177,7 @@ struct spkhip_handle {
   };
   int cur_level = -1, cur_grid = 0;   // tags of the next profiled launches
   size_t upd_group = 8;                // blocks per full trailing update
-  int lowp = 0;                        // large-front updates: 1 bf16, 2 scaled fp16 operands
+  int lowp = 0;  // large-front updates: 1 bf16, 2 scaled fp16, 3 split-bf16 operands
   int mode = 0;                        // precision mode of the last factor
   std::vector<Rec> recs;
   ~spkhip_handle() {
@@ -1442,12 +1442,15 @@ int64_t spkhip_factor(spkhip_t h, int64_t precision) {
   if (guard(h)) return -4;
   if (!h->have_values) return -3;
   if (precision != SPKHIP_FP64 && precision != SPKHIP_FP32 && precision != SPKHIP_FP32_BF16 &&
-      precision != SPKHIP_FP32_FP16)
+      precision != SPKHIP_FP32_FP16 && precision != SPKHIP_FP32_BF16X3)
     return -4;
   return wrap([&]() -> int64_t {
     hipck(hipSetDevice(h->device));
     hipck(hipEventRecord(h->ev0, h->stream));
-    h->lowp = precision == SPKHIP_FP32_BF16 ? 1 : precision == SPKHIP_FP32_FP16 ? 2 : 0;
+    h->lowp = precision == SPKHIP_FP32_BF16   ? 1
+              : precision == SPKHIP_FP32_FP16 ? 2
+              : precision == SPKHIP_FP32_BF16X3 ? 3
+                                                : 0;
     if (precision == SPKHIP_FP64) {
       run_assemble<double>(h);
       run_factor<double>(h);

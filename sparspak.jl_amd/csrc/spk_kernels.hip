@@ -1206,8 +1206,13 @@ blk_gemm_kernel(DevSym s, const GemmDesc* __restrict__ desc, int ndesc,
   typedef typename Mfma16<T>::v4 v4;
   // double-buffered: As[buf][i * KP + k] = L(r0 + i, k0 + k),
   //                  Bs[buf][j * KP + k] = U(k0 + k, j0 + j)
-  __shared__ T As[2][TM * KP];
-  __shared__ T Bs[2][TM * KP];
+  // (the split-bf16 form stages four bf16 arrays, hi and lo of A and B, in
+  // the same bytes: 2 buffers x 4 x 128 rows of 20 halves)
+  constexpr int kLdsT = 2 * 2 * TM * KP * (int)sizeof(T);
+  constexpr int kLdsH = std::is_same<T, float>::value ? 2 * 4 * TM * 20 * 2 : 0;
+  __shared__ __attribute__((aligned(16))) unsigned char lds_raw[kLdsT > kLdsH ? kLdsT : kLdsH];
+  T (*As)[TM * KP] = reinterpret_cast<T (*)[TM * KP]>(lds_raw);
+  T (*Bs)[TM * KP] = reinterpret_cast<T (*)[TM * KP]>(lds_raw + 2 * TM * KP * sizeof(T));
   const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
   const int di = find_blk(desc, ndesc, blockIdx.x);
   const GemmDesc d = desc[di];
@@ -1310,6 +1315,61 @@ blk_gemm_kernel(DevSym s, const GemmDesc* __restrict__ desc, int ndesc,
           for (int c = 0; c < 4; ++c)
             acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16f16(bv[c], av[a], acc[a][c], 0, 0, 0);
         if (ch + 1 < nch) hstore(cur ^ 1);
+        __syncthreads();
+      }
+      done = true;
+    }
+    if (lowp == 3 && d.big) {
+      // split bf16: each operand x = hi + lo (hi = bf16(x), lo = bf16(x -
+      // hi)), and each product taken as lo_b hi_a + hi_b lo_a + hi_b hi_a
+      // (the lo lo term dropped): about 16 bits per operand at three times
+      // the matrix-core work of plain bf16, in a kernel bound by its operand
+      // loads
+      constexpr int KB = 20;
+      unsigned short* Ah = reinterpret_cast<unsigned short*>(lds_raw);
+      unsigned short* Al = Ah + 2 * TM * KB;
+      unsigned short* Bh = Al + 2 * TM * KB;
+      unsigned short* Bl = Bh + 2 * TM * KB;
+      auto sstore = [&](int buf) {
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+          const unsigned short ha = f2bf(ra[t]);
+          const unsigned short la = f2bf(ra[t] - __uint_as_float((unsigned)ha << 16));
+          const unsigned short hb = f2bf(rb[t]);
+          const unsigned short lb = f2bf(rb[t] - __uint_as_float((unsigned)hb << 16));
+          const int ia = buf * TM * KB + ai * KB + ak + 2 * t;
+          const int ib = buf * TM * KB + (bj + 16 * t) * KB + bk;
+          Ah[ia] = ha;
+          Al[ia] = la;
+          Bh[ib] = hb;
+          Bl[ib] = lb;
+        }
+      };
+      gload(0);
+      sstore(0);
+      __syncthreads();
+      for (int ch = 0; ch < nch; ++ch) {
+        const int cur = ch & 1;
+        if (ch + 1 < nch) gload((ch + 1) * KC);
+        bf16x4 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int oa = cur * TM * KB + (wr + t * 16 + l16) * KB + 4 * lk;
+          const int ob = cur * TM * KB + (wc + t * 16 + l16) * KB + 4 * lk;
+          ah[t] = *reinterpret_cast<const bf16x4*>(Ah + oa);
+          al[t] = *reinterpret_cast<const bf16x4*>(Al + oa);
+          bh[t] = *reinterpret_cast<const bf16x4*>(Bh + ob);
+          bl[t] = *reinterpret_cast<const bf16x4*>(Bl + ob);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bl[c], ah[a], acc[a][c], 0, 0, 0);
+            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bh[c], al[a], acc[a][c], 0, 0, 0);
+            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bh[c], ah[a], acc[a][c], 0, 0, 0);
+          }
+        if (ch + 1 < nch) sstore(cur ^ 1);
         __syncthreads();
       }
       done = true;

@@ -14,6 +14,7 @@ const SPKHIP_FP64 = 0
 const SPKHIP_FP32 = 1
 const SPKHIP_FP32_BF16 = 2
 const SPKHIP_FP32_FP16 = 3
+const SPKHIP_FP32_BF16X3 = 4
 
 check(rc, what) = rc == 0 ? nothing :
     rc == -1 ? error("$what: zero pivot in a diagonal block") :

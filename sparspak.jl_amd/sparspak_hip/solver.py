@@ -25,10 +25,12 @@ FP64 = 0
 FP32 = 1
 FP32_BF16 = 2  # float factor, bf16 operands in the trailing updates of large fronts
 FP32_FP16 = 3  # the same with diagonally scaled fp16 operands
+FP32_BF16X3 = 4  # the same with split-bf16 operands (three bf16 products per pair)
 _PREC = {"fp64": FP64, "float64": FP64, FP64: FP64,
          "fp32": FP32, "float32": FP32, FP32: FP32,
          "fp32_bf16": FP32_BF16, FP32_BF16: FP32_BF16,
-         "fp32_fp16": FP32_FP16, FP32_FP16: FP32_FP16}
+         "fp32_fp16": FP32_FP16, FP32_FP16: FP32_FP16,
+         "fp32_bf16x3": FP32_BF16X3, FP32_BF16X3: FP32_BF16X3}
 
 
 def _csc_arrays(m):

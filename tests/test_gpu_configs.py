@@ -184,6 +184,27 @@ def test_fp16_trailing_updates_refine_to_1e14():
     assert its["fp32_fp16"] <= its["fp32_bf16"]
 
 
+def test_split_bf16_trailing_updates_refine_to_1e14():
+    # split-bf16 operands (hi + lo, three products): the refinement needs at
+    # most as many steps as plain float updates on this operator need plus
+    # one, and no more than plain bf16
+    _say("case convdiff3d 24^3 fp32_bf16x3")
+    A = convdiff3d(24)
+    n = A.shape[0]
+    xt = uniform(2, n, -1, 1)
+    b = A @ xt
+    its = {}
+    for prec in ("fp32", "fp32_bf16", "fp32_bf16x3"):
+        s = _solver(A, precision=prec)
+        x = b.copy()
+        triangularsolve(s, x, refine=True, tol=1e-14, max_iter=30)
+        eta = backward_error(A, x, b)
+        its[prec] = s.slvr.stats().refine_iters
+        _say(f"  {prec}: eta {eta:.2e} iters {its[prec]}")
+        assert eta <= 1e-14
+    assert its["fp32_bf16x3"] <= min(its["fp32_bf16"], its["fp32"] + 1)
+
+
 def test_bf16_trailing_updates_refine_to_1e14():
     # float factor with bf16 operands in the trailing updates of fronts wider
     # than 256 columns (the 24^3 top separator is), FP64 refinement
