@@ -866,8 +866,9 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
       if (lp.small_cnt > 0) {
         Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
         hipLaunchKernelGGL((spk::fwd_small_kernel<T, MAXW>), dim3(lp.small_cnt), dim3(kWG),
-                           (size_t)lp.small_maxL * sizeof(T), st, h->ds,
-                           h->d_small.p + lp.small_off, lnz, h->d_ipvt.p, rhs, ldr, nrhs, vec.p);
+                           (size_t)(MAXW + 1) * lp.small_maxL * sizeof(T), st, h->ds,
+                           h->d_small.p + lp.small_off, lnz, h->d_ipvt.p, rhs, ldr, nrhs, vec.p,
+                           lp.small_maxL);
         hipck(hipGetLastError());
       }
       if (lp.large_cnt == 0) continue;
@@ -921,8 +922,9 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
       if (lp.small_cnt > 0) {
         Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);
         hipLaunchKernelGGL((spk::bwd_small_kernel<T, MAXW>), dim3(lp.small_cnt), dim3(kWG),
-                           (size_t)lp.small_maxL * sizeof(T), st, h->ds,
-                           h->d_small.p + lp.small_off, lnz, unz, rhs, ldr, nrhs);
+                           (size_t)(MAXW + 1) * lp.small_maxL * sizeof(T), st, h->ds,
+                           h->d_small.p + lp.small_off, lnz, unz, rhs, ldr, nrhs,
+                           lp.small_maxL);
         hipck(hipGetLastError());
       }
     }

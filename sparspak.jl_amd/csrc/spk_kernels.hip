@@ -1565,15 +1565,23 @@ __device__ void wave_upper_solve(T* yb, const T* D, int nq) {
   }
 }
 
-// Small fronts, forward: everything in one workgroup. Dynamic LDS: yW[L].
+// Small fronts, forward: everything in one workgroup and in LDS. y starts
+// as the own rows of x and zero update rows, receives the children's update
+// vectors in a fixed child order, then per piece: the piece's whole panel
+// (rows from its first row to L) is staged once, wave 0 applies the
+// interchanges and the unit-lower block solve, and every row below takes its
+// dot product with the block's entries from the staged panel (the order of
+// dot_lrow). Only the update rows go back to the vector arena (the parent
+// reads nothing else), the own rows to x. Dynamic LDS: the staged panel
+// (MAXW x maxL, maxL the level's longest small front) and yW[maxL].
 template <typename T, int MAXW>
 __global__ void __launch_bounds__(kWG)
 fwd_small_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict__ lnz,
                  const int64_t* __restrict__ ipvt, T* __restrict__ rhs, int64_t ldr, int nrhs,
-                 T* __restrict__ vec) {
+                 T* __restrict__ vec, int maxL) {
   extern __shared__ unsigned char smem_raw[];
-  T* yW = reinterpret_cast<T*>(smem_raw);
-  __shared__ T D[MAXW * MAXW];
+  T* D = reinterpret_cast<T*>(smem_raw);  // MAXW x maxL (leading dimension MAXW)
+  T* yW = D + MAXW * maxL;
   __shared__ int spiv[MAXW];
   const int tid = threadIdx.x, wave = wave_id();
   const int f = list[blockIdx.x];
@@ -1583,39 +1591,57 @@ fwd_small_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict
   for (int q = 0; q < nrhs; ++q) {
     T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
     T* x = rhs + (int64_t)q * ldr;
-    for (int r = tid; r < L; r += kWG) yg[r] = (r < W) ? x[fc0 + r] : T(0);
+    for (int r = tid; r < L; r += kWG) yW[r] = (r < W) ? x[fc0 + r] : T(0);
     __syncthreads();
-    gather_children<T>(s, f, q, nrhs, yg, vec);
-    for (int r = tid; r < L; r += kWG) yW[r] = yg[r];
-    __syncthreads();
+    for (int ci = s.fchp[f]; ci < s.fchp[f + 1]; ++ci) {
+      const int c = s.fch[ci];
+      const int Lc = s.fL[c], Wc = s.fW[c];
+      const int32_t* rel = s.relmap + s.frel[c];
+      const T* uc = vec + s.fvec[c] * nrhs + (int64_t)q * Lc + Wc;
+      for (int t = tid; t < Lc - Wc; t += kWG) yW[rel[t]] += uc[t];
+      __syncthreads();
+    }
     for (int p = sp0; p < sp0 + np; ++p) {
       int64_t pc0, nq64;
       piece_cols(s, p, pc0, nq64);
       const int nq = (int)nq64;
       const int cq = (int)(pc0 - fc0);
-      stage_block<T, MAXW>(D, lnz + (s.xlnz[pc0] - 1), nq, L - cq);
+      const int jl = L - cq;  // the panel's rows
+      const T* lp = lnz + (s.xlnz[pc0] - 1);
+      for (int idx = tid; idx < jl * nq; idx += kWG) {
+        const int i = idx % jl, kk = idx / jl;
+        D[i + kk * MAXW] = lp[i + (int64_t)kk * jl];
+      }
       for (int i = tid; i < nq; i += kWG) spiv[i] = (int)ipvt[pc0 + i] - 1;
       __syncthreads();
       if (wave == 0) wave_lower_solve<T, MAXW>(yW + cq, D, spiv, nq);
       __syncthreads();
-      for (int r = cq + nq + tid; r < L; r += kWG)
-        yW[r] -= dot_lrow<T>(s, lnz, pc0, nq, r, yW + cq);
+      for (int r = cq + nq + tid; r < L; r += kWG) {
+        const T* dr = D + (r - cq);
+        T acc = T(0);
+        for (int kk = 0; kk < nq; ++kk) acc += dr[kk * MAXW] * yW[cq + kk];
+        yW[r] -= acc;
+      }
       __syncthreads();
     }
-    for (int r = tid; r < L; r += kWG) yg[r] = yW[r];
+    for (int r = W + tid; r < L; r += kWG) yg[r] = yW[r];
     for (int r = tid; r < W; r += kWG) x[fc0 + r] = yW[r];
     __syncthreads();
   }
 }
 
-// Small fronts, backward. Dynamic LDS: yW[L].
+// Small fronts, backward: y gathered from x through the index list, then
+// per piece in reverse its U rows staged once in LDS, the U12 products (one
+// wave per row, fixed reduction order) and the upper block solve by wave 0.
+// Dynamic LDS: the staged rows (MAXW x maxL) and yW[maxL].
 template <typename T, int MAXW>
 __global__ void __launch_bounds__(kWG)
 bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict__ lnz,
-                 const T* __restrict__ unz, T* __restrict__ rhs, int64_t ldr, int nrhs) {
+                 const T* __restrict__ unz, T* __restrict__ rhs, int64_t ldr, int nrhs,
+                 int maxL) {
   extern __shared__ unsigned char smem_raw[];
-  T* yW = reinterpret_cast<T*>(smem_raw);
-  __shared__ T D[MAXW * MAXW];
+  T* D = reinterpret_cast<T*>(smem_raw);  // MAXW x maxL (leading dimension MAXW)
+  T* yW = D + MAXW * maxL;
   const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
   const int f = list[blockIdx.x];
   const int L = s.fL[f], W = s.fW[f];
@@ -1631,11 +1657,19 @@ bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict
       piece_cols(s, p, pc0, nq64);
       const int nq = (int)nq64;
       const int cq = (int)(pc0 - fc0);
-      stage_block<T, MAXW>(D, lnz + (s.xlnz[pc0] - 1), nq, L - cq);
+      // the piece's U rows staged once: D(c, j - cq) = U(c, j) for j from
+      // the piece's first column to L (the block from the packed LU in lnz,
+      // the rest from unz)
+      const int jl = L - cq;
+      const T* lp = lnz + (s.xlnz[pc0] - 1);
+      for (int idx = tid; idx < nq * jl; idx += kWG) {
+        const int c = idx % nq, j = idx / nq;
+        D[c + j * MAXW] = (j < nq) ? lp[c + (int64_t)j * jl] : unz[s.ucol[pc0 + c] + cq + j];
+      }
+      __syncthreads();
       for (int c = wave; c < nq; c += 4) {
-        const T* u = unz + s.ucol[pc0 + c];
         T acc = T(0);
-        for (int j = cq + nq + lane; j < L; j += 64) acc += u[j] * yW[j];
+        for (int j = cq + nq + lane; j < L; j += 64) acc += D[c + (j - cq) * MAXW] * yW[j];
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
         if (lane == 0) yW[cq + c] -= acc;
       }
