@@ -841,6 +841,22 @@ void run_assemble(spkhip_handle* h) {
   hipck(hipGetLastError());
 }
 
+// Small fronts per workgroup in the solves (one wave each): four when their
+// staging fits in 150 KB of LDS, fewer otherwise.
+template <typename T, int MAXW>
+int small_fronts_per_wg(int maxL) {
+  static bool attrs = false;
+  if (!attrs) {
+    hipck(hipFuncSetAttribute((const void*)spk::fwd_small_kernel<T, MAXW>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    hipck(hipFuncSetAttribute((const void*)spk::bwd_small_kernel<T, MAXW>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    attrs = true;
+  }
+  const size_t per = (size_t)(MAXW + 1) * std::max(maxL, 1) * sizeof(T);
+  return (int)std::max<size_t>(1, std::min<size_t>(4, 150 * 1024 / per));
+}
+
 template <typename T, int MAXW, int GW>
 void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bwd) {
   hipStream_t st = h->stream;
@@ -865,10 +881,12 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
       h->cur_grid = lp.small_cnt;
       if (lp.small_cnt > 0) {
         Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
-        hipLaunchKernelGGL((spk::fwd_small_kernel<T, MAXW>), dim3(lp.small_cnt), dim3(kWG),
-                           (size_t)(MAXW + 1) * lp.small_maxL * sizeof(T), st, h->ds,
-                           h->d_small.p + lp.small_off, lnz, h->d_ipvt.p, rhs, ldr, nrhs, vec.p,
-                           lp.small_maxL);
+        const int fpw = small_fronts_per_wg<T, MAXW>(lp.small_maxL);
+        hipLaunchKernelGGL((spk::fwd_small_kernel<T, MAXW>),
+                           dim3((lp.small_cnt + fpw - 1) / fpw), dim3(kWG),
+                           (size_t)fpw * (MAXW + 1) * lp.small_maxL * sizeof(T), st, h->ds,
+                           h->d_small.p + lp.small_off, lp.small_cnt, fpw, lnz, h->d_ipvt.p,
+                           rhs, ldr, nrhs, vec.p, lp.small_maxL);
         hipck(hipGetLastError());
       }
       if (lp.large_cnt == 0) continue;
@@ -921,10 +939,12 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
       }
       if (lp.small_cnt > 0) {
         Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);
-        hipLaunchKernelGGL((spk::bwd_small_kernel<T, MAXW>), dim3(lp.small_cnt), dim3(kWG),
-                           (size_t)(MAXW + 1) * lp.small_maxL * sizeof(T), st, h->ds,
-                           h->d_small.p + lp.small_off, lnz, unz, rhs, ldr, nrhs,
-                           lp.small_maxL);
+        const int fpw = small_fronts_per_wg<T, MAXW>(lp.small_maxL);
+        hipLaunchKernelGGL((spk::bwd_small_kernel<T, MAXW>),
+                           dim3((lp.small_cnt + fpw - 1) / fpw), dim3(kWG),
+                           (size_t)fpw * (MAXW + 1) * lp.small_maxL * sizeof(T), st, h->ds,
+                           h->d_small.p + lp.small_off, lp.small_cnt, fpw, lnz, unz, rhs, ldr,
+                           nrhs, lp.small_maxL);
         hipck(hipGetLastError());
       }
     }

@@ -1565,41 +1565,46 @@ __device__ void wave_upper_solve(T* yb, const T* D, int nq) {
   }
 }
 
-// Small fronts, forward: everything in one workgroup and in LDS. y starts
-// as the own rows of x and zero update rows, receives the children's update
-// vectors in a fixed child order, then per piece: the piece's whole panel
-// (rows from its first row to L) is staged once, wave 0 applies the
+// Small fronts, one wave per front (fpw <= 4 fronts per workgroup, as many
+// as the LDS holds), everything in LDS; lane loops stride 64 and a wave
+// barrier orders the steps.
+// Forward: y starts as the own rows of x and zero update rows, receives the
+// children's update vectors in a fixed child order, then per piece: the
+// piece's whole panel (rows from its first row to L) is staged once, the
 // interchanges and the unit-lower block solve, and every row below takes its
 // dot product with the block's entries from the staged panel (the order of
 // dot_lrow). Only the update rows go back to the vector arena (the parent
-// reads nothing else), the own rows to x. Dynamic LDS: the staged panel
-// (MAXW x maxL, maxL the level's longest small front) and yW[maxL].
+// reads nothing else), the own rows to x. Dynamic LDS per wave: the staged
+// panel (MAXW x maxL, maxL the level's longest small front) and yW[maxL].
 template <typename T, int MAXW>
 __global__ void __launch_bounds__(kWG)
-fwd_small_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict__ lnz,
-                 const int64_t* __restrict__ ipvt, T* __restrict__ rhs, int64_t ldr, int nrhs,
-                 T* __restrict__ vec, int maxL) {
+fwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw,
+                 const T* __restrict__ lnz, const int64_t* __restrict__ ipvt,
+                 T* __restrict__ rhs, int64_t ldr, int nrhs, T* __restrict__ vec, int maxL) {
   extern __shared__ unsigned char smem_raw[];
-  T* D = reinterpret_cast<T*>(smem_raw);  // MAXW x maxL (leading dimension MAXW)
+  __shared__ int spiv_all[4][MAXW];
+  const int lane = threadIdx.x % 64, wave = wave_id();
+  const int item = blockIdx.x * fpw + wave;
+  if (wave >= fpw || item >= cnt) return;
+  T* D = reinterpret_cast<T*>(smem_raw) + (int64_t)wave * (MAXW + 1) * maxL;
   T* yW = D + MAXW * maxL;
-  __shared__ int spiv[MAXW];
-  const int tid = threadIdx.x, wave = wave_id();
-  const int f = list[blockIdx.x];
+  int* spiv = spiv_all[wave];
+  const int f = list[item];
   const int L = s.fL[f], W = s.fW[f];
   const int64_t fc0 = s.fc0[f];
   const int sp0 = s.fsp0[f], np = s.fnp[f];
   for (int q = 0; q < nrhs; ++q) {
     T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
     T* x = rhs + (int64_t)q * ldr;
-    for (int r = tid; r < L; r += kWG) yW[r] = (r < W) ? x[fc0 + r] : T(0);
-    __syncthreads();
+    for (int r = lane; r < L; r += 64) yW[r] = (r < W) ? x[fc0 + r] : T(0);
+    __builtin_amdgcn_wave_barrier();
     for (int ci = s.fchp[f]; ci < s.fchp[f + 1]; ++ci) {
       const int c = s.fch[ci];
       const int Lc = s.fL[c], Wc = s.fW[c];
       const int32_t* rel = s.relmap + s.frel[c];
       const T* uc = vec + s.fvec[c] * nrhs + (int64_t)q * Lc + Wc;
-      for (int t = tid; t < Lc - Wc; t += kWG) yW[rel[t]] += uc[t];
-      __syncthreads();
+      for (int t = lane; t < Lc - Wc; t += 64) yW[rel[t]] += uc[t];
+      __builtin_amdgcn_wave_barrier();
     }
     for (int p = sp0; p < sp0 + np; ++p) {
       int64_t pc0, nq64;
@@ -1608,50 +1613,52 @@ fwd_small_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict
       const int cq = (int)(pc0 - fc0);
       const int jl = L - cq;  // the panel's rows
       const T* lp = lnz + (s.xlnz[pc0] - 1);
-      for (int idx = tid; idx < jl * nq; idx += kWG) {
+      for (int idx = lane; idx < jl * nq; idx += 64) {
         const int i = idx % jl, kk = idx / jl;
         D[i + kk * MAXW] = lp[i + (int64_t)kk * jl];
       }
-      for (int i = tid; i < nq; i += kWG) spiv[i] = (int)ipvt[pc0 + i] - 1;
-      __syncthreads();
-      if (wave == 0) wave_lower_solve<T, MAXW>(yW + cq, D, spiv, nq);
-      __syncthreads();
-      for (int r = cq + nq + tid; r < L; r += kWG) {
+      for (int i = lane; i < nq; i += 64) spiv[i] = (int)ipvt[pc0 + i] - 1;
+      __builtin_amdgcn_wave_barrier();
+      wave_lower_solve<T, MAXW>(yW + cq, D, spiv, nq);
+      __builtin_amdgcn_wave_barrier();
+      for (int r = cq + nq + lane; r < L; r += 64) {
         const T* dr = D + (r - cq);
         T acc = T(0);
         for (int kk = 0; kk < nq; ++kk) acc += dr[kk * MAXW] * yW[cq + kk];
         yW[r] -= acc;
       }
-      __syncthreads();
+      __builtin_amdgcn_wave_barrier();
     }
-    for (int r = W + tid; r < L; r += kWG) yg[r] = yW[r];
-    for (int r = tid; r < W; r += kWG) x[fc0 + r] = yW[r];
-    __syncthreads();
+    for (int r = W + lane; r < L; r += 64) yg[r] = yW[r];
+    for (int r = lane; r < W; r += 64) x[fc0 + r] = yW[r];
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
-// Small fronts, backward: y gathered from x through the index list, then
-// per piece in reverse its U rows staged once in LDS, the U12 products (one
-// wave per row, fixed reduction order) and the upper block solve by wave 0.
-// Dynamic LDS: the staged rows (MAXW x maxL) and yW[maxL].
+// Small fronts, backward, one wave per front: y gathered from x through the
+// index list, then per piece in reverse its U rows staged once in LDS, the
+// U12 products (fixed reduction order) and the upper block solve. Dynamic
+// LDS per wave: the staged rows (MAXW x maxL) and yW[maxL].
 template <typename T, int MAXW>
 __global__ void __launch_bounds__(kWG)
-bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict__ lnz,
-                 const T* __restrict__ unz, T* __restrict__ rhs, int64_t ldr, int nrhs,
-                 int maxL) {
+bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw,
+                 const T* __restrict__ lnz, const T* __restrict__ unz, T* __restrict__ rhs,
+                 int64_t ldr, int nrhs, int maxL) {
   extern __shared__ unsigned char smem_raw[];
-  T* D = reinterpret_cast<T*>(smem_raw);  // MAXW x maxL (leading dimension MAXW)
+  const int lane = threadIdx.x % 64, wave = wave_id();
+  const int item = blockIdx.x * fpw + wave;
+  if (wave >= fpw || item >= cnt) return;
+  T* D = reinterpret_cast<T*>(smem_raw) + (int64_t)wave * (MAXW + 1) * maxL;
   T* yW = D + MAXW * maxL;
-  const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
-  const int f = list[blockIdx.x];
+  const int f = list[item];
   const int L = s.fL[f], W = s.fW[f];
   const int64_t fc0 = s.fc0[f];
   const int sp0 = s.fsp0[f], np = s.fnp[f];
   const int32_t* I = s.lindx + (s.xlindx[sp0] - 1);
   for (int q = 0; q < nrhs; ++q) {
     T* x = rhs + (int64_t)q * ldr;
-    for (int r = tid; r < L; r += kWG) yW[r] = x[I[r] - 1];
-    __syncthreads();
+    for (int r = lane; r < L; r += 64) yW[r] = x[I[r] - 1];
+    __builtin_amdgcn_wave_barrier();
     for (int p = sp0 + np - 1; p >= sp0; --p) {
       int64_t pc0, nq64;
       piece_cols(s, p, pc0, nq64);
@@ -1662,23 +1669,23 @@ bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict
       // the rest from unz)
       const int jl = L - cq;
       const T* lp = lnz + (s.xlnz[pc0] - 1);
-      for (int idx = tid; idx < nq * jl; idx += kWG) {
+      for (int idx = lane; idx < nq * jl; idx += 64) {
         const int c = idx % nq, j = idx / nq;
         D[c + j * MAXW] = (j < nq) ? lp[c + (int64_t)j * jl] : unz[s.ucol[pc0 + c] + cq + j];
       }
-      __syncthreads();
-      for (int c = wave; c < nq; c += 4) {
+      __builtin_amdgcn_wave_barrier();
+      for (int c = 0; c < nq; ++c) {
         T acc = T(0);
         for (int j = cq + nq + lane; j < L; j += 64) acc += D[c + (j - cq) * MAXW] * yW[j];
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
         if (lane == 0) yW[cq + c] -= acc;
       }
-      __syncthreads();
-      if (wave == 0) wave_upper_solve<T, MAXW>(yW + cq, D, nq);
-      __syncthreads();
+      __builtin_amdgcn_wave_barrier();
+      wave_upper_solve<T, MAXW>(yW + cq, D, nq);
+      __builtin_amdgcn_wave_barrier();
     }
-    for (int c = tid; c < W; c += kWG) x[fc0 + c] = yW[c];
-    __syncthreads();
+    for (int c = lane; c < W; c += 64) x[fc0 + c] = yW[c];
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
