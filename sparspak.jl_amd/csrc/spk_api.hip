@@ -892,8 +892,11 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
       if (lp.large_cnt == 0) continue;
       {
         Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
-        hipLaunchKernelGGL(spk::fwd_assemble_kernel<T>, dim3(lp.large_cnt), dim3(kWG), 0, st,
-                           h->ds, h->d_large.p + lp.large_off, rhs, ldr, nrhs, vec.p);
+        const size_t ylds = (size_t)lp.large_maxL * sizeof(T);
+        const bool staged = ylds <= 64 * 1024;
+        hipLaunchKernelGGL(spk::fwd_assemble_kernel<T>, dim3(lp.large_cnt), dim3(kWG),
+                           staged ? ylds : 0, st, h->ds, h->d_large.p + lp.large_off, rhs, ldr,
+                           nrhs, vec.p, (int)staged);
         hipck(hipGetLastError());
       }
       for (const LevelPlan::SolveStep& S : lp.fsteps) {

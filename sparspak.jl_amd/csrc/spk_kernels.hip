@@ -1513,7 +1513,8 @@ __device__ inline T dot_lrow(const DevSym& s, const T* __restrict__ lnz, int64_t
   return acc;
 }
 
-// Children's update vectors into y (fixed child order).
+// Children's update vectors into y (fixed child order; y in LDS or in the
+// vector arena).
 template <typename T>
 __device__ void gather_children(const DevSym& s, int f, int q, int nrhs, T* yg, T* vec) {
   for (int ci = s.fchp[f]; ci < s.fchp[f + 1]; ++ci) {
@@ -1694,16 +1695,23 @@ bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw,
 template <typename T>
 __global__ void __launch_bounds__(kWG)
 fwd_assemble_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict__ rhs,
-                    int64_t ldr, int nrhs, T* __restrict__ vec) {
+                    int64_t ldr, int nrhs, T* __restrict__ vec, int staged) {
+  extern __shared__ unsigned char smem_raw[];
   const int f = list[blockIdx.x];
   const int L = s.fL[f], W = s.fW[f];
   const int64_t fc0 = s.fc0[f];
   for (int q = 0; q < nrhs; ++q) {
     T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
     const T* x = rhs + (int64_t)q * ldr;
-    for (int r = threadIdx.x; r < L; r += kWG) yg[r] = (r < W) ? x[fc0 + r] : T(0);
+    // y in LDS when the level's longest front fits (staged), else in place
+    T* y = staged ? reinterpret_cast<T*>(smem_raw) : yg;
+    for (int r = threadIdx.x; r < L; r += kWG) y[r] = (r < W) ? x[fc0 + r] : T(0);
     __syncthreads();
-    gather_children<T>(s, f, q, nrhs, yg, vec);
+    gather_children<T>(s, f, q, nrhs, y, vec);
+    if (staged) {
+      for (int r = threadIdx.x; r < L; r += kWG) yg[r] = y[r];
+      __syncthreads();
+    }
   }
 }
 
