@@ -708,9 +708,11 @@ template <typename T>
 void set_lds_attrs() {
   static bool done = false;
   if (done) return;
-  hipck(hipFuncSetAttribute((const void*)spk::small_front_kernel<T>,
+  hipck(hipFuncSetAttribute((const void*)spk::small_front_kernel<T, kWG>,
                             hipFuncAttributeMaxDynamicSharedMemorySize,
                             kSmallLmax * kSmallLmax * (int)sizeof(T)));
+  hipck(hipFuncSetAttribute((const void*)spk::small_front_kernel<T, 64>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
   hipck(hipFuncSetAttribute((const void*)spk::blk_diag_kernel<T, kNqLU>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
   hipck(hipFuncSetAttribute((const void*)spk::blk_diag_kernel<T, kNqSPD>,
@@ -746,11 +748,22 @@ void run_factor(spkhip_handle* h) {
     h->cur_level = (int)(&lp - h->levels.data());
     h->cur_grid = lp.small_cnt;
     if (lp.small_cnt > 0) {
-      size_t lds = (size_t)lp.small_maxL * lp.small_maxL * sizeof(T);
+      // levels of many small fronts: one wave per front, as many per
+      // workgroup as 150 KB of LDS hold (at most four); otherwise one
+      // workgroup per front
+      const size_t per = (size_t)lp.small_maxL * lp.small_maxL * sizeof(T);
       Prof pr(h, SPKHIP_K_SMALL, lp.small_flops, lp.small_elems * sizeof(T));
-      hipLaunchKernelGGL(spk::small_front_kernel<T>, dim3(lp.small_cnt), dim3(kWG), lds, st,
-                         h->ds, h->d_small.p + lp.small_off, lnz.p, unz.p, h->d_ipvt.p,
-                         arena.p, h->d_fail.p);
+      if (lp.small_cnt >= 16384) {
+        const int fpw = (int)std::max<size_t>(1, std::min<size_t>(4, 150 * 1024 / per));
+        hipLaunchKernelGGL((spk::small_front_kernel<T, 64>), dim3((lp.small_cnt + fpw - 1) / fpw),
+                           dim3(kWG), fpw * per, st, h->ds, h->d_small.p + lp.small_off,
+                           lp.small_cnt, fpw, lp.small_maxL, lnz.p, unz.p, h->d_ipvt.p, arena.p,
+                           h->d_fail.p);
+      } else {
+        hipLaunchKernelGGL((spk::small_front_kernel<T, kWG>), dim3(lp.small_cnt), dim3(kWG), per,
+                           st, h->ds, h->d_small.p + lp.small_off, lp.small_cnt, 1,
+                           lp.small_maxL, lnz.p, unz.p, h->d_ipvt.p, arena.p, h->d_fail.p);
+      }
       hipck(hipGetLastError());
     }
     if (lp.items_cnt > 0) {

@@ -102,15 +102,20 @@ __global__ void scatter_a2_kernel(const double* __restrict__ nzval,
 // multipliers are formed with the reciprocal of the pivot as ggetrf!/gtrsm!
 // do (GenericBlasLapackFragments.jl:80-84, 376-395).
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int G>
 __device__ void lds_partial_lu(T* F, int L, int cq, int nq, int pivot,
-                               int* piv_out, int* zero_out) {
-  __shared__ int s_kp;
-  __shared__ T s_inv;
-  const int tid = threadIdx.x;
+                               int* piv_out, int* zero_out, int gt, int* s_kp_, T* s_inv_) {
+  // G threads (a workgroup of 256 or one wave of 64), gt = thread index in
+  // the group; s_kp_ / s_inv_ one LDS slot of the group
+  auto gsync = [] {
+    if constexpr (G == kWG)
+      __syncthreads();
+    else
+      __builtin_amdgcn_wave_barrier();
+  };
   for (int k = 0; k < nq; ++k) {
     const int col = cq + k;
-    if (tid == 0) {
+    if (gt == 0) {
       int kp = col;
       if (pivot) {
         T amax = fabs(F[col + col * L]);
@@ -122,61 +127,75 @@ __device__ void lds_partial_lu(T* F, int L, int cq, int nq, int pivot,
           }
         }
       }
-      s_kp = kp;
+      *s_kp_ = kp;
       T pv = F[kp + col * L];
       if (pv == T(0)) *zero_out = 1;
-      s_inv = T(1) / pv;
+      *s_inv_ = T(1) / pv;
       piv_out[k] = kp - cq;
     }
-    __syncthreads();
-    const int kp = s_kp;
+    gsync();
+    const int kp = *s_kp_;
     const bool nonzero = (F[kp + col * L] != T(0));
     if (kp != col && nonzero) {
-      for (int c = cq + tid; c < L; c += kWG) {
+      for (int c = cq + gt; c < L; c += G) {
         T t = F[col + c * L];
         F[col + c * L] = F[kp + c * L];
         F[kp + c * L] = t;
       }
     }
-    __syncthreads();
-    const T inv = s_inv;
+    gsync();
+    const T inv = *s_inv_;
     // Rows inside the diagonal block are scaled only for a nonzero pivot
     // (ggetrf!); rows below it always (gtrsm! multiplies by 1/U(j,j)).
-    for (int r = col + 1 + tid; r < L; r += kWG)
+    for (int r = col + 1 + gt; r < L; r += G)
       if (r >= cq + nq || nonzero) F[r + col * L] *= inv;
-    __syncthreads();
+    gsync();
     const int m = L - col - 1;
     const int cnt = m * m;
-    for (int idx = tid; idx < cnt; idx += kWG) {
+    for (int idx = gt; idx < cnt; idx += G) {
       int r = col + 1 + idx % m;
       int c = col + 1 + idx / m;
       F[r + c * L] -= F[r + col * L] * F[col + c * L];
     }
-    __syncthreads();
+    gsync();
   }
 }
 
-// One workgroup per small front (L <= the LDS limit of the launch):
-// gather the pieces' panels and the children's update matrices into LDS,
-// factor, write the panels back to lnz/unz/ipvt and the update matrix to the
-// arena.
-template <typename T>
+// One small front per group of G threads (G = 256: a workgroup; G = 64: a
+// wave, fpw fronts per workgroup, for levels of many tiny fronts): gather
+// the pieces' panels and the children's update matrices into LDS (L x L per
+// front), factor, write the panels back to lnz/unz/ipvt and the update
+// matrix to the arena.
+template <typename T, int G>
 __global__ void __launch_bounds__(kWG)
-small_front_kernel(DevSym s, const int32_t* __restrict__ list,
+small_front_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw, int maxL,
                    T* __restrict__ lnz, T* __restrict__ unz,
                    int64_t* __restrict__ ipvt, T* __restrict__ arena,
                    unsigned long long* __restrict__ first_fail) {
   extern __shared__ unsigned char smem_raw[];
-  T* F = reinterpret_cast<T*>(smem_raw);
-  __shared__ int s_piv[kSmallLmax];  // a piece of a small front has <= L columns
-  __shared__ int s_zero;
-  const int tid = threadIdx.x;
-  const int f = list[blockIdx.x];
+  __shared__ int s_piv_all[kWG / G][kSmallLmax];  // a piece of a small front has <= L columns
+  __shared__ int s_zero_all[kWG / G];
+  __shared__ int s_kp_all[kWG / G];
+  __shared__ T s_inv_all[kWG / G];
+  const int grp = (G == kWG) ? 0 : wave_id();
+  const int tid = (G == kWG) ? (int)threadIdx.x : (int)threadIdx.x % 64;
+  const int item = blockIdx.x * fpw + grp;
+  if (grp >= fpw || item >= cnt) return;
+  auto gsync = [] {
+    if constexpr (G == kWG)
+      __syncthreads();
+    else
+      __builtin_amdgcn_wave_barrier();
+  };
+  T* F = reinterpret_cast<T*>(smem_raw) + (int64_t)grp * maxL * maxL;
+  int* s_piv = s_piv_all[grp];
+  int& s_zero = s_zero_all[grp];
+  const int f = list[item];
   const int L = s.fL[f], W = s.fW[f];
   const int64_t fc0 = s.fc0[f];
   const int sp0 = s.fsp0[f], np = s.fnp[f];
-  for (int i = tid; i < L * L; i += kWG) F[i] = T(0);
-  __syncthreads();
+  for (int i = tid; i < L * L; i += G) F[i] = T(0);
+  gsync();
   // pieces' panels
   for (int p = sp0; p < sp0 + np; ++p) {
     int64_t pc0, nq;
@@ -184,26 +203,26 @@ small_front_kernel(DevSym s, const int32_t* __restrict__ list,
     const int cq = (int)(pc0 - fc0);
     const int jl = L - cq;
     const T* lp = lnz + (s.xlnz[pc0] - 1);
-    for (int idx = tid; idx < jl * nq; idx += kWG) {
+    for (int idx = tid; idx < jl * nq; idx += G) {
       int r = idx % jl, c = idx / jl;
       if (!s.sym || r >= c) F[(cq + r) + (cq + c) * L] = lp[idx];
     }
     if (!s.sym) {
       const int ul = jl - (int)nq;
       const T* up = unz + (s.xunz[pc0] - 1);
-      for (int idx = tid; idx < ul * nq; idx += kWG) {
+      for (int idx = tid; idx < ul * nq; idx += G) {
         int r = idx % ul, c = idx / ul;
         F[(cq + c) + (cq + (int)nq + r) * L] = up[idx];
       }
     }
   }
-  __syncthreads();
+  gsync();
   if (s.sym) {  // upper rows of the own columns from the lower triangle
-    for (int idx = tid; idx < W * L; idx += kWG) {
+    for (int idx = tid; idx < W * L; idx += G) {
       int r = idx % W, j = idx / W;
       if (j > r) F[r + j * L] = F[j + r * L];
     }
-    __syncthreads();
+    gsync();
   }
   // children's update matrices (one child at a time: rows of different
   // children may coincide)
@@ -213,26 +232,27 @@ small_front_kernel(DevSym s, const int32_t* __restrict__ list,
     const int32_t* rel = s.relmap + s.frel[c];
     const T* cb = arena + s.fcb[c];
     const int ld = s.fcbld[c];
-    for (int idx = tid; idx < Rc * Rc; idx += kWG) {
+    for (int idx = tid; idx < Rc * Rc; idx += G) {
       int tp = idx % Rc, t = idx / Rc;
       F[rel[tp] + rel[t] * L] += cb[tp + (int64_t)t * ld];
     }
-    __syncthreads();
+    gsync();
   }
   // factor piece by piece
   if (tid == 0) s_zero = 0;
-  __syncthreads();
+  gsync();
   for (int p = sp0; p < sp0 + np; ++p) {
     int64_t pc0, nq;
     piece_cols(s, p, pc0, nq);
     const int cq = (int)(pc0 - fc0);
-    lds_partial_lu<T>(F, L, cq, (int)nq, s.pivot, s_piv, &s_zero);
-    for (int k = tid; k < nq; k += kWG) ipvt[pc0 + k] = s_piv[k] + 1;
+    lds_partial_lu<T, G>(F, L, cq, (int)nq, s.pivot, s_piv, &s_zero, tid, &s_kp_all[grp],
+                         &s_inv_all[grp]);
+    for (int k = tid; k < nq; k += G) ipvt[pc0 + k] = s_piv[k] + 1;
     if (tid == 0 && s_zero) {
       atomicMin(first_fail, (unsigned long long)(p + 1));
       s_zero = 0;
     }
-    __syncthreads();
+    gsync();
   }
   // write back panels
   for (int p = sp0; p < sp0 + np; ++p) {
@@ -241,13 +261,13 @@ small_front_kernel(DevSym s, const int32_t* __restrict__ list,
     const int cq = (int)(pc0 - fc0);
     const int jl = L - cq;
     T* lp = lnz + (s.xlnz[pc0] - 1);
-    for (int idx = tid; idx < jl * nq; idx += kWG) {
+    for (int idx = tid; idx < jl * nq; idx += G) {
       int r = idx % jl, c = idx / jl;
       lp[idx] = F[(cq + r) + (cq + c) * L];
     }
     const int ul = jl - (int)nq;
     T* up = unz + (s.xunz[pc0] - 1);
-    for (int idx = tid; idx < ul * nq; idx += kWG) {
+    for (int idx = tid; idx < ul * nq; idx += G) {
       int r = idx % ul, c = idx / ul;
       up[idx] = F[(cq + c) + (cq + (int)nq + r) * L];
     }
@@ -255,7 +275,7 @@ small_front_kernel(DevSym s, const int32_t* __restrict__ list,
   // update matrix
   const int R = L - W;
   T* cb = arena + s.fcb[f];
-  for (int idx = tid; idx < R * R; idx += kWG) {
+  for (int idx = tid; idx < R * R; idx += G) {
     int tp = idx % R, t = idx / R;
     cb[idx] = F[(W + tp) + (W + t) * L];
   }
