@@ -613,8 +613,15 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         if (bi < 0) continue;
         const Blk& B = fblocks[fi][bi];
         if (B.c1 >= h->fL[large[fi]]) continue;
-        for (i32 r0 = 0; r0 < B.c1 - B.c0; ++r0)
-          sdesc.push_back(sd(large[fi], B, r0, 0));
+        for (i32 r0 = 0; r0 < B.c1 - B.c0; ++r0) {
+          spk::SolveDesc D = sd(large[fi], B, r0, 0);
+          // unz offset of the row in front coordinates (as DevSym::ucol)
+          const i64 g = h->fc0[large[fi]] + B.c0 + r0;
+          const i64 pp = h->snode[g] - 1;
+          const i64 pc0 = h->xsuper[pp] - 1, nq = h->xsuper[pp + 1] - h->xsuper[pp];
+          D.ucol = h->xunz[g] - 1 - (pc0 - h->fc0[large[fi]] + nq);
+          sdesc.push_back(D);
+        }
       }
       S.gemv_cnt = (int)sdesc.size() - S.gemv_off;
       S.chain_off = (int)sdesc.size();

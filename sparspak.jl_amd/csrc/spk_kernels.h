@@ -100,7 +100,7 @@ struct SolveDesc {
   int32_t L, W;    // the front's index-list length and own columns
   int64_t fc0;     // the front's first column
   int64_t fvec;    // the front's offset in the solve vector arena
-  int64_t pad;
+  int64_t ucol;    // backward GEMV: s.ucol of its row (unz offset)
 };
 
 }  // namespace spk

@@ -2036,6 +2036,9 @@ fwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   for (int q = 0; q < nrhs; ++q) {
     T* yg = vec + d.fvec * nrhs + (int64_t)q * L;
     if (tid < bw) zs[tid] = yg[c0 + tid];
+    // this tile's rows of y, requested with the block's entries (no other
+    // workgroup touches them)
+    const T yr = (wave == 0 && r < L) ? yg[r] : T(0);
     __syncthreads();
     T acc = T(0);
     if (r < L) {
@@ -2048,7 +2051,7 @@ fwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
     }
     red[wave][lane] = acc;
     __syncthreads();
-    if (wave == 0 && r < L) yg[r] -= ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+    if (wave == 0 && r < L) yg[r] = yr - (((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]);
     __syncthreads();
   }
 }
@@ -2097,7 +2100,7 @@ bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   const int c = d.c0 + d.r0;
   const int L = d.L;
   const int64_t fc0 = d.fc0;
-  const T* u = unz + s.ucol[fc0 + c];
+  const T* u = unz + d.ucol;  // the row's U entries (its unz offset rides in the descriptor)
   for (int q = 0; q < nrhs; ++q) {
     T* yg = vec + d.fvec * nrhs + (int64_t)q * L;
     T acc = T(0);
