@@ -180,8 +180,15 @@ struct spkhip_handle {
   int lowp = 0;  // large-front updates: 1 bf16, 2 scaled fp16, 3 split-bf16 operands
   int mode = 0;                        // precision mode of the last factor
   std::vector<Rec> recs;
+  // the solve's launch sequence captured as a HIP graph and replayed while
+  // the buffers it names are unchanged (SPKHIP_SOLVE_GRAPH=0 disables)
+  int sym_gen = 0;                     // bumped by every symbolic upload
+  hipGraphExec_t g_exec = nullptr;
+  std::vector<const void*> g_key;
+  std::vector<const void*> g_seen;  // last key run directly (captured on its second use)
   ~spkhip_handle() {
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+    if (g_exec) (void)hipGraphExecDestroy(g_exec);
   }
 };
 
@@ -877,6 +884,21 @@ int small_fronts_per_wg(int maxL) {
   return (int)std::max<size_t>(1, std::min<size_t>(4, 150 * 1024 / per));
 }
 
+// The per-block solve data of the current factor (once per factorisation).
+template <typename T>
+void solve_prep(spkhip_handle* h) {
+  if (h->prepped || h->pdesc.empty()) return;
+  DevBuf<T>& prep = Bufs<T>::prep(h);
+  prep.ensure((size_t)h->prep_units * 4);
+  h->d_prepi.ensure(h->pdesc.size() * spk::kPrepInts);
+  Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
+  hipLaunchKernelGGL(spk::solve_prep_kernel<T>, dim3((unsigned)h->pdesc.size()), dim3(kWG), 0,
+                     h->stream, h->ds, h->d_pdesc.p, Bufs<T>::lnz(h).p, Bufs<T>::unz(h).p,
+                     h->d_ipvt.p, prep.p, h->d_prepi.p);
+  hipck(hipGetLastError());
+  h->prepped = true;
+}
+
 template <typename T, int MAXW, int GW>
 void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bwd) {
   hipStream_t st = h->stream;
@@ -886,15 +908,7 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
   const T* lnz = Bufs<T>::lnz(h).p;
   const T* unz = Bufs<T>::unz(h).p;
   DevBuf<T>& prep = Bufs<T>::prep(h);
-  if (!h->prepped && !h->pdesc.empty()) {
-    prep.ensure((size_t)h->prep_units * 4);
-    h->d_prepi.ensure(h->pdesc.size() * spk::kPrepInts);
-    Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
-    hipLaunchKernelGGL(spk::solve_prep_kernel<T>, dim3((unsigned)h->pdesc.size()), dim3(kWG), 0,
-                       st, h->ds, h->d_pdesc.p, lnz, unz, h->d_ipvt.p, prep.p, h->d_prepi.p);
-    hipck(hipGetLastError());
-    h->prepped = true;
-  }
+  solve_prep<T>(h);
   if (fwd)
     for (const LevelPlan& lp : h->levels) {
       h->cur_level = (int)(&lp - h->levels.data());
@@ -974,11 +988,55 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
 }
 
 template <typename T>
-void solve_levels(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bwd) {
+void solve_levels_direct(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bwd) {
   if (h->kind == SPKHIP_KIND_LU)
     solve_levels_w<T, 64, kGW>(h, rhs, ldr, nrhs, fwd, bwd);
   else
     solve_levels_w<T, 96, 96>(h, rhs, ldr, nrhs, fwd, bwd);
+}
+
+// Both sweeps replay a captured graph of their launches when nothing they
+// name has changed (the buffers, the right-hand-side shape, the symbolic
+// generation); the capture itself allocates nothing and contains no memset
+// (every buffer is sized and the solve data built before it). A key is
+// captured on its second use, so one-shot handles pay no capture.
+template <typename T>
+void solve_levels(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bwd) {
+  const char* ge = std::getenv("SPKHIP_SOLVE_GRAPH");
+  const bool use_graph = !(ge && ge[0] == '0');
+  if (!use_graph || h->prof || !(fwd && bwd)) {
+    solve_levels_direct<T>(h, rhs, ldr, nrhs, fwd, bwd);
+    return;
+  }
+  set_lds_attrs<T>();
+  small_fronts_per_wg<T, 64>(1);
+  small_fronts_per_wg<T, 96>(1);
+  DevBuf<T>& vec = Bufs<T>::vec(h);
+  vec.ensure((size_t)std::max<i64>(h->vec_elems * nrhs, 1));
+  solve_prep<T>(h);
+  const std::vector<const void*> key = {
+      (const void*)rhs, (const void*)(intptr_t)ldr, (const void*)(intptr_t)nrhs,
+      (const void*)(intptr_t)sizeof(T), (const void*)(intptr_t)h->sym_gen, vec.p,
+      Bufs<T>::prep(h).p, h->d_prepi.p, Bufs<T>::lnz(h).p, Bufs<T>::unz(h).p, h->d_ipvt.p};
+  if (!h->g_exec || h->g_key != key) {
+    if (h->g_seen != key) {
+      h->g_seen = key;
+      solve_levels_direct<T>(h, rhs, ldr, nrhs, true, true);
+      return;
+    }
+    if (h->g_exec) {
+      hipck(hipGraphExecDestroy(h->g_exec));
+      h->g_exec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    hipck(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    solve_levels_direct<T>(h, rhs, ldr, nrhs, true, true);
+    hipck(hipStreamEndCapture(h->stream, &g));
+    hipck(hipGraphInstantiate(&h->g_exec, g, nullptr, nullptr, 0));
+    hipck(hipGraphDestroy(g));
+    h->g_key = key;
+  }
+  hipck(hipGraphLaunch(h->g_exec, h->stream));
 }
 
 // Full solve of device B (double, original numbering) with the current
@@ -1171,6 +1229,7 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   d.sym = (kind == SPKHIP_KIND_SPD) ? 1 : 0;
   sync(h);
   h->have_sym = true;
+  h->sym_gen += 1;
   h->have_csc = false;
   h->have_values = false;
   h->factored = false;
