@@ -245,6 +245,34 @@ typedef struct spkhip_kstat {
 int64_t spkhip_profile_enable(spkhip_t h, int64_t on);
 int64_t spkhip_profile_get(spkhip_t h, int64_t kclass, spkhip_kstat_t* out);
 
+/* Plan knobs of the numeric phase and the autotuner that picks them. No
+ * counterpart in the reference (its dense kernels have no tiles to choose).
+ *   small_lmax      fronts (fundamental supernodes) with at most this many
+ *                   rows are factored whole in LDS, larger ones block by
+ *                   block on the matrix cores: the small-versus-MFMA split
+ *                   (0..64, default 64)
+ *   block_cols      widest block of consecutive pieces of an LU front, the
+ *                   tile width of the diagonal, off-diagonal and update
+ *                   kernels (32..128, default 128; SPD blocks are one piece)
+ *   upd_group       blocks per wide trailing update, i.e. its depth in
+ *                   blocks (1..64, default 8)
+ *   small_wave_min  small fronts in a level from which one wave (instead of
+ *                   one workgroup) takes a front (>= 1, default 16384)
+ *   factor_ms       autotune: the best measured factor time; ignored by set
+ * Every setting in range gives a correct factor. set rebuilds the launch
+ * plan when a knob changes (the factor must then be recomputed: -3 from the
+ * solves until it is). autotune needs values (inmatrix); it times
+ * spkhip_factor at `precision` over a coordinate search of the knobs (one
+ * untimed factor per plan, then the minimum of `reps` timed ones, reps 1..10),
+ * keeps the fastest, and leaves the handle factored with it. */
+typedef struct spkhip_tuning {
+  int64_t small_lmax, block_cols, upd_group, small_wave_min;
+  double factor_ms;
+} spkhip_tuning_t;
+int64_t spkhip_get_tuning(spkhip_t h, spkhip_tuning_t* out);
+int64_t spkhip_set_tuning(spkhip_t h, const spkhip_tuning_t* t);
+int64_t spkhip_autotune(spkhip_t h, int64_t precision, int64_t reps, spkhip_tuning_t* out);
+
 /* Library/device information. */
 int64_t spkhip_device_count(void);
 const char* spkhip_version(void);

@@ -176,7 +176,13 @@ struct spkhip_handle {
     int level, grid;
   };
   int cur_level = -1, cur_grid = 0;   // tags of the next profiled launches
+  // plan knobs (set by spkhip_set_tuning / spkhip_autotune; any value in
+  // range gives a correct factor, they trade launches against occupancy)
   size_t upd_group = 8;                // blocks per full trailing update
+  int small_lmax = kSmallLmax;         // fronts with L <= this are factored whole in LDS
+  int block_cols = 128;                // widest block of pieces in a large LU front
+  int small_wave_min = 16384;          // small fronts per level from which one wave takes one
+  double tuned_ms = 0;                 // best factor time of the last autotune
   int lowp = 0;  // large-front updates: 1 bf16, 2 scaled fp16, 3 split-bf16 operands
   int mode = 0;                        // precision mode of the last factor
   std::vector<Rec> recs;
@@ -347,8 +353,12 @@ void build_fronts(spkhip_handle* h) {
       }
     }
   }
-  // storage: small fronts keep only their update matrix, large fronts the
-  // whole L x L work matrix (update matrix = its trailing block)
+}
+
+// Storage: small fronts keep only their update matrix, large fronts the
+// whole L x L work matrix (update matrix = its trailing block).
+void build_storage(spkhip_handle* h) {
+  const i32 nf = h->nfront;
   h->foff.assign(nf, 0);
   h->fcb.assign(nf, 0);
   h->fcbld.assign(nf, 0);
@@ -356,7 +366,7 @@ void build_fronts(spkhip_handle* h) {
   i64 off = 0, voff = 0;
   for (i32 f = 0; f < nf; ++f) {
     i64 L = h->fL[f], W = h->fW[f], R = L - W;
-    if (L <= kSmallLmax) {
+    if (L <= h->small_lmax) {
       h->foff[f] = off;
       h->fcb[f] = off;
       h->fcbld[f] = (i32)std::max<i64>(R, 1);
@@ -400,7 +410,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
     std::vector<i32> large;
     for (i32 f : bylev[l]) {
       solve.push_back(f);
-      if (h->fL[f] <= kSmallLmax) {
+      if (h->fL[f] <= h->small_lmax) {
         small.push_back(f);
         lp.small_maxL = std::max(lp.small_maxL, h->fL[f]);
       } else {
@@ -412,7 +422,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
     lp.large_off = (int)largel.size();
     for (i32 f : bylev[l]) {
       lp.max_w = std::max(lp.max_w, h->fW[f]);
-      if (h->fL[f] > kSmallLmax) largel.push_back(f);
+      if (h->fL[f] > h->small_lmax) largel.push_back(f);
     }
     lp.large_cnt = (int)largel.size() - lp.large_off;
     lp.ut_off = (int)ut_front.size();
@@ -443,7 +453,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         cbin += rc * rc;
       }
       lp.solve_elems += panel;
-      if (h->fL[f] <= kSmallLmax) {
+      if (h->fL[f] <= h->small_lmax) {
         lp.small_flops += fl;
         lp.small_elems += 2 * panel + R * R + cbin;
       } else {
@@ -491,7 +501,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
     size_t maxg = 0;
     for (i32 f : large) {
       std::vector<Blk> bl;
-      const i32 gw = (h->kind == SPKHIP_KIND_LU) ? kGW : 0;
+      const i32 gw = (h->kind == SPKHIP_KIND_LU) ? h->block_cols : 0;
       for (i32 p = h->fsp0[f]; p < h->fsp0[f] + h->fnp[f]; ++p) {
         const i32 cq = (i32)(h->xsuper[p] - 1 - h->fc0[f]);
         const i32 nq = (i32)(h->xsuper[p + 1] - h->xsuper[p]);
@@ -767,7 +777,7 @@ void run_factor(spkhip_handle* h) {
       // workgroup per front
       const size_t per = (size_t)lp.small_maxL * lp.small_maxL * sizeof(T);
       Prof pr(h, SPKHIP_K_SMALL, lp.small_flops, lp.small_elems * sizeof(T));
-      if (lp.small_cnt >= 16384) {
+      if (lp.small_cnt >= h->small_wave_min) {
         const int fpw = (int)std::max<size_t>(1, std::min<size_t>(4, 150 * 1024 / per));
         hipLaunchKernelGGL((spk::small_front_kernel<T, 64>), dim3((lp.small_cnt + fpw - 1) / fpw),
                            dim3(kWG), fpw * per, st, h->ds, h->d_small.p + lp.small_off,
@@ -1054,6 +1064,69 @@ void solve_device(spkhip_handle* h, double* d_b, i64 ldb, int nrhs) {
   hipck(hipGetLastError());
 }
 
+// Storage, launch plan and their device copies: everything that depends on
+// the plan knobs (small_lmax, block_cols, upd_group). Rebuilt when they
+// change; the factor must be recomputed afterwards.
+void replan(spkhip_handle* h) {
+  hipStream_t st = h->stream;
+  build_storage(h);
+  h->d_foff.upload(h->foff, st);
+  h->d_fcb.upload(h->fcb, st);
+  h->d_fcbld.upload(h->fcbld, st);
+  h->d_fvec.upload(h->fvec, st);
+  std::vector<i32> small, it_front, it_col, ea_child, ea_t, solve, largel, ut_front, ut_piece,
+      ut_r0;
+  std::vector<spk::SolveDesc> sdesc;
+  std::vector<i64> it_ea;
+  std::vector<BlkDesc> bdesc;
+  std::vector<spk::GemmDesc> gdesc;
+  build_plan(h, small, it_front, it_col, it_ea, ea_child, ea_t, bdesc, gdesc, solve, largel, sdesc,
+             ut_front, ut_piece, ut_r0);
+  if (largel.empty()) largel.push_back(0);
+  if (sdesc.empty()) sdesc.push_back(spk::SolveDesc{});
+  if (small.empty()) small.push_back(0);
+  h->d_small.upload(small, st);
+  if (it_front.empty()) {
+    it_front.push_back(0);
+    it_col.push_back(0);
+  }
+  h->d_it_front.upload(it_front, st);
+  h->d_it_col.upload(it_col, st);
+  h->d_it_ea.upload(it_ea, st);
+  if (ea_child.empty()) {
+    ea_child.push_back(0);
+    ea_t.push_back(0);
+  }
+  h->d_ea_child.upload(ea_child, st);
+  h->d_ea_t.upload(ea_t, st);
+  if (bdesc.empty()) bdesc.push_back(BlkDesc{});
+  if (gdesc.empty()) gdesc.push_back(spk::GemmDesc{});
+  h->d_bdesc.upload(bdesc, st);
+  h->d_gdesc.upload(gdesc, st);
+  if (solve.empty()) solve.push_back(0);
+  h->d_solve.upload(solve, st);
+  h->d_large.upload(largel, st);
+  if (ut_front.empty()) {
+    ut_front.push_back(0);
+    ut_piece.push_back(0);
+    ut_r0.push_back(0);
+  }
+  h->d_ut_front.upload(ut_front, st);
+  h->d_ut_piece.upload(ut_piece, st);
+  h->d_ut_r0.upload(ut_r0, st);
+  h->d_sdesc.upload(sdesc, st);
+  if (!h->pdesc.empty()) h->d_pdesc.upload(h->pdesc, st);
+  DevSym& d = h->ds;
+  d.foff = h->d_foff.p;
+  d.fcb = h->d_fcb.p;
+  d.fcbld = h->d_fcbld.p;
+  d.fvec = h->d_fvec.p;
+  sync(h);
+  h->sym_gen += 1;
+  h->factored = false;
+  h->prepped = false;
+}
+
 void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i64* xsuper,
                      const i64* snode, const i64* xlindx, const i64* lindx, const i64* xlnz,
                      const i64* xunz, const i64* perm, const i64* invp) {
@@ -1118,17 +1191,6 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   // tuning knob: SPKHIP_UPD_GROUP=G takes G blocks per full trailing update
   if (const char* e = std::getenv("SPKHIP_UPD_GROUP")) h->upd_group = std::max(1, std::atoi(e));
   build_fronts(h);
-  std::vector<i32> small, it_front, it_col, ea_child, ea_t, solve, largel, ut_front, ut_piece,
-      ut_r0;
-  std::vector<spk::SolveDesc> sdesc;
-  std::vector<i64> it_ea;
-  std::vector<BlkDesc> bdesc;
-  std::vector<spk::GemmDesc> gdesc;
-  build_plan(h, small, it_front, it_col, it_ea, ea_child, ea_t, bdesc, gdesc, solve, largel, sdesc,
-             ut_front, ut_piece, ut_r0);
-
-  if (largel.empty()) largel.push_back(0);
-  if (sdesc.empty()) sdesc.push_back(spk::SolveDesc{});
   hipStream_t st = h->stream;
   h->d_xsuper.upload(h->xsuper, st);
   h->d_snode.upload(h->snode, st);
@@ -1143,15 +1205,11 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   h->d_fc0.upload(h->fc0, st);
   h->d_fsp0.upload(h->fsp0, st);
   h->d_fnp.upload(h->fnp, st);
-  h->d_foff.upload(h->foff, st);
-  h->d_fcb.upload(h->fcb, st);
-  h->d_fcbld.upload(h->fcbld, st);
   h->d_frel.upload(h->frel, st);
   h->d_relmap.upload(h->relmap, st);
   h->d_fchp.upload(h->fchp, st);
   if (h->fch.empty()) h->fch.push_back(0);
   h->d_fch.upload(h->fch, st);
-  h->d_fvec.upload(h->fvec, st);
   {  // per-column base offsets of the reference layout in front coordinates
     std::vector<i64> lcol(n), ucol(n);
     for (i32 f = 0; f < h->nfront; ++f) {
@@ -1168,38 +1226,6 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
     h->d_lcol.upload(lcol, st);
     h->d_ucol.upload(ucol, st);
   }
-  if (small.empty()) small.push_back(0);
-  h->d_small.upload(small, st);
-  if (it_front.empty()) {
-    it_front.push_back(0);
-    it_col.push_back(0);
-  }
-  h->d_it_front.upload(it_front, st);
-  h->d_it_col.upload(it_col, st);
-  h->d_it_ea.upload(it_ea, st);
-  if (ea_child.empty()) {
-    ea_child.push_back(0);
-    ea_t.push_back(0);
-  }
-  h->d_ea_child.upload(ea_child, st);
-  h->d_ea_t.upload(ea_t, st);
-  if (bdesc.empty()) bdesc.push_back(BlkDesc{});
-  if (gdesc.empty()) gdesc.push_back(spk::GemmDesc{});
-  h->d_bdesc.upload(bdesc, st);
-  h->d_gdesc.upload(gdesc, st);
-  if (solve.empty()) solve.push_back(0);
-  h->d_solve.upload(solve, st);
-  h->d_large.upload(largel, st);
-  if (ut_front.empty()) {
-    ut_front.push_back(0);
-    ut_piece.push_back(0);
-    ut_r0.push_back(0);
-  }
-  h->d_ut_front.upload(ut_front, st);
-  h->d_ut_piece.upload(ut_piece, st);
-  h->d_ut_r0.upload(ut_r0, st);
-  h->d_sdesc.upload(sdesc, st);
-  if (!h->pdesc.empty()) h->d_pdesc.upload(h->pdesc, st);
   DevSym& d = h->ds;
   d.n = n;
   d.nsuper = nsuper;
@@ -1215,24 +1241,18 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   d.fc0 = h->d_fc0.p;
   d.fsp0 = h->d_fsp0.p;
   d.fnp = h->d_fnp.p;
-  d.foff = h->d_foff.p;
-  d.fcb = h->d_fcb.p;
-  d.fcbld = h->d_fcbld.p;
   d.frel = h->d_frel.p;
   d.relmap = h->d_relmap.p;
   d.fchp = h->d_fchp.p;
   d.fch = h->d_fch.p;
-  d.fvec = h->d_fvec.p;
   d.lcol = h->d_lcol.p;
   d.ucol = h->d_ucol.p;
   d.pivot = (kind == SPKHIP_KIND_LU) ? 1 : 0;
   d.sym = (kind == SPKHIP_KIND_SPD) ? 1 : 0;
-  sync(h);
+  replan(h);
   h->have_sym = true;
-  h->sym_gen += 1;
   h->have_csc = false;
   h->have_values = false;
-  h->factored = false;
   h->t_upload = ms_since(t0);
 }
 
@@ -1508,6 +1528,36 @@ int64_t inmatrix_common(spkhip_t h) {
   h->have_values = true;
   h->factored = false;
   return 0;
+}
+// plan knobs <-> spkhip_tuning_t
+spkhip_tuning_t current_tuning(const spkhip_handle* h) {
+  spkhip_tuning_t t;
+  t.small_lmax = h->small_lmax;
+  t.block_cols = h->block_cols;
+  t.upd_group = (int64_t)h->upd_group;
+  t.small_wave_min = h->small_wave_min;
+  t.factor_ms = h->tuned_ms;
+  return t;
+}
+
+bool tuning_valid(const spkhip_tuning_t& t) {
+  return t.small_lmax >= 0 && t.small_lmax <= kSmallLmax && t.block_cols >= 32 &&
+         t.block_cols <= kGW && t.upd_group >= 1 && t.upd_group <= 64 && t.small_wave_min >= 1 &&
+         t.small_wave_min <= (int64_t)1 << 30;
+}
+
+// Sets the knobs; a change rebuilds storage and plan (the factor is then
+// stale until the next spkhip_factor).
+void apply_tuning(spkhip_handle* h, const spkhip_tuning_t& t) {
+  const bool same = t.small_lmax == h->small_lmax && t.block_cols == h->block_cols &&
+                    (size_t)t.upd_group == h->upd_group &&
+                    t.small_wave_min == h->small_wave_min;
+  if (same) return;
+  h->small_lmax = (int)t.small_lmax;
+  h->block_cols = (int)t.block_cols;
+  h->upd_group = (size_t)t.upd_group;
+  h->small_wave_min = (int)t.small_wave_min;
+  if (h->have_sym) replan(h);
 }
 }  // namespace
 
@@ -1835,6 +1885,89 @@ int64_t spkhip_stats(spkhip_t h, spkhip_stats_t* s) {
   s->t_refine_ms = h->t_refine;
   s->backward_error = h->backward_error;
   return 0;
+}
+
+int64_t spkhip_get_tuning(spkhip_t h, spkhip_tuning_t* out) {
+  if (guard(h) || !out) return -4;
+  *out = current_tuning(h);
+  return 0;
+}
+
+int64_t spkhip_set_tuning(spkhip_t h, const spkhip_tuning_t* t) {
+  if (guard(h) || !t || !tuning_valid(*t)) return -4;
+  return wrap([&]() -> int64_t {
+    apply_tuning(h, *t);
+    return 0;
+  });
+}
+
+int64_t spkhip_autotune(spkhip_t h, int64_t precision, int64_t reps, spkhip_tuning_t* out) {
+  if (guard(h)) return -4;
+  if (!h->have_values) return -3;
+  if (reps < 1 || reps > 10) return -4;
+  const spkhip_tuning_t orig = current_tuning(h);
+  int64_t rc = wrap([&]() -> int64_t {
+    // one untimed factor per plan (first-touch of the arena, kernel
+    // attributes), then the fastest of reps timed ones
+    auto measure = [&](const spkhip_tuning_t& t, double* ms) -> int64_t {
+      apply_tuning(h, t);
+      int64_t r = spkhip_factor(h, precision);
+      if (r) return r;
+      double m = 1e300;
+      for (int64_t i = 0; i < reps; ++i) {
+        r = spkhip_factor(h, precision);
+        if (r) return r;
+        m = std::min(m, h->t_factor);
+      }
+      *ms = m;
+      return 0;
+    };
+    spkhip_tuning_t best = orig;
+    double best_ms = 0;
+    int64_t r = measure(best, &best_ms);
+    if (r) return r;
+    // coordinate search from the current setting; a change is kept only
+    // when it is faster by more than 1% (timing noise)
+    const std::vector<int64_t> cand[4] = {{64, 48, 32},
+                                          {128, 96, 64},
+                                          {8, 4, 16, 2},
+                                          {16384, 4096, 65536}};
+    for (int knob = 0; knob < 4; ++knob) {
+      if (knob == 1 && h->kind != SPKHIP_KIND_LU) continue;  // SPD blocks are one piece
+      for (int64_t v : cand[knob]) {
+        spkhip_tuning_t t = best;
+        int64_t* field = knob == 0   ? &t.small_lmax
+                         : knob == 1 ? &t.block_cols
+                         : knob == 2 ? &t.upd_group
+                                     : &t.small_wave_min;
+        if (*field == v) continue;
+        *field = v;
+        double ms = 0;
+        r = measure(t, &ms);
+        if (r) return r;
+        if (ms < 0.99 * best_ms) {
+          best = t;
+          best_ms = ms;
+        }
+      }
+    }
+    apply_tuning(h, best);
+    r = spkhip_factor(h, precision);
+    if (r) return r;
+    h->tuned_ms = best_ms;
+    if (out) {
+      *out = best;
+      out->factor_ms = best_ms;
+    }
+    return 0;
+  });
+  if (rc) {  // a failed search leaves the knobs as they were (unfactored)
+    (void)wrap([&]() -> int64_t {
+      apply_tuning(h, orig);
+      return 0;
+    });
+  }
+  return rc;
 }
 
 int64_t spkhip_lufactor_f64(int64_t n, int64_t nsuper, const int64_t* xsuper,

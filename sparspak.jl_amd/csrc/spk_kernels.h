@@ -25,7 +25,7 @@
 namespace spk {
 
 constexpr int kWG = 256;      // threads per workgroup (4 waves)
-constexpr int kSmallLmax = 64; // fronts with L <= this are factored in LDS
+constexpr int kSmallLmax = 64; // largest front the LDS small-front kernels take
 
 // Device view of the symbolic factor and the front schedule.
 struct DevSym {

@@ -120,6 +120,34 @@ factor!(h::Handle; precision=SPKHIP_FP64) = check(
     ccall((:spkhip_factor, libspk), Int64, (Ptr{Cvoid}, Int64), h.ptr, precision),
     "An empty problem. No matrix.")   # the message _factor! raises (SpkSparseBase.jl:387)
 
+# plan knobs and the autotuner (spkhip_tuning_t; no counterpart in the reference)
+mutable struct Tuning
+    small_lmax::Int64
+    block_cols::Int64
+    upd_group::Int64
+    small_wave_min::Int64
+    factor_ms::Float64
+end
+Tuning() = Tuning(0, 0, 0, 0, 0.0)
+
+function tuning(h::Handle)
+    t = Ref(Tuning())
+    check(ccall((:spkhip_get_tuning, libspk), Int64, (Ptr{Cvoid}, Ref{Tuning}), h.ptr, t),
+          "spkhip_get_tuning")
+    return t[]
+end
+
+settuning!(h::Handle, t::Tuning) = check(
+    ccall((:spkhip_set_tuning, libspk), Int64, (Ptr{Cvoid}, Ref{Tuning}), h.ptr, Ref(t)),
+    "spkhip_set_tuning")
+
+function autotune!(h::Handle; precision=SPKHIP_FP32, reps=2)
+    t = Ref(Tuning())
+    check(ccall((:spkhip_autotune, libspk), Int64, (Ptr{Cvoid}, Int64, Int64, Ref{Tuning}),
+                h.ptr, precision, reps, t), "spkhip_autotune")
+    return t[]
+end
+
 function solve!(h::Handle, B::StridedVecOrMat{Float64}; refine=false, tol=1e-14, maxiter=10)
     nrhs = size(B, 2)
     ldb = stride(B, 2)

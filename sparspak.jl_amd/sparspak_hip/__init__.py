@@ -4,9 +4,11 @@ Public surface (mirrors the reference's API, Julia bangs dropped):
     Problem, SparseSolver, SparseSpdSolver,
     findorder, findorderperm, symbolicfactor, inmatrix, factor,
     triangularsolve, solve, sparspaklu, sparspaklu_refactor, ldiv, backslash
+plus autotune (plan knobs chosen by timing the factorisation; no counterpart
+in the reference).
 """
 from .problem import Problem
-from .solver import (FP32, FP64, SparseSolver, SparseSpdSolver, backslash,
+from .solver import (FP32, FP64, SparseSolver, SparseSpdSolver, autotune, backslash,
                      factor, findorder, findorderperm, inmatrix, ldiv, solve,
                      sparspaklu, sparspaklu_refactor, symbolicfactor,
                      triangularsolve)
@@ -16,5 +18,5 @@ __all__ = [
     "Problem", "SparseSolver", "SparseSpdSolver", "findorder", "findorderperm",
     "symbolicfactor", "inmatrix", "factor", "triangularsolve", "solve",
     "sparspaklu", "sparspaklu_refactor", "ldiv", "backslash", "Graph",
-    "Ordering", "SymbolicFactor", "FP32", "FP64",
+    "Ordering", "SymbolicFactor", "FP32", "FP64", "autotune",
 ]

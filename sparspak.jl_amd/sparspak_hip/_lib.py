@@ -38,6 +38,13 @@ class KStat(ctypes.Structure):
                 ("bytes", ctypes.c_double)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("small_lmax", i64), ("block_cols", i64), ("upd_group", i64),
+                ("small_wave_min", i64), ("factor_ms", ctypes.c_double)]
+
+
+TUNING_KNOBS = ("small_lmax", "block_cols", "upd_group", "small_wave_min")
+
 KCLASSES = ["assemble", "small_front", "large_init", "large_diag", "large_update",
             "large_writeback", "fwd_solve", "bwd_solve", "residual", "large_offdiag",
             "fwd_gemv", "bwd_gemv"]
@@ -93,6 +100,9 @@ _SIGS = {
     "spkhip_ldltsolve_f64": (i64, [i64, i64p, i64p, i64p, i64p, dp, dp]),
     "spkhip_profile_enable": (i64, [vp, i64]),
     "spkhip_profile_get": (i64, [vp, i64, ctypes.POINTER(KStat)]),
+    "spkhip_get_tuning": (i64, [vp, ctypes.POINTER(Tuning)]),
+    "spkhip_set_tuning": (i64, [vp, ctypes.POINTER(Tuning)]),
+    "spkhip_autotune": (i64, [vp, i64, i64, ctypes.POINTER(Tuning)]),
     "spkhip_device_count": (i64, []),
     "spkhip_version": (ctypes.c_char_p, []),
 }

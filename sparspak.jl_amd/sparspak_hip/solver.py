@@ -174,6 +174,34 @@ class _SparseBase:
             rhs[...] = B
         return True
 
+    def get_tuning(self):
+        self._upload()
+        t = _lib.Tuning()
+        check(self._dev.L.spkhip_get_tuning(self._dev.h, ctypes.byref(t)), "spkhip_get_tuning")
+        return {f: getattr(t, f) for f in _lib.TUNING_KNOBS + ("factor_ms",)}
+
+    def set_tuning(self, **knobs):
+        """Plan knobs (small_lmax, block_cols, upd_group, small_wave_min);
+        the factor must be recomputed afterwards."""
+        cur = self.get_tuning()
+        bad = set(knobs) - set(_lib.TUNING_KNOBS)
+        if bad:
+            raise ValueError(f"unknown tuning knobs {sorted(bad)}")
+        cur.update(knobs)
+        t = _lib.Tuning(*(int(cur[f]) for f in _lib.TUNING_KNOBS), 0.0)
+        check(self._dev.L.spkhip_set_tuning(self._dev.h, ctypes.byref(t)), "spkhip_set_tuning")
+
+    def _autotune(self, precision=FP32, reps=2):
+        t = _lib.Tuning()
+        rc = int(self._dev.L.spkhip_autotune(self._dev.h, _PREC[precision], int(reps),
+                                             ctypes.byref(t)))
+        self.errflag = rc
+        if rc == -1:
+            raise RuntimeError("Numerical factorization: zero pivot during autotuning.")
+        check(rc, "spkhip_autotune")
+        self.precision = _PREC[precision]
+        return {f: getattr(t, f) for f in _lib.TUNING_KNOBS + ("factor_ms",)}
+
     def stats(self):
         st = _lib.Stats()
         check(self._dev.L.spkhip_stats(self._dev.h, ctypes.byref(st)), "spkhip_stats")
@@ -327,6 +355,21 @@ def factor(s, precision=FP64):
         s._factordone = True
         return True
     return False
+
+
+def autotune(s, precision=FP32, reps=2):
+    """Choose the plan knobs (the small-versus-MFMA front split, the block
+    width, the update depth, the small-front wave threshold) by timing the
+    factorisation of the current values at `precision`; leaves `s`
+    factored with the fastest setting and returns it (with its factor_ms).
+    No counterpart in the reference."""
+    if not s._inmatrixdone:
+        raise RuntimeError("Sequence error. Matrix input not done yet.")
+    s._trisolvedone = False
+    best = s.slvr._autotune(precision, reps)
+    s.precision = _PREC[precision]
+    s._factordone = True
+    return best
 
 
 def triangularsolve(s, rhs=None, refine=None, tol=1e-14, max_iter=10):
