@@ -19,7 +19,9 @@ A step is: inmatrix (values already resident in HBM) -> factor -> solve of
 one right-hand side b = A x_true (x_true ~ U[-1, 1], splitmix64 seed 2) with
 the backward-error certificate (and refinement for fp32). The host symbolic
 phase (MMD, etree, supernodes) runs once before the timed region, as in
-sparspaklu! refactorisation. Prints one JSON line.
+sparspaklu! refactorisation, and so does the autotuner that picks each path's
+plan knobs (front split, block width, update depth) by timing its factor
+(--no-autotune keeps the defaults). Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--grid 64] [--cpu-grid 36]
 """
@@ -66,6 +68,8 @@ def main():
     ap.add_argument("--cpu-grid", type=int, default=36)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--paths", default="fp64,fp32,fp32_bf16,fp32_bf16x3")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default plan knobs instead of autotuning each path")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (three times), the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -132,6 +136,17 @@ def main():
 
     results = {}
     for path in args.paths.split(","):
+        tuning, t_tune = None, 0.0
+        if not args.no_autotune:
+            # plan knobs chosen by timing this path's factorisation (setup,
+            # like the symbolic phase; outside the timed steps)
+            t0 = time.perf_counter()
+            _lib.check(L.spkhip_inmatrix_device(h, ctypes.c_void_p(d_nz.data_ptr())), "inmatrix")
+            tu = _lib.Tuning()
+            _lib.check(L.spkhip_autotune(h, prec[path], 2, ctypes.byref(tu)), "autotune")
+            t_tune = time.perf_counter() - t0
+            tuning = {f: getattr(tu, f) for f in _lib.TUNING_KNOBS}
+            log(f"{path}: autotune {t_tune:.1f}s -> {tuning} ({tu.factor_ms:.1f} ms)")
         for _ in range(args.warmup):
             step(path)
         torch.cuda.synchronize()
@@ -167,6 +182,7 @@ def main():
             "factor_tflops": F / t_factor / 1e12,
             "refine_iters": int(st.refine_iters), "backward_error": eta,
             "forward_error_vs_xtrue": fwd, "kernels": kst,
+            "tuning": tuning, "autotune_s": t_tune,
         }
         log(f"{path}: factor {t_factor * 1e3:.1f} ms ({F / t_factor / 1e12:.2f} TF/s), "
             f"solve+ir {t_solve * 1e3:.1f} ms, iters {st.refine_iters}, eta {eta:.2e}, "
@@ -228,7 +244,7 @@ def main():
         "config": {"workload": f"3D upwind convection-diffusion 7-point {g}^3 LU "
                                "(SparseSolver numeric phase: inmatrix+factor+solve)",
                    "n": n, "nnz": int(A.nnz), "F_LU": F, "headline_path": head,
-                   "host_symbolic_s": t_sym},
+                   "host_symbolic_s": t_sym, "plan_knobs": R["tuning"] or "default"},
         "paths": {p: {k: v for k, v in results[p].items() if k != "kernels"} for p in results},
         "kernels": {p: results[p]["kernels"] for p in results},
         "roofline": roof,
