@@ -70,6 +70,9 @@ def main():
     ap.add_argument("--paths", default="fp64,fp32,fp32_bf16,fp32_bf16x3")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default plan knobs instead of autotuning each path")
+    ap.add_argument("--tuning", default=None,
+                    help="small_lmax,block_cols,upd_group,small_wave_min for every path "
+                         "instead of autotuning (e.g. to profile the plan a bench run chose)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (three times), the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -137,7 +140,12 @@ def main():
     results = {}
     for path in args.paths.split(","):
         tuning, t_tune = None, 0.0
-        if not args.no_autotune:
+        if args.tuning:
+            vals = [int(v) for v in args.tuning.split(",")]
+            tu = _lib.Tuning(*vals, 0.0)
+            _lib.check(L.spkhip_set_tuning(h, ctypes.byref(tu)), "set_tuning")
+            tuning = dict(zip(_lib.TUNING_KNOBS, vals))
+        elif not args.no_autotune:
             # plan knobs chosen by timing this path's factorisation (setup,
             # like the symbolic phase; outside the timed steps)
             t0 = time.perf_counter()

@@ -87,6 +87,12 @@ struct LevelPlan {
   };
   struct Group {  // one block step of the level's large fronts
     Launch diag, offd, gemm;
+    // gemm2: the part of a group's wide update outside the next group's
+    // rows and columns, run on the second stream beside the next group's
+    // panel work; w1 marks a step whose gemm holds the part inside them
+    // (which must wait for the previous gemm2)
+    Launch gemm2;
+    bool w1 = false;
   };
   double small_flops = 0, small_elems = 0, init_elems = 0, wb_elems = 0, solve_elems = 0;
   int max_w = 0;                    // widest front of the level
@@ -106,6 +112,11 @@ struct LevelPlan {
 struct spkhip_handle {
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream (default priority) for the wide updates that overlap the
+  // next block group's panel work; joined through xev events
+  hipStream_t stream2 = nullptr;
+  std::vector<hipEvent_t> xev;
+  size_t xev_used = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // symbolic
   bool have_sym = false;
@@ -194,6 +205,7 @@ struct spkhip_handle {
   std::vector<const void*> g_seen;  // last key run directly (captured on its second use)
   ~spkhip_handle() {
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : xev) (void)hipEventDestroy(e);
     if (g_exec) (void)hipGraphExecDestroy(g_exec);
   }
 };
@@ -234,6 +246,19 @@ struct Prof {
     }
   }
 };
+
+// `to` waits for everything issued to `from` so far (a fresh event each
+// time, so that no wait can see a later record of the same event).
+void stream_join(spkhip_handle* h, hipStream_t from, hipStream_t to) {
+  if (h->xev_used == h->xev.size()) {
+    hipEvent_t e = nullptr;
+    hipck(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    h->xev.push_back(e);
+  }
+  hipEvent_t e = h->xev[h->xev_used++];
+  hipck(hipEventRecord(e, from));
+  hipck(hipStreamWaitEvent(to, e, 0));
+}
 
 void prof_collect(spkhip_handle* h) {
   if (!h->prof || h->recs.empty()) return;
@@ -497,11 +522,8 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
     struct Blk {
       i32 p0, np, c0, c1;
     };
-    std::vector<std::vector<Blk>> fblocks;
-    size_t maxg = 0;
-    for (i32 f : large) {
+    auto partition = [&](i32 f, i32 gw) {
       std::vector<Blk> bl;
-      const i32 gw = (h->kind == SPKHIP_KIND_LU) ? h->block_cols : 0;
       for (i32 p = h->fsp0[f]; p < h->fsp0[f] + h->fnp[f]; ++p) {
         const i32 cq = (i32)(h->xsuper[p] - 1 - h->fc0[f]);
         const i32 nq = (i32)(h->xsuper[p + 1] - h->xsuper[p]);
@@ -512,8 +534,19 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
           bl.push_back(Blk{p, 1, cq, cq + nq});
         }
       }
-      maxg = std::max(maxg, bl.size());
-      fblocks.push_back(bl);
+      return bl;
+    };
+    // the factor's blocks (block_cols wide) and the solve's (kGW wide: the
+    // solve data is built from lnz/unz, so its blocks need not be the
+    // factor's, and fewer blocks mean fewer dependent solve steps)
+    std::vector<std::vector<Blk>> fblocks, sblocks;
+    size_t maxg = 0, maxgs = 0;
+    const bool lub = h->kind == SPKHIP_KIND_LU;
+    for (i32 f : large) {
+      fblocks.push_back(partition(f, lub ? h->block_cols : 0));
+      sblocks.push_back(partition(f, lub ? kGW : 0));
+      maxg = std::max(maxg, fblocks.back().size());
+      maxgs = std::max(maxgs, sblocks.back().size());
     }
     for (size_t gi = 0; gi < maxg; ++gi) {
       LevelPlan::Group G;
@@ -557,15 +590,18 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
       // first of a pair -> border of the second; second -> the rest with
       // both blocks; an unpaired last block -> the rest with itself
       G.gemm.off = (int)gdesc.size();
-      auto rect = [&](i32 f, i32 k0, i32 k1, i32 r0, i32 r1, i32 j0, i32 j1) {
+      std::vector<spk::GemmDesc> g2desc;
+      auto rect = [&](i32 f, i32 k0, i32 k1, i32 r0, i32 r1, i32 j0, i32 j1, bool second) {
         if (r1 <= r0 || j1 <= j0 || k1 <= k0) return;
+        LevelPlan::Launch& Lc = second ? G.gemm2 : G.gemm;
         const i32 ntr = (r1 - r0 + 127) / 128, ntc = (j1 - j0 + 127) / 128;
-        gdesc.push_back(spk::GemmDesc{f, k0, k1, r0, r1 - r0, j0, j1 - j0, G.gemm.wg, ntr,
-                                      h->fW[f] > 256 ? 1 : 0});
-        G.gemm.wg += ntr * ntc;
+        (second ? g2desc : gdesc)
+            .push_back(spk::GemmDesc{f, k0, k1, r0, r1 - r0, j0, j1 - j0, Lc.wg, ntr,
+                                     h->fW[f] > 256 ? 1 : 0});
+        Lc.wg += ntr * ntc;
         const double m = r1 - r0, nn = j1 - j0, kk = k1 - k0;
-        G.gemm.flops += 2.0 * m * nn * kk;
-        G.gemm.elems += 2.0 * m * nn + (m + nn) * kk;
+        Lc.flops += 2.0 * m * nn * kk;
+        Lc.elems += 2.0 * m * nn + (m + nn) * kk;
       };
       for (size_t fi = 0; fi < large.size(); ++fi) {
         if (gi >= fblocks[fi].size()) continue;
@@ -577,14 +613,26 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         if (gi + 1 < ge) {
           // border of the group's remaining blocks [B.c1, E) x ..., E = their end
           const i32 E = fblocks[fi][ge - 1].c1;
-          rect(f, B.c0, B.c1, B.c1, L, B.c1, E);
-          rect(f, B.c0, B.c1, B.c1, E, E, L);
+          rect(f, B.c0, B.c1, B.c1, L, B.c1, E, false);
+          rect(f, B.c0, B.c1, B.c1, E, E, L, false);
         } else {
-          // last block of the group: the rest with the whole group
-          rect(f, fblocks[fi][gs].c0, B.c1, B.c1, L, B.c1, L);
+          // last block of the group: the rest with the whole group, split at
+          // E2 = the end of the next group: the next group's rows and
+          // columns [E, E2) now, [E2, L)^2 on the second stream (no later
+          // step before the next group's own wide update touches it)
+          const i32 E = B.c1, k0 = fblocks[fi][gs].c0;
+          const size_t nb = fblocks[fi].size();
+          const i32 E2 = ge < nb ? fblocks[fi][std::min(ge + h->upd_group, nb) - 1].c1 : E;
+          rect(f, k0, E, E, L, E, E2, false);
+          rect(f, k0, E, E, E2, E2, L, false);
+          rect(f, k0, E, E2, L, E2, L, true);
+          if (E2 > E) G.w1 = true;
         }
       }
       G.gemm.cnt = (int)gdesc.size() - G.gemm.off;
+      G.gemm2.off = (int)gdesc.size();
+      for (spk::GemmDesc& D : g2desc) gdesc.push_back(D);
+      G.gemm2.cnt = (int)g2desc.size();
       lp.groups.push_back(G);
     }
     // blocked solve steps over the same blocks; each block's solve data
@@ -597,23 +645,23 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
     };
     std::vector<std::vector<spk::SolveDesc>> bdata(large.size());
     for (size_t fi = 0; fi < large.size(); ++fi)
-      for (const Blk& B : fblocks[fi]) {
+      for (const Blk& B : sblocks[fi]) {
         const spk::SolveDesc D = sd(large[fi], B, (i32)h->pdesc.size(), (i32)h->prep_units);
         h->pdesc.push_back(D);
         bdata[fi].push_back(D);
         const i64 ng = (B.c1 - B.c0 + 3) / 4;
         h->prep_units += 4 * ng * (ng + 1) + ng;
       }
-    for (size_t gi = 0; gi < maxg; ++gi) {
+    for (size_t gi = 0; gi < maxgs; ++gi) {
       LevelPlan::SolveStep S;
       S.chain_off = (int)sdesc.size();
       for (size_t fi = 0; fi < large.size(); ++fi)
-        if (gi < fblocks[fi].size()) sdesc.push_back(bdata[fi][gi]);
+        if (gi < sblocks[fi].size()) sdesc.push_back(bdata[fi][gi]);
       S.chain_cnt = (int)sdesc.size() - S.chain_off;
       S.gemv_off = (int)sdesc.size();
       for (size_t fi = 0; fi < large.size(); ++fi)
-        if (gi < fblocks[fi].size()) {
-          const Blk& B = fblocks[fi][gi];
+        if (gi < sblocks[fi].size()) {
+          const Blk& B = sblocks[fi][gi];
           const i32 t = h->fL[large[fi]] - B.c1;
           for (i32 r0 = 0; r0 < t; r0 += 64)
             sdesc.push_back(sd(large[fi], B, r0, 0));
@@ -622,13 +670,13 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
       S.gemv_cnt = (int)sdesc.size() - S.gemv_off;
       lp.fsteps.push_back(S);
     }
-    for (size_t si = 0; si < maxg; ++si) {
+    for (size_t si = 0; si < maxgs; ++si) {
       LevelPlan::SolveStep S;
       S.gemv_off = (int)sdesc.size();
       for (size_t fi = 0; fi < large.size(); ++fi) {
-        const i64 bi = (i64)fblocks[fi].size() - 1 - (i64)si;
+        const i64 bi = (i64)sblocks[fi].size() - 1 - (i64)si;
         if (bi < 0) continue;
-        const Blk& B = fblocks[fi][bi];
+        const Blk& B = sblocks[fi][bi];
         if (B.c1 >= h->fL[large[fi]]) continue;
         for (i32 r0 = 0; r0 < B.c1 - B.c0; ++r0) {
           spk::SolveDesc D = sd(large[fi], B, r0, 0);
@@ -643,9 +691,9 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
       S.gemv_cnt = (int)sdesc.size() - S.gemv_off;
       S.chain_off = (int)sdesc.size();
       for (size_t fi = 0; fi < large.size(); ++fi) {
-        const i64 bi = (i64)fblocks[fi].size() - 1 - (i64)si;
+        const i64 bi = (i64)sblocks[fi].size() - 1 - (i64)si;
         if (bi < 0) continue;
-        const Blk& B = fblocks[fi][bi];
+        const Blk& B = sblocks[fi][bi];
         sdesc.push_back(bdata[fi][bi]);
         S.elems += (double)(B.c1 - B.c0) * (h->fL[large[fi]] - B.c0);
       }
@@ -768,6 +816,11 @@ void run_factor(spkhip_handle* h) {
                      h->d_fail.p, (int64_t)1, ~0ull);
   set_lds_attrs<T>();
   const bool lu = h->kind == SPKHIP_KIND_LU;
+  // wide updates overlap the next group's panel work on the second stream
+  // (not under the per-launch profile, whose events are on the first)
+  const bool two = !h->prof;
+  hipStream_t st2 = two ? h->stream2 : st;
+  h->xev_used = 0;
   for (const LevelPlan& lp : h->levels) {
     h->cur_level = (int)(&lp - h->levels.data());
     h->cur_grid = lp.small_cnt;
@@ -806,6 +859,7 @@ void run_factor(spkhip_handle* h) {
                          h->d_ea_child.p, h->d_ea_t.p, lnz.p, arena.p, (int)staged);
       hipck(hipGetLastError());
       }
+      bool pending = false;  // second-stream work not yet joined
       for (const LevelPlan::Group& G : lp.groups) {
         {
           h->cur_grid = G.diag.wg;
@@ -833,6 +887,19 @@ void run_factor(spkhip_handle* h) {
                                h->d_bdesc.p + G.offd.off, G.offd.cnt, arena.p, h->d_ipvt.p);
           hipck(hipGetLastError());
         }
+        // the part of a wide update inside the next group's rows and
+        // columns overlaps the previous group's outside part
+        if (two && pending && G.w1) stream_join(h, st2, st);
+        if (G.gemm2.cnt > 0) {
+          // the group's panels are final here: the outside part starts
+          if (two) stream_join(h, st, st2);
+          h->cur_grid = G.gemm2.wg;
+          Prof pr(h, SPKHIP_K_UPDATE, G.gemm2.flops, G.gemm2.elems * sizeof(T));
+          hipLaunchKernelGGL(spk::blk_gemm_kernel<T>, dim3(G.gemm2.wg), dim3(kWG), 0, st2, h->ds,
+                             h->d_gdesc.p + G.gemm2.off, G.gemm2.cnt, arena.p, h->lowp);
+          hipck(hipGetLastError());
+          pending = two;
+        }
         if (G.gemm.cnt > 0) {
           h->cur_grid = G.gemm.wg;
           Prof pr(h, SPKHIP_K_UPDATE, G.gemm.flops, G.gemm.elems * sizeof(T));
@@ -841,6 +908,7 @@ void run_factor(spkhip_handle* h) {
           hipck(hipGetLastError());
         }
       }
+      if (pending) stream_join(h, st2, st);
       Prof pr(h, SPKHIP_K_WRITEBACK, 0.0, lp.wb_elems * sizeof(T));
       if (lp.ut_cnt > 0)
         hipLaunchKernelGGL((spk::large_utile_kernel<T, true>), dim3(lp.ut_cnt), dim3(kWG), 0, st,
@@ -1435,6 +1503,7 @@ int64_t spkhip_create(spkhip_t* out, int64_t device) {
     try {
       hipck(hipSetDevice(h->device));
       hipck(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+      hipck(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
       hipck(hipEventCreate(&h->ev0));
       hipck(hipEventCreate(&h->ev1));
     } catch (...) {
@@ -1450,14 +1519,16 @@ int64_t spkhip_destroy(spkhip_t h) {
   if (!h) return 0;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->stream2) (void)hipStreamSynchronize(h->stream2);
   int dev = h->device;
-  hipStream_t st = h->stream;
+  hipStream_t st = h->stream, st2 = h->stream2;
   hipEvent_t e0 = h->ev0, e1 = h->ev1;
   delete h;  // frees device buffers
   (void)dev;
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (st) (void)hipStreamDestroy(st);
+  if (st2) (void)hipStreamDestroy(st2);
   return 0;
 }
 
