@@ -607,6 +607,43 @@ constexpr int kRLD = 129;
 // Most columns of one piece (the SPD limit 90, rounded up).
 constexpr int kMaxNq = 96;
 
+// One thread's triangular solve against a piece's diagonal block, in place
+// in LDS: X[j XS] (j < nq) becomes
+//   x_j = (a_j - sum_{k<j} x_k M(k, j)) * r_j,   M(k, j) = M[k MK + j MJ],
+// r_j = 1 / M(j, j) (or rd[j] when given; 1 when UNIT). Columns go in
+// blocks of eight: the earlier columns' contributions in a runtime loop over
+// k with eight independent sums, then the block's own triangle in
+// registers. Each x_j receives the same operations in the same order as the
+// plain dot form (k ascending, then the scaling), so the result is the
+// same, while the code stays a few hundred bytes (a fully unrolled solve is
+// fetched from L2 once per piece and runs at instruction-fetch speed).
+template <typename T, int XS, int MK, int MJ, bool UNIT>
+__device__ inline void blk_trsv(T* X, const T* M, const T* rd, int nq) {
+  for (int jb = 0; jb < nq; jb += 8) {
+    int jc[8];  // columns of the block, clamped inside the piece
+#pragma unroll
+    for (int c = 0; c < 8; ++c) jc[c] = min(jb + c, nq - 1);
+    T xb[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) xb[c] = X[jc[c] * XS];
+#pragma unroll 2
+    for (int kk = 0; kk < jb; ++kk) {
+      const T xk = X[kk * XS];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) xb[c] -= xk * M[kk * MK + jc[c] * MJ];
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+#pragma unroll
+      for (int cp = 0; cp < c; ++cp) xb[c] -= xb[cp] * M[jc[cp] * MK + jc[c] * MJ];
+      if (!UNIT) xb[c] = xb[c] * (rd ? rd[jc[c]] : T(1) / M[jc[c] * MK + jc[c] * MJ]);
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (jb + c < nq) X[(jb + c) * XS] = xb[c];
+  }
+}
+
 // Pivoted LU of the nq x nq block D (LDS, leading dimension LD, nq <= 64) by
 // one wave, with rows that never move during the elimination: lane i owns
 // LDS row i of the block and tracks the position pos its row holds after the
@@ -941,49 +978,14 @@ blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ aren
     if (tid == 0 && s_zero) atomicMin(first_fail, (unsigned long long)(d.piece + pi + 1));
     if (tid >= 128) {
       // U12 of the region (rows already interchanged): unit-lower solve,
-      // u_i = a_i - sum_k<i L(i, k) u_k; for pieces of up to 48 columns the
-      // thread's column is held in registers (loops unrolled to NQ, cut at
-      // nq by uniform branches)
-      for (int c = tid - 128; c < rest; c += 128) {
-        T* uc = D + (nq + c) * LD;
-        if constexpr (NQ <= 48) {
-          T u[NQ];
-#pragma unroll
-          for (int i = 0; i < NQ; ++i) u[i] = (i < nq) ? uc[i] : T(0);
-#pragma unroll
-          for (int i = 1; i < NQ; ++i) {
-            if (i < nq) {
-              T v = u[i];
-#pragma unroll
-              for (int kk = 0; kk < i; ++kk) v -= D[i + kk * LD] * u[kk];
-              u[i] = v;
-            }
-          }
-#pragma unroll
-          for (int i = 1; i < NQ; ++i)
-            if (i < nq) uc[i] = u[i];
-        } else {
-          for (int i = 1; i < nq; ++i) {
-            T v = uc[i];
-            const T* li = D + i;
-#pragma unroll 8
-            for (int kk = 0; kk < i; ++kk) v -= li[kk * LD] * uc[kk];
-            uc[i] = v;
-          }
-        }
-      }
+      // u_i = a_i - sum_k<i L(i, k) u_k, one column per thread
+      for (int c = tid - 128; c < rest; c += 128)
+        blk_trsv<T, 1, LD, 1, true>(D + (nq + c) * LD, D, nullptr, nq);
     } else {
-      // L21 of the region: x_j = (a_j - sum_k<j x_k U(k, j)) / U(j, j)
-      for (int r = tid; r < rest; r += 128) {
-        T* x = D + nq + r;
-        for (int j = 0; j < nq; ++j) {
-          T v = x[j * LD];
-          const T* uj = D + j * LD;
-#pragma unroll 8
-          for (int kk = 0; kk < j; ++kk) v -= x[kk * LD] * uj[kk];
-          x[j * LD] = v * dinv[j];
-        }
-      }
+      // L21 of the region: x_j = (a_j - sum_k<j x_k U(k, j)) / U(j, j), one
+      // row per thread
+      for (int r = tid; r < rest; r += 128)
+        blk_trsv<T, LD, 1, LD, false>(D + nq + r, D, dinv, nq);
     }
     __syncthreads();
     if (rest > 0) {
@@ -1085,35 +1087,8 @@ blk_offdiag_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
         }
         __syncthreads();
       }
-      if (wave == 0) {  // X_q U_qq^-1, one row per lane
-        T* xr = X + lane + o * 64;
-        const T* uq = Us + o * NQ;
-        if constexpr (NQ <= 48) {
-          // the row in registers (loops unrolled to NQ, cut at nq)
-          T x[NQ];
-#pragma unroll
-          for (int j = 0; j < NQ; ++j) x[j] = (j < nq) ? xr[j * 64] : T(0);
-#pragma unroll
-          for (int j = 0; j < NQ; ++j) {
-            if (j < nq) {
-              T v = x[j];
-#pragma unroll
-              for (int k = 0; k < j; ++k) v -= x[k] * uq[k * NQ + j];
-              x[j] = v * (T(1) / uq[j * NQ + j]);
-            }
-          }
-#pragma unroll
-          for (int j = 0; j < NQ; ++j)
-            if (j < nq) xr[j * 64] = x[j];
-        } else {
-          for (int j = 0; j < nq; ++j) {
-            T v = xr[j * 64];
-#pragma unroll 8
-            for (int k = 0; k < j; ++k) v -= xr[k * 64] * uq[k * NQ + j];
-            xr[j * 64] = v * (T(1) / uq[j * NQ + j]);
-          }
-        }
-      }
+      if (wave == 0)  // X_q U_qq^-1, one row per lane
+        blk_trsv<T, 64, NQ, 1, false>(X + lane + o * 64, Us + o * NQ, nullptr, nq);
       __syncthreads();
     }
     store_tile<T, 1, 64>(Fx, L, nr, bw, X);
@@ -1164,32 +1139,7 @@ blk_offdiag_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
             a[kp] = t;
           }
         }
-        const T* lq = Ls + o * NQ;
-        if constexpr (NQ <= 48) {
-          // the column in registers (loops unrolled to NQ, cut at nq)
-          T u[NQ];
-#pragma unroll
-          for (int i = 0; i < NQ; ++i) u[i] = (i < nq) ? a[i] : T(0);
-#pragma unroll
-          for (int i = 1; i < NQ; ++i) {
-            if (i < nq) {
-              T v = u[i];
-#pragma unroll
-              for (int kk = 0; kk < i; ++kk) v -= lq[i + kk * NQ] * u[kk];
-              u[i] = v;
-            }
-          }
-#pragma unroll
-          for (int i = 1; i < NQ; ++i)
-            if (i < nq) a[i] = u[i];
-        } else {
-          for (int i = 1; i < nq; ++i) {
-            T v = a[i];
-#pragma unroll 8
-            for (int kk = 0; kk < i; ++kk) v -= lq[i + kk * NQ] * a[kk];
-            a[i] = v;
-          }
-        }
+        blk_trsv<T, 1, NQ, 1, true>(a, Ls + o * NQ, nullptr, nq);
       }
       __syncthreads();
     }
