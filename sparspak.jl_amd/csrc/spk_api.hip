@@ -850,13 +850,21 @@ void run_factor(spkhip_handle* h) {
         hipLaunchKernelGGL((spk::large_utile_kernel<T, false>), dim3(lp.ut_cnt), dim3(kWG), 0, st,
                            h->ds, h->d_ut_front.p + lp.ut_off, h->d_ut_piece.p + lp.ut_off,
                            h->d_ut_r0.p + lp.ut_off, lnz.p, unz.p, arena.p);
-      // the column is assembled in LDS when the longest one fits in 64 KB
+      // the column is assembled in LDS when the longest one fits in 64 KB;
+      // LU levels whose columns fit 16 KB take one wave per column
       const size_t clds = (size_t)lp.large_maxL * sizeof(T);
       const bool staged = clds <= 64 * 1024;
-      hipLaunchKernelGGL(spk::large_init_kernel<T>, dim3(lp.items_cnt), dim3(kWG),
-                         staged ? clds : 0, st, h->ds, h->d_it_front.p + lp.items_off,
-                         h->d_it_col.p + lp.items_off, h->d_it_ea.p + lp.items_off,
-                         h->d_ea_child.p, h->d_ea_t.p, lnz.p, arena.p, (int)staged);
+      if (lu && clds <= 16 * 1024)
+        hipLaunchKernelGGL(spk::large_init_wave_kernel<T>, dim3((lp.items_cnt + 3) / 4),
+                           dim3(kWG), 4 * clds, st, h->ds, h->d_it_front.p + lp.items_off,
+                           h->d_it_col.p + lp.items_off, h->d_it_ea.p + lp.items_off,
+                           h->d_ea_child.p, h->d_ea_t.p, lnz.p, arena.p, lp.items_cnt,
+                           lp.large_maxL);
+      else
+        hipLaunchKernelGGL(spk::large_init_kernel<T>, dim3(lp.items_cnt), dim3(kWG),
+                           staged ? clds : 0, st, h->ds, h->d_it_front.p + lp.items_off,
+                           h->d_it_col.p + lp.items_off, h->d_it_ea.p + lp.items_off,
+                           h->d_ea_child.p, h->d_ea_t.p, lnz.p, arena.p, (int)staged);
       hipck(hipGetLastError());
       }
       bool pending = false;  // second-stream work not yet joined

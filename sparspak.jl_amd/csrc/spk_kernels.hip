@@ -455,6 +455,76 @@ large_init_kernel(DevSym s, const int32_t* __restrict__ it_front,
   }
 }
 
+// One wave per front column, four per workgroup: the staged (LDS) assembly
+// of large_init_kernel for LU levels whose longest column fits a quarter of
+// the LDS budget, so that the many short columns of the lower levels take a
+// quarter of the workgroups. A child's update rows are distinct, and one
+// wave's LDS operations complete in issue order, so the children are added
+// one after another without barriers, in the same order as there.
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+large_init_wave_kernel(DevSym s, const int32_t* __restrict__ it_front,
+                       const int32_t* __restrict__ it_col,
+                       const int64_t* __restrict__ it_ea,
+                       const int32_t* __restrict__ ea_child,
+                       const int32_t* __restrict__ ea_t, const T* __restrict__ lnz,
+                       T* __restrict__ arena, int nitems, int maxL) {
+  extern __shared__ unsigned char smem_raw[];
+  const int lane = threadIdx.x % 64, wave = wave_id();
+  const int item = blockIdx.x * 4 + wave;
+  if (item >= nitems) return;  // no workgroup barrier below
+  T* Fs = reinterpret_cast<T*>(smem_raw) + (size_t)wave * maxL;
+  const int f = it_front[item];
+  const int j = it_col[item];
+  const int L = s.fL[f], W = s.fW[f];
+  const int64_t fc0 = s.fc0[f];
+  T* F = arena + s.foff[f] + (int64_t)j * L;
+  const int64_t e0 = it_ea[item], e1 = it_ea[item + 1];
+  const int r1 = (j < W) ? piece_start_of_col(s, fc0, j) : W;
+  const T* lc = (j < W) ? lnz + s.lcol[fc0 + j] : nullptr;
+  for (int r0 = lane; r0 < L; r0 += 8 * 64) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int r = r0 + u * 64;
+      v[u] = (r >= L) ? T(0) : (r < r1) ? F[r] : (lc ? lc[r] : T(0));
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (r0 + u * 64 < L) Fs[r0 + u * 64] = v[u];
+  }
+  __builtin_amdgcn_wave_barrier();
+  for (int64_t e = e0; e < e1; ++e) {
+    const int c = ea_child[e];
+    const int t = ea_t[e];
+    const int Rc = s.fL[c] - s.fW[c];
+    const int32_t* rel = s.relmap + s.frel[c];
+    const T* col = arena + s.fcb[c] + (int64_t)t * s.fcbld[c];
+    for (int tp0 = lane; tp0 < Rc; tp0 += 8 * 64) {
+      int ri[8];
+      T cv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int tp = tp0 + u * 64;
+        ri[u] = (tp < Rc) ? rel[tp] : 0;
+        cv[u] = (tp < Rc) ? col[tp] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (tp0 + u * 64 < Rc) Fs[ri[u]] += cv[u];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  for (int r0 = lane; r0 < L; r0 += 8 * 64) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = (r0 + u * 64 < L) ? Fs[r0 + u * 64] : T(0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (r0 + u * 64 < L) F[r0 + u * 64] = v[u];
+  }
+}
+
 // L part of the own columns back to lnz (contiguous per column).
 template <typename T>
 __global__ void __launch_bounds__(kWG)
