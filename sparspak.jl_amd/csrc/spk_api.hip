@@ -79,6 +79,7 @@ struct DevBuf {
 struct LevelPlan {
   int small_off = 0, small_cnt = 0, small_maxL = 0;
   int items_off = 0, items_cnt = 0;
+  int wb_off = 0, wb_cnt = 0;       // write-back items: the large fronts' pivot columns
   int large_maxL = 0;               // longest index list of the level's large fronts
   struct Launch {
     int off = 0, cnt = 0, wg = 0;
@@ -138,7 +139,8 @@ struct spkhip_handle {
   DevBuf<i64> d_foff, d_fcb, d_frel, d_fvec, d_it_ea, d_lcol, d_ucol;
   DevBuf<BlkDesc> d_bdesc;
   DevBuf<spk::GemmDesc> d_gdesc;
-  DevBuf<i32> d_large, d_ut_front, d_ut_piece, d_ut_r0;
+  DevBuf<i32> d_large, d_ut_front, d_ut_piece, d_ut_r0, d_wb_front, d_wb_col;
+  std::vector<i32> wb_front, wb_col;  // write-back items (host copy, built with the plan)
   DevBuf<spk::SolveDesc> d_sdesc;
   // per-block solve data (packed triangles, composed interchanges), built
   // once per factorisation at the first solve
@@ -421,6 +423,8 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
                 std::vector<i32>& ut_piece, std::vector<i32>& ut_r0) {
   const i32 nf = h->nfront;
   h->pdesc.clear();
+  h->wb_front.clear();
+  h->wb_col.clear();
   h->prep_units = 0;
   h->prepped = false;
   std::vector<std::vector<i32>> bylev(h->nlevel);
@@ -517,6 +521,13 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
       }
     }
     lp.items_cnt = (int)it_front.size() - lp.items_off;
+    lp.wb_off = (int)h->wb_front.size();
+    for (i32 f : large)
+      for (i32 j = 0; j < h->fW[f]; ++j) {
+        h->wb_front.push_back(f);
+        h->wb_col.push_back(j);
+      }
+    lp.wb_cnt = (int)h->wb_front.size() - lp.wb_off;
     // blocks of consecutive pieces, at most kGW columns each (a piece wider
     // than that forms its own block); SPD keeps one piece per block
     struct Blk {
@@ -922,9 +933,10 @@ void run_factor(spkhip_handle* h) {
         hipLaunchKernelGGL((spk::large_utile_kernel<T, true>), dim3(lp.ut_cnt), dim3(kWG), 0, st,
                            h->ds, h->d_ut_front.p + lp.ut_off, h->d_ut_piece.p + lp.ut_off,
                            h->d_ut_r0.p + lp.ut_off, lnz.p, unz.p, arena.p);
-      hipLaunchKernelGGL(spk::large_writeback_kernel<T>, dim3(lp.items_cnt), dim3(kWG), 0, st,
-                         h->ds, h->d_it_front.p + lp.items_off, h->d_it_col.p + lp.items_off,
-                         lnz.p, unz.p, arena.p);
+      if (lp.wb_cnt > 0)
+        hipLaunchKernelGGL(spk::large_writeback_kernel<T>, dim3(lp.wb_cnt), dim3(kWG), 0, st,
+                           h->ds, h->d_wb_front.p + lp.wb_off, h->d_wb_col.p + lp.wb_off,
+                           lnz.p, unz.p, arena.p);
       hipck(hipGetLastError());
     }
   }
@@ -1190,6 +1202,12 @@ void replan(spkhip_handle* h) {
   h->d_ut_front.upload(ut_front, st);
   h->d_ut_piece.upload(ut_piece, st);
   h->d_ut_r0.upload(ut_r0, st);
+  if (h->wb_front.empty()) {
+    h->wb_front.push_back(0);
+    h->wb_col.push_back(0);
+  }
+  h->d_wb_front.upload(h->wb_front, st);
+  h->d_wb_col.upload(h->wb_col, st);
   h->d_sdesc.upload(sdesc, st);
   if (!h->pdesc.empty()) h->d_pdesc.upload(h->pdesc, st);
   DevSym& d = h->ds;
