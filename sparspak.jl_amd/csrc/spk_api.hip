@@ -2026,7 +2026,7 @@ int64_t spkhip_autotune(spkhip_t h, int64_t precision, int64_t reps, spkhip_tuni
     // coordinate search from the current setting; a change is kept only
     // when it is faster by more than 1% (timing noise)
     const std::vector<int64_t> cand[4] = {{64, 48, 32},
-                                          {128, 96, 64},
+                                          {128, 96, 64, 48, 32},
                                           {8, 4, 16, 2},
                                           {16384, 4096, 65536}};
     for (int knob = 0; knob < 4; ++knob) {

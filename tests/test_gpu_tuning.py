@@ -112,7 +112,7 @@ def test_autotune_picks_a_searched_setting(precision):
     s = _prepared(A)
     best = autotune(s, precision, reps=1)
     assert best["small_lmax"] in (64, 48, 32)
-    assert best["block_cols"] in (128, 96, 64)
+    assert best["block_cols"] in (128, 96, 64, 48, 32)
     assert best["upd_group"] in (8, 4, 16, 2)
     assert best["small_wave_min"] in (16384, 4096, 65536)
     assert best["factor_ms"] > 0
