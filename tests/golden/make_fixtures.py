@@ -1,7 +1,7 @@
 """Extract the golden vectors and fixed test matrices held in the reference's
 own Julia tests into tests/golden/fixtures.json (data only: inputs and
-expected outputs). Run in the development container, where the reference is
-mounted; the JSON is committed and the GPU box never reads the reference.
+expected outputs). Run where the reference's sources are present; the JSON is
+committed, and the tests read only the JSON, never the reference.
 
     python tests/golden/make_fixtures.py [/root/reference]
 """
