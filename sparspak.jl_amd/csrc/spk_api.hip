@@ -1024,11 +1024,11 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
       if (lp.large_cnt == 0) continue;
       {
         Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
-        const size_t ylds = (size_t)lp.large_maxL * sizeof(T);
-        const bool staged = ylds <= 64 * 1024;
-        hipLaunchKernelGGL(spk::fwd_assemble_kernel<T>, dim3(lp.large_cnt), dim3(kWG),
-                           staged ? ylds : 0, st, h->ds, h->d_large.p + lp.large_off, rhs, ldr,
-                           nrhs, vec.p, (int)staged);
+        hipLaunchKernelGGL(spk::fwd_assemble_items_kernel<T>,
+                           dim3((lp.items_cnt + kWG - 1) / kWG), dim3(kWG), 0, st, h->ds,
+                           h->d_it_front.p + lp.items_off, h->d_it_col.p + lp.items_off,
+                           h->d_it_ea.p + lp.items_off, h->d_ea_child.p, h->d_ea_t.p, rhs, ldr,
+                           nrhs, vec.p, lp.items_cnt);
         hipck(hipGetLastError());
       }
       for (const LevelPlan::SolveStep& S : lp.fsteps) {
@@ -1055,8 +1055,10 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
       h->cur_grid = lp.large_cnt;
       if (lp.large_cnt > 0) {
         Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);
-        hipLaunchKernelGGL(spk::bwd_gather_kernel<T>, dim3(lp.large_cnt), dim3(kWG), 0, st,
-                           h->ds, h->d_large.p + lp.large_off, rhs, ldr, nrhs, vec.p);
+        hipLaunchKernelGGL(spk::bwd_gather_items_kernel<T>,
+                           dim3((lp.items_cnt + kWG - 1) / kWG), dim3(kWG), 0, st, h->ds,
+                           h->d_it_front.p + lp.items_off, h->d_it_col.p + lp.items_off, rhs,
+                           ldr, nrhs, vec.p, lp.items_cnt);
         hipck(hipGetLastError());
       }
       for (const LevelPlan::SolveStep& S : lp.bsteps) {

@@ -1553,19 +1553,6 @@ __device__ inline T dot_lrow(const DevSym& s, const T* __restrict__ lnz, int64_t
   return acc;
 }
 
-// Children's update vectors into y (fixed child order; y in LDS or in the
-// vector arena).
-template <typename T>
-__device__ void gather_children(const DevSym& s, int f, int q, int nrhs, T* yg, T* vec) {
-  for (int ci = s.fchp[f]; ci < s.fchp[f + 1]; ++ci) {
-    const int c = s.fch[ci];
-    const int Lc = s.fL[c], Wc = s.fW[c];
-    const int32_t* rel = s.relmap + s.frel[c];
-    const T* uc = vec + s.fvec[c] * nrhs + (int64_t)q * Lc + Wc;
-    for (int t = threadIdx.x; t < Lc - Wc; t += kWG) yg[rel[t]] += uc[t];
-    __syncthreads();
-  }
-}
 
 // Unit-lower solve of one piece's diagonal block on yb (LDS) by wave 0,
 // after the piece's interchanges (glaswp! then gtrsm! 'l','l','n','u').
@@ -1730,30 +1717,54 @@ bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw,
   }
 }
 
-// Large fronts, forward: y = [rhs on own columns; 0] + children (one
-// workgroup per front).
+// Large fronts, forward: y = [rhs on own columns; 0] + children, one thread
+// per (front, row) item of the level (the factor's column items: a front's
+// rows and columns share its index list). The children's entries come from
+// the item's (child, row) list in ascending child order, so each entry is
+// the same sum in the same order as a per-front gather; a level's fronts
+// spread over many workgroups.
 template <typename T>
 __global__ void __launch_bounds__(kWG)
-fwd_assemble_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict__ rhs,
-                    int64_t ldr, int nrhs, T* __restrict__ vec, int staged) {
-  extern __shared__ unsigned char smem_raw[];
-  const int f = list[blockIdx.x];
+fwd_assemble_items_kernel(DevSym s, const int32_t* __restrict__ it_front,
+                          const int32_t* __restrict__ it_col,
+                          const int64_t* __restrict__ it_ea,
+                          const int32_t* __restrict__ ea_child,
+                          const int32_t* __restrict__ ea_t, const T* __restrict__ rhs,
+                          int64_t ldr, int nrhs, T* __restrict__ vec, int nitems) {
+  const int item = blockIdx.x * kWG + threadIdx.x;
+  if (item >= nitems) return;
+  const int f = it_front[item], r = it_col[item];
   const int L = s.fL[f], W = s.fW[f];
   const int64_t fc0 = s.fc0[f];
+  const int64_t e0 = it_ea[item], e1 = it_ea[item + 1];
   for (int q = 0; q < nrhs; ++q) {
-    T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
-    const T* x = rhs + (int64_t)q * ldr;
-    // y in LDS when the level's longest front fits (staged), else in place
-    T* y = staged ? reinterpret_cast<T*>(smem_raw) : yg;
-    for (int r = threadIdx.x; r < L; r += kWG) y[r] = (r < W) ? x[fc0 + r] : T(0);
-    __syncthreads();
-    gather_children<T>(s, f, q, nrhs, y, vec);
-    if (staged) {
-      for (int r = threadIdx.x; r < L; r += kWG) yg[r] = y[r];
-      __syncthreads();
+    T y = (r < W) ? rhs[(int64_t)q * ldr + fc0 + r] : T(0);
+    for (int64_t e = e0; e < e1; ++e) {
+      const int c = ea_child[e], t = ea_t[e];
+      y += vec[s.fvec[c] * nrhs + (int64_t)q * s.fL[c] + s.fW[c] + t];
     }
+    vec[s.fvec[f] * nrhs + (int64_t)q * L + r] = y;
   }
 }
+
+// Backward: the update rows of each large front's solve vector gathered
+// from the solution by the front's index list, one thread per (front, row)
+// item of the level.
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+bwd_gather_items_kernel(DevSym s, const int32_t* __restrict__ it_front,
+                        const int32_t* __restrict__ it_col, const T* __restrict__ rhs,
+                        int64_t ldr, int nrhs, T* __restrict__ vec, int nitems) {
+  const int item = blockIdx.x * kWG + threadIdx.x;
+  if (item >= nitems) return;
+  const int f = it_front[item], j = it_col[item];
+  const int L = s.fL[f], W = s.fW[f];
+  if (j < W) return;
+  const int g = s.lindx[s.xlindx[s.fsp0[f]] - 1 + j] - 1;
+  for (int q = 0; q < nrhs; ++q)
+    vec[s.fvec[f] * nrhs + (int64_t)q * L + j] = rhs[(int64_t)q * ldr + g];
+}
+
 
 // Per-block solve data, built once per factorisation and read by every
 // solve (refinement solves many times). A block of bw columns is padded to
@@ -2076,36 +2087,10 @@ fwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   }
 }
 
-// Backward: x of a large front's update rows, x[I[W:L]], gathered into the
-// front's vector (rows W..L-1; the forward pass no longer needs them), so
-// that the backward GEMVs read it contiguously. One workgroup per front.
-template <typename T>
-__global__ void __launch_bounds__(kWG)
-bwd_gather_kernel(DevSym s, const int32_t* __restrict__ list, const T* __restrict__ rhs,
-                  int64_t ldr, int nrhs, T* __restrict__ vec) {
-  const int f = list[blockIdx.x];
-  const int L = s.fL[f], W = s.fW[f];
-  const int32_t* I = s.lindx + (s.xlindx[s.fsp0[f]] - 1);
-  for (int q = 0; q < nrhs; ++q) {
-    const T* x = rhs + (int64_t)q * ldr;
-    T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
-    for (int j0 = W + threadIdx.x; j0 < L; j0 += 4 * kWG) {
-      int g[4];
-      T v[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) g[u] = (j0 + u * kWG < L) ? I[j0 + u * kWG] - 1 : 0;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = (j0 + u * kWG < L) ? x[g[u]] : T(0);
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (j0 + u * kWG < L) yg[j0 + u * kWG] = v[u];
-    }
-  }
-}
 
 // Backward: t[c] = U[c, c1:L] x[c1:L] for row c = c0 + r0 of the block, one
 // workgroup per row, stored in y[c]. x[c1:L] is read from the front's vector:
-// its update rows from bwd_gather_kernel, its own columns as the backward
+// its update rows from bwd_gather_items_kernel, its own columns as the backward
 // chains of the later blocks left them. Every thread takes columns c1 + tid
 // + 256 m, sixteen loads in flight; per-thread sums are added over the wave,
 // then the four waves in a fixed order.
