@@ -122,6 +122,9 @@ def main():
     d_nz = torch.from_numpy(np.ascontiguousarray(A.data, dtype=np.float64)).to(dev)
     d_b = torch.from_numpy(np.ascontiguousarray(b)).to(dev)
     d_x = torch.empty_like(d_b)
+    # the library orders its reads of d_nz / d_x after torch's stream
+    _lib.check(L.spkhip_set_caller_stream(
+        h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "set_caller_stream")
     torch.cuda.synchronize()
 
     prec = {"fp64": 0, "fp32": 1, "fp32_bf16": 2, "fp32_fp16": 3, "fp32_bf16x3": 4}

@@ -11,9 +11,21 @@
  *   -2     workspace / size limit
  *   -3     sequence error (a phase called before the one it depends on)
  *   -4     bad argument
+ *   -5     refinement did not reach the requested backward error within
+ *          max_iter corrections (the solution is returned; its backward
+ *          error is in spkhip_stats)
+ *   -6     the solution or residual is not finite (NaN or Inf): never
+ *          certified; backward_error is +inf
  *   <= -1000  HIP runtime error (-1000 - hipError_t)
  * No C++ exception crosses this boundary. One handle serialises its calls on
  * its own HIP stream; different handles may be used from different threads.
+ *
+ * Device buffers of the caller (the _device entry points): before reading
+ * them, the handle's stream waits for all work the caller has issued on its
+ * stream — the legacy default stream (what PyTorch's default stream is on
+ * ROCm) unless spkhip_set_caller_stream names another — so a producer kernel
+ * launched just before the call needs no host synchronisation. Every entry
+ * point returns only after its own work has completed.
  */
 #ifndef SPARSPAK_HIP_H
 #define SPARSPAK_HIP_H
@@ -73,6 +85,22 @@ int64_t spk_symbfact(int64_t n, const int64_t* xadj, const int64_t* adj,
 void spk_supernode_tree(int64_t nsuper, const int64_t* xsuper,
                         const int64_t* snode, const int64_t* xlindx,
                         const int64_t* lindx, int64_t* sparent);
+/* The front (fundamental supernode) schedule spkhip_symbolic_upload builds,
+ * on the host, with the relative-index maps it uploads. These maps replace
+ * the per-pair maps of _ldindx!/_igathr! (SpkSpdMMOps.jl:70-102): for the
+ * t-th update row of front f (t = 0 .. L_f - W_f - 1, the rows of its index
+ * list after its W_f own columns), relmap[frel[f] - 1 + t] is the 0-based
+ * position of that row in the parent front's index list (of length L_p),
+ * i.e. L_p - 1 - relind with relind = map[row] = distance from the bottom of
+ * the list (_ldindx!). Outputs (1-based values): fsp0[f] first supernode of
+ * front f, fnp[f] its supernodes, fparent[f] the parent front (0: root),
+ * frel[f] (nfront + 1 entries) offsets into relmap. Array sizes: nsuper
+ * (nsuper + 1 for frel) and nsub = xlindx[nsuper] - 1 for relmap. Returns
+ * nfront, -4 when the index lists do not nest. No device is used. */
+int64_t spk_front_maps(int64_t n, int64_t nsuper, const int64_t* xsuper,
+                       const int64_t* snode, const int64_t* xlindx,
+                       const int64_t* lindx, int64_t* fsp0, int64_t* fnp,
+                       int64_t* fparent, int64_t* frel, int64_t* relmap);
 
 /* ======================================================================
  * Device numeric phase.
@@ -169,6 +197,11 @@ int64_t spkhip_solve_refine_device(spkhip_t h, int64_t nrhs, double* d_b,
  * offsets[nseg] = n). Default: one segment. No counterpart in the reference
  * (it factors one system at a time). */
 int64_t spkhip_set_segments(spkhip_t h, int64_t nseg, const int64_t* offsets);
+/* The stream (a hipStream_t, NULL = the legacy default stream) on which the
+ * caller produces the device buffers it passes to the _device entry points;
+ * they order themselves after that stream's earlier work. No counterpart in
+ * the reference (it has no device). */
+int64_t spkhip_set_caller_stream(spkhip_t h, void* stream);
 
 /* Copies the factor back in the reference layout: lnz (|lnz|), unz (|unz|),
  * ipvt (n, local 1..nj pivots). Double variant converts a float factor. */

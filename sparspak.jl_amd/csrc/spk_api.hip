@@ -169,10 +169,17 @@ struct spkhip_handle {
   DevBuf<unsigned long long> d_fail;
   i64 first_fail = 0;
   // solve / refine work
-  DevBuf<double> d_b, d_x, d_r, d_tmp64, d_norms;
+  DevBuf<double> d_b, d_x, d_r, d_tmp64, d_norms, d_bm;
   std::vector<i64> seg{0};            // row segments for the backward error
-  DevBuf<i64> d_seg, d_clo;
+  // row chunks of the segments (device copies), rebuilt when seg changes
+  DevBuf<i64> d_clo;
   DevBuf<i32> d_cseg;
+  std::vector<i64> chunk_key;         // the seg vector the chunks were built for
+  int nchunk = 0;
+  // the caller's stream whose earlier work the _device entry points wait
+  // for before they read caller buffers (nullptr: the legacy default stream)
+  hipStream_t caller_stream = nullptr;
+  hipEvent_t ev_caller = nullptr;
   int refine_iters = 0;
   double backward_error = 0;
   // timing
@@ -208,6 +215,7 @@ struct spkhip_handle {
   ~spkhip_handle() {
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
     for (hipEvent_t e : xev) (void)hipEventDestroy(e);
+    if (ev_caller) (void)hipEventDestroy(ev_caller);
     if (g_exec) (void)hipGraphExecDestroy(g_exec);
   }
 };
@@ -715,6 +723,17 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
 }
 
 void sync(spkhip_handle* h) { hipck(hipStreamSynchronize(h->stream)); }
+
+// The handle's streams do not synchronise with other streams by themselves
+// (non-blocking): before an entry point reads a caller's device buffer, the
+// handle's stream waits for everything the caller issued so far on its
+// stream (the legacy default stream unless spkhip_set_caller_stream named
+// another), without a host synchronisation.
+void wait_caller(spkhip_handle* h) {
+  if (!h->ev_caller) hipck(hipEventCreateWithFlags(&h->ev_caller, hipEventDisableTiming));
+  hipck(hipEventRecord(h->ev_caller, h->caller_stream));
+  hipck(hipStreamWaitEvent(h->stream, h->ev_caller, 0));
+}
 
 // Map of each CSC entry of A (original numbering) to its lnz/unz slot, with
 // the reference's search semantics (SparseCSCInterface.jl:112-163): found ->
@@ -1682,6 +1701,7 @@ int64_t spkhip_inmatrix_device(spkhip_t h, const double* d_nzval) {
   return wrap([&]() -> int64_t {
     auto t0 = Clock::now();
     h->d_nzval.ensure((size_t)std::max<i64>(h->nnz, 1));
+    wait_caller(h);
     if (h->nnz)
       hipck(hipMemcpyAsync(h->d_nzval.p, d_nzval, h->nnz * sizeof(double),
                            hipMemcpyDeviceToDevice, h->stream));
@@ -1733,6 +1753,7 @@ int64_t spkhip_solve_device(spkhip_t h, int64_t nrhs, double* d_b, int64_t ldb) 
   if (nrhs < 0 || ldb < h->n) return -4;
   if (nrhs == 0) return 0;
   return wrap([&]() -> int64_t {
+    wait_caller(h);
     hipck(hipEventRecord(h->ev0, h->stream));
     if (h->precision == SPKHIP_FP64)
       solve_device<double>(h, d_b, ldb, (int)nrhs);
@@ -1800,7 +1821,6 @@ int64_t refine_impl(spkhip_t h, const double* d_bm, double* d_xo, int nrhs, doub
   // per (segment, column) norms: r, x, b, and ||A||_inf per segment
   if (h->seg.size() < 2 || h->seg.back() != n) h->seg = {0, n};
   const int nseg = (int)h->seg.size() - 1;
-  h->d_seg.upload(h->seg, st);
   h->d_norms.ensure((size_t)3 * nseg * nrhs + nseg);
   double* nr_r = h->d_norms.p;
   double* nr_x = nr_r + (size_t)nseg * nrhs;
@@ -1808,17 +1828,22 @@ int64_t refine_impl(spkhip_t h, const double* d_bm, double* d_xo, int nrhs, doub
   double* nr_a = nr_b + (size_t)nseg * nrhs;
   std::vector<double> nrm((size_t)3 * nseg * nrhs + nseg);
   // row chunks of at most 8192 rows, none crossing a segment boundary
-  std::vector<i64> clo;
-  std::vector<i32> cseg;
-  for (int i = 0; i < nseg; ++i)
-    for (i64 r = h->seg[i]; r < h->seg[i + 1]; r += 8192) {
-      clo.push_back(r);
-      cseg.push_back(i);
-    }
-  const int nch = (int)cseg.size();
-  clo.push_back(n);
-  h->d_clo.upload(clo, st);
-  h->d_cseg.upload(cseg.empty() ? std::vector<i32>{0} : cseg, st);
+  // (built once per segment layout)
+  if (h->chunk_key != h->seg) {
+    std::vector<i64> clo;
+    std::vector<i32> cseg;
+    for (int i = 0; i < nseg; ++i)
+      for (i64 r = h->seg[i]; r < h->seg[i + 1]; r += 8192) {
+        clo.push_back(r);
+        cseg.push_back(i);
+      }
+    h->nchunk = (int)cseg.size();
+    clo.push_back(n);
+    h->d_clo.upload(clo, st);
+    h->d_cseg.upload(cseg.empty() ? std::vector<i32>{0} : cseg, st);
+    h->chunk_key = h->seg;
+  }
+  const int nch = h->nchunk;
   auto segmax = [&](const double* v, int ncol, double* out) {
     hipLaunchKernelGGL(spk::fill_kernel<double>, dim3(grid_for((i64)nseg * ncol)), dim3(kWG), 0,
                        st, out, (int64_t)nseg * ncol, 0.0);
@@ -1853,19 +1878,29 @@ int64_t refine_impl(spkhip_t h, const double* d_bm, double* d_xo, int nrhs, doub
     hipck(hipMemcpyAsync(nrm.data(), h->d_norms.p, nrm.size() * sizeof(double),
                          hipMemcpyDeviceToHost, st));
     sync(h);
+    // a non-finite norm (the reductions count NaN as +inf) or quotient marks
+    // the solution as non-finite: never certified, whatever tol is
     eta_max = 0;
+    bool finite = true;
     const size_t sq = (size_t)nseg * nrhs;
     for (size_t i = 0; i < sq; ++i) {
-      const double den = nrm[3 * sq + i % nseg] * nrm[sq + i] + nrm[2 * sq + i];
+      const double na = nrm[3 * sq + i % nseg], nx = nrm[sq + i], nb = nrm[2 * sq + i];
+      if (!std::isfinite(nrm[i]) || !std::isfinite(na) || !std::isfinite(nx) ||
+          !std::isfinite(nb))
+        finite = false;
+      const double den = na * nx + nb;
       const double eta = den > 0 ? nrm[i] / den : 0.0;
-      eta_max = std::max(eta_max, eta);
+      if (!std::isfinite(eta)) finite = false;
+      if (eta > eta_max) eta_max = eta;
     }
-    if (eta_max <= tol || iters >= max_iter || !std::isfinite(eta_max)) break;
+    if (!finite) eta_max = HUGE_VAL;
+    if (!finite || eta_max <= tol || iters >= max_iter) break;
     src = h->d_r.p;
     ++iters;
   }
   h->refine_iters = iters;
   h->backward_error = eta_max;
+  if (!std::isfinite(eta_max)) return -6;
   return (eta_max <= tol) ? 0 : -5;
 }
 }  // namespace
@@ -1880,10 +1915,11 @@ int64_t spkhip_solve_refine_device(spkhip_t h, int64_t nrhs, double* d_b, int64_
   if (nrhs == 0) return 0;
   return wrap([&]() -> int64_t {
     const i64 n = h->n;
-    hipck(hipEventRecord(h->ev0, h->stream));
-    DevBuf<double> bm;
-    bm.alloc((size_t)n * nrhs);
+    DevBuf<double>& bm = h->d_bm;
+    bm.ensure((size_t)n * nrhs);
     h->d_x.ensure((size_t)n * nrhs);
+    wait_caller(h);
+    hipck(hipEventRecord(h->ev0, h->stream));
     for (i64 q = 0; q < nrhs; ++q)
       hipck(hipMemcpyAsync(bm.p + q * n, d_b + q * ldb, n * sizeof(double),
                            hipMemcpyDeviceToDevice, h->stream));
@@ -1916,7 +1952,7 @@ int64_t spkhip_solve_refine(spkhip_t h, int64_t nrhs, double* b, int64_t ldb, do
       hipck(hipMemcpyAsync(h->d_b.p + q * n, b + q * ldb, n * sizeof(double),
                            hipMemcpyHostToDevice, h->stream));
     int64_t rc = spkhip_solve_refine_device(h, nrhs, h->d_b.p, n, tol, max_iter);
-    if (rc != 0 && rc != -5) return rc;
+    if (rc != 0 && rc != -5 && rc != -6) return rc;
     for (i64 q = 0; q < nrhs; ++q)
       hipck(hipMemcpyAsync(b + q * ldb, h->d_b.p + q * n, n * sizeof(double),
                            hipMemcpyDeviceToHost, h->stream));
@@ -1935,6 +1971,45 @@ int64_t spkhip_download_f32(spkhip_t h, float* lnz, float* unz, int64_t* ipvt) {
   if (guard(h)) return -4;
   if (!h->factored) return -3;
   return wrap([&]() -> int64_t { return download_impl<float>(h, lnz, unz, ipvt); });
+}
+
+int64_t spk_front_maps(int64_t n, int64_t nsuper, const int64_t* xsuper, const int64_t* snode,
+                       const int64_t* xlindx, const int64_t* lindx, int64_t* fsp0,
+                       int64_t* fnp, int64_t* fparent, int64_t* frel, int64_t* relmap) {
+  if (n <= 0 || nsuper <= 0 || !xsuper || !snode || !xlindx || !lindx || !fsp0 || !fnp ||
+      !fparent || !frel || !relmap)
+    return -4;
+  return wrap([&]() -> int64_t {
+    // the schedule construction of spkhip_symbolic_upload, on a host-only
+    // handle (no device calls)
+    spkhip_handle hh;
+    hh.n = n;
+    hh.nsuper = nsuper;
+    hh.xsuper.assign(xsuper, xsuper + nsuper + 1);
+    hh.snode.assign(snode, snode + n);
+    hh.xlindx.assign(xlindx, xlindx + nsuper + 1);
+    hh.nsub = hh.xlindx[nsuper] - 1;
+    hh.lindx.resize(std::max<i64>(hh.nsub, 1));
+    for (i64 k = 0; k < hh.nsub; ++k) hh.lindx[k] = (i32)lindx[k];
+    build_fronts(&hh);
+    i64 nrel = 0;
+    for (i32 f = 0; f < hh.nfront; ++f) {
+      fsp0[f] = hh.fsp0[f] + 1;
+      fnp[f] = hh.fnp[f];
+      fparent[f] = hh.fpar[f] + 1;
+      frel[f] = hh.frel[f] + 1;
+      nrel += hh.fL[f] - hh.fW[f];
+    }
+    frel[hh.nfront] = nrel + 1;
+    for (i64 k = 0; k < nrel; ++k) relmap[k] = hh.relmap[k];
+    return hh.nfront;
+  });
+}
+
+int64_t spkhip_set_caller_stream(spkhip_t h, void* stream) {
+  if (guard(h)) return -4;
+  h->caller_stream = (hipStream_t)stream;
+  return 0;
 }
 
 int64_t spkhip_set_segments(spkhip_t h, int64_t nseg, const int64_t* offsets) {

@@ -7,6 +7,17 @@ namespace spk {
 // (scalar) value so that per-wave index arithmetic stays on the scalar unit.
 __device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / 64); }
 
+// Orders one wave's LDS accesses: every access before it is complete and
+// visible to the wave's other lanes before any access after it. The
+// wavefront-scope fences emit no instructions (one wave's LDS operations
+// complete in issue order) but keep the compiler from moving LDS loads or
+// stores across the barrier, which the barrier intrinsic alone does not.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ inline int64_t ld_i64(const int64_t* p, int64_t i) { return p[i]; }
 
 // Column range of piece p (0-based supernode): first column (0-based) and
@@ -111,7 +122,7 @@ __device__ void lds_partial_lu(T* F, int L, int cq, int nq, int pivot,
     if constexpr (G == kWG)
       __syncthreads();
     else
-      __builtin_amdgcn_wave_barrier();
+      wave_lds_sync();
   };
   for (int k = 0; k < nq; ++k) {
     const int col = cq + k;
@@ -185,7 +196,7 @@ small_front_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw,
     if constexpr (G == kWG)
       __syncthreads();
     else
-      __builtin_amdgcn_wave_barrier();
+      wave_lds_sync();
   };
   T* F = reinterpret_cast<T*>(smem_raw) + (int64_t)grp * maxL * maxL;
   int* s_piv = s_piv_all[grp];
@@ -493,7 +504,7 @@ large_init_wave_kernel(DevSym s, const int32_t* __restrict__ it_front,
     for (int u = 0; u < 8; ++u)
       if (r0 + u * 64 < L) Fs[r0 + u * 64] = v[u];
   }
-  __builtin_amdgcn_wave_barrier();
+  wave_lds_sync();
   for (int64_t e = e0; e < e1; ++e) {
     const int c = ea_child[e];
     const int t = ea_t[e];
@@ -513,7 +524,7 @@ large_init_wave_kernel(DevSym s, const int32_t* __restrict__ it_front,
       for (int u = 0; u < 8; ++u)
         if (tp0 + u * 64 < Rc) Fs[ri[u]] += cv[u];
     }
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
   }
   for (int r0 = lane; r0 < L; r0 += 8 * 64) {
     T v[8];
@@ -782,10 +793,10 @@ __device__ void pos_getrf(T* D, int nq, int ncols, int* piv, int* perm, T* dinv,
     }
     for (; j0 < nq; ++j0)
       if (below) D[lane + j0 * LD] -= l * D[blane + j0 * LD];
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
   }
   if (mine) perm[pos] = lane;
-  __builtin_amdgcn_wave_barrier();
+  wave_lds_sync();
   // rows to their positions: position p takes LDS row perm[p]
   const T* srow = D + (mine ? perm[lane] : 0);
   T* drow = D + lane;
@@ -794,18 +805,18 @@ __device__ void pos_getrf(T* D, int nq, int ncols, int* piv, int* perm, T* dinv,
     T x[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) x[u] = srow[(c0 + u) * LD];
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
     if (mine) {
 #pragma unroll
       for (int u = 0; u < 8; ++u) drow[(c0 + u) * LD] = x[u];
     }
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
   }
   for (; c0 < ncols; ++c0) {
     const T x = srow[c0 * LD];
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
     if (mine) drow[c0 * LD] = x;
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
   }
 }
 
@@ -980,7 +991,7 @@ __device__ void nopiv_getrf(T* D, int nq, int* piv, T* dinv, int* zero) {
           if (j0 + jj < nq) D[i + (j0 + jj) * LD] = x[jj] - l * u[jj];
       }
     }
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
   }
 }
 
@@ -1568,12 +1579,12 @@ __device__ void wave_lower_solve(T* yb, const T* D, const int* piv, int nq) {
         yb[ip] = t;
       }
     }
-  __builtin_amdgcn_wave_barrier();
+  wave_lds_sync();
   for (int k = 0; k < nq; ++k) {
     const T yk = yb[k];
     if (yk != T(0))
       for (int i = k + 1 + lane; i < nq; i += 64) yb[i] -= yk * D[i + k * MAXW];
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
   }
 }
 
@@ -1585,11 +1596,11 @@ __device__ void wave_upper_solve(T* yb, const T* D, int nq) {
     T yk = yb[k];
     if (yk != T(0)) {
       yk /= D[k + k * MAXW];
-      __builtin_amdgcn_wave_barrier();
+      wave_lds_sync();
       if (lane == 0) yb[k] = yk;
       for (int i = lane; i < k; i += 64) yb[i] -= yk * D[i + k * MAXW];
     }
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
   }
 }
 
@@ -1625,14 +1636,14 @@ fwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw,
     T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
     T* x = rhs + (int64_t)q * ldr;
     for (int r = lane; r < L; r += 64) yW[r] = (r < W) ? x[fc0 + r] : T(0);
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
     for (int ci = s.fchp[f]; ci < s.fchp[f + 1]; ++ci) {
       const int c = s.fch[ci];
       const int Lc = s.fL[c], Wc = s.fW[c];
       const int32_t* rel = s.relmap + s.frel[c];
       const T* uc = vec + s.fvec[c] * nrhs + (int64_t)q * Lc + Wc;
       for (int t = lane; t < Lc - Wc; t += 64) yW[rel[t]] += uc[t];
-      __builtin_amdgcn_wave_barrier();
+      wave_lds_sync();
     }
     for (int p = sp0; p < sp0 + np; ++p) {
       int64_t pc0, nq64;
@@ -1646,20 +1657,20 @@ fwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw,
         D[i + kk * MAXW] = lp[i + (int64_t)kk * jl];
       }
       for (int i = lane; i < nq; i += 64) spiv[i] = (int)ipvt[pc0 + i] - 1;
-      __builtin_amdgcn_wave_barrier();
+      wave_lds_sync();
       wave_lower_solve<T, MAXW>(yW + cq, D, spiv, nq);
-      __builtin_amdgcn_wave_barrier();
+      wave_lds_sync();
       for (int r = cq + nq + lane; r < L; r += 64) {
         const T* dr = D + (r - cq);
         T acc = T(0);
         for (int kk = 0; kk < nq; ++kk) acc += dr[kk * MAXW] * yW[cq + kk];
         yW[r] -= acc;
       }
-      __builtin_amdgcn_wave_barrier();
+      wave_lds_sync();
     }
     for (int r = W + lane; r < L; r += 64) yg[r] = yW[r];
     for (int r = lane; r < W; r += 64) x[fc0 + r] = yW[r];
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
   }
 }
 
@@ -1686,7 +1697,7 @@ bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw,
   for (int q = 0; q < nrhs; ++q) {
     T* x = rhs + (int64_t)q * ldr;
     for (int r = lane; r < L; r += 64) yW[r] = x[I[r] - 1];
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
     for (int p = sp0 + np - 1; p >= sp0; --p) {
       int64_t pc0, nq64;
       piece_cols(s, p, pc0, nq64);
@@ -1701,19 +1712,19 @@ bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw,
         const int c = idx % nq, j = idx / nq;
         D[c + j * MAXW] = (j < nq) ? lp[c + (int64_t)j * jl] : unz[s.ucol[pc0 + c] + cq + j];
       }
-      __builtin_amdgcn_wave_barrier();
+      wave_lds_sync();
       for (int c = 0; c < nq; ++c) {
         T acc = T(0);
         for (int j = cq + nq + lane; j < L; j += 64) acc += D[c + (j - cq) * MAXW] * yW[j];
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
         if (lane == 0) yW[cq + c] -= acc;
       }
-      __builtin_amdgcn_wave_barrier();
+      wave_lds_sync();
       wave_upper_solve<T, MAXW>(yW + cq, D, nq);
-      __builtin_amdgcn_wave_barrier();
+      wave_lds_sync();
     }
     for (int c = lane; c < W; c += 64) x[fc0 + c] = yW[c];
-    __builtin_amdgcn_wave_barrier();
+    wave_lds_sync();
   }
 }
 
@@ -2313,21 +2324,29 @@ __global__ void residual_kernel(const int64_t* __restrict__ rowptr,
   }
 }
 
-// Row sums |A| for ||A||_inf.
+// |x| for a max-norm that must not lose a NaN: fmax drops a NaN operand, so
+// a NaN counts as +inf (the norm then reports the vector as non-finite).
+__device__ inline double abs_nan_inf(double x) {
+  const double a = fabs(x);
+  return (a != a) ? __builtin_huge_val() : a;
+}
+
+// Row sums |A| for ||A||_inf (a NaN entry makes its row sum +inf).
 __global__ void rowabs_kernel(const int64_t* __restrict__ rowptr,
                               const double* __restrict__ val,
                               double* __restrict__ out, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double acc = 0.0;
-  for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) acc += fabs(val[k]);
+  for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) acc += abs_nan_inf(val[k]);
   out[i] = acc;
 }
 
 // Max |v| over row chunks [clo[c], clo[c+1]) of each column, folded into the
 // per-(segment, column) maxima out[cseg[c] + q * nseg] by an integer
-// atomicMax on the bit patterns (a non-negative double orders as its bits);
-// out starts at +0.0. One block per (chunk, column).
+// atomicMax on the bit patterns (a non-negative double orders as its bits,
+// +inf above every finite value); out starts at +0.0. NaN entries count as
+// +inf. One block per (chunk, column).
 __global__ void segmax_chunk_kernel(const double* __restrict__ v, int64_t ld,
                                     const int64_t* __restrict__ clo,
                                     const int32_t* __restrict__ cseg, int nchunks, int nseg,
@@ -2341,11 +2360,11 @@ __global__ void segmax_chunk_kernel(const double* __restrict__ v, int64_t ld,
   for (; r + 7 * kWG < hi; r += 8 * kWG) {
     double a[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) a[u] = fabs(vq[r + u * kWG]);
+    for (int u = 0; u < 8; ++u) a[u] = abs_nan_inf(vq[r + u * kWG]);
 #pragma unroll
     for (int u = 0; u < 8; ++u) m = fmax(m, a[u]);
   }
-  for (; r < hi; r += kWG) m = fmax(m, fabs(vq[r]));
+  for (; r < hi; r += kWG) m = fmax(m, abs_nan_inf(vq[r]));
   red[threadIdx.x] = m;
   __syncthreads();
   for (int off = kWG / 2; off > 0; off >>= 1) {

@@ -120,6 +120,25 @@ factor!(h::Handle; precision=SPKHIP_FP64) = check(
     ccall((:spkhip_factor, libspk), Int64, (Ptr{Cvoid}, Int64), h.ptr, precision),
     "An empty problem. No matrix.")   # the message _factor! raises (SpkSparseBase.jl:387)
 
+# spkhip_stats_t (field order of include/sparspak_hip.h)
+mutable struct Stats
+    n::Int64; nsuper::Int64; nfront::Int64; nlevel::Int64
+    nnz_lnz::Int64; nnz_unz::Int64
+    first_fail_supernode::Int64; refine_iters::Int64
+    factor_flops::Float64; solve_flops::Float64; front_bytes::Float64
+    t_upload_ms::Float64; t_inmatrix_ms::Float64; t_factor_ms::Float64
+    t_solve_ms::Float64; t_refine_ms::Float64
+    backward_error::Float64
+end
+Stats() = Stats(zeros(Int64, 8)..., zeros(Float64, 9)...)
+
+function stats(h::Handle)
+    st = Ref(Stats())
+    check(ccall((:spkhip_stats, libspk), Int64, (Ptr{Cvoid}, Ref{Stats}), h.ptr, st),
+          "spkhip_stats")
+    return st[]
+end
+
 # plan knobs and the autotuner (spkhip_tuning_t; no counterpart in the reference)
 mutable struct Tuning
     small_lmax::Int64
@@ -159,7 +178,14 @@ function solve!(h::Handle, B::StridedVecOrMat{Float64}; refine=false, tol=1e-14,
             ccall((:spkhip_solve, libspk), Int64, (Ptr{Cvoid}, Int64, Ptr{Float64}, Int64),
                   h.ptr, nrhs, B, ldb)
     end
-    (rc == 0 || rc == -5) || check(rc, "Triangular Solve.")
+    if rc == -5
+        st = stats(h)
+        @warn "iterative refinement stopped at backward error $(st.backward_error), above tol=$tol"
+    elseif rc == -6
+        throw(DomainError(rc, "Triangular Solve: the solution is not finite (NaN or Inf)"))
+    else
+        check(rc, "Triangular Solve.")
+    end
     return B
 end
 

@@ -6,6 +6,7 @@ numeric phase.
 """
 import ctypes
 import os
+import warnings
 
 import numpy as np
 
@@ -65,6 +66,8 @@ _SIGS = {
     "spk_symbfact": (i64, [i64, i64p, i64p, i64p, i64p, i64p, i64, i64p,
                            i64p, i64, i64p, i64p]),
     "spk_supernode_tree": (None, [i64, i64p, i64p, i64p, i64p, i64p]),
+    "spk_front_maps": (i64, [i64, i64, i64p, i64p, i64p, i64p, i64p, i64p, i64p, i64p,
+                             i64p]),
     # device numeric phase
     "spkhip_create": (i64, [ctypes.POINTER(vp), i64]),
     "spkhip_destroy": (i64, [vp]),
@@ -80,6 +83,7 @@ _SIGS = {
     "spkhip_solve_refine_device": (i64, [vp, i64, vp, i64, ctypes.c_double,
                                          i64]),
     "spkhip_set_segments": (i64, [vp, i64, i64p]),
+    "spkhip_set_caller_stream": (i64, [vp, vp]),
     "spkhip_download": (i64, [vp, dp, dp, i64p]),
     "spkhip_download_f32": (i64, [vp, fp, fp, i64p]),
     "spkhip_stats": (i64, [vp, ctypes.POINTER(Stats)]),
@@ -152,6 +156,10 @@ class HipError(RuntimeError):
     pass
 
 
+class RefinementWarning(RuntimeWarning):
+    """Iterative refinement stopped above the requested backward error."""
+
+
 def check(rc, what):
     """Translate a library return code into an exception (0 passes)."""
     rc = int(rc)
@@ -160,3 +168,19 @@ def check(rc, what):
     if rc <= -1000:
         raise HipError(f"{what}: HIP error {-1000 - rc}")
     raise RuntimeError(f"{what}: error code {rc}")
+
+
+def check_refine(rc, what, backward_error):
+    """Return code of a refined solve: -5 (tolerance not reached within
+    max_iter corrections; the solution is still returned) warns with the
+    backward error reached, -6 (NaN or Inf in the solution) raises
+    FloatingPointError, anything else as check()."""
+    rc = int(rc)
+    if rc == -5:
+        warnings.warn(f"{what}: iterative refinement stopped at backward error "
+                      f"{backward_error:.3e}, above the requested tolerance",
+                      RefinementWarning, stacklevel=3)
+        return rc
+    if rc == -6:
+        raise FloatingPointError(f"{what}: the solution is not finite (NaN or Inf)")
+    return check(rc, what)

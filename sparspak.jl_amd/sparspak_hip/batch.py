@@ -120,9 +120,18 @@ class BatchedSolver:
         """inmatrix for every system; values: list of per-system nzval arrays
         (CSC order of each system) or None for the matrices' own values."""
         self.upload()
+        if values is not None:
+            if len(values) != len(self.systems):
+                raise ValueError(f"{len(values)} value arrays for {len(self.systems)} systems")
+            for i, (v, (_, cp, *_r)) in enumerate(zip(values, self.systems)):
+                if np.size(v) != int(cp[-1] - 1):
+                    raise ValueError(f"system {i}: {np.size(v)} values for a pattern of "
+                                     f"{int(cp[-1] - 1)} entries")
         nz = self.nzval if values is None else np.concatenate(
-            [np.asarray(v, dtype=np.float64) for v in values])
+            [np.asarray(v, dtype=np.float64).ravel() for v in values])
         nz = np.ascontiguousarray(nz, dtype=np.float64)
+        if nz.shape[0] != self.nnz:
+            raise ValueError("value count does not match the batched pattern")
         check(self._dev.L.spkhip_inmatrix(self._dev.h, pd(nz)), "spkhip_inmatrix")
 
     def factor(self, precision=FP64):
@@ -147,8 +156,8 @@ class BatchedSolver:
         if refine:
             rc = L.spkhip_solve_refine(self._dev.h, B.shape[1], pd(B), self.n, float(tol),
                                        int(max_iter))
-            if rc not in (0, -5):
-                check(rc, "spkhip_solve_refine")
+            _lib.check_refine(rc, "spkhip_solve_refine",
+                              self.stats().backward_error if rc == -5 else 0.0)
         else:
             check(L.spkhip_solve(self._dev.h, B.shape[1], pd(B), self.n), "spkhip_solve")
         if stacked:

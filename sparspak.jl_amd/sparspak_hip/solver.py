@@ -164,8 +164,9 @@ class _SparseBase:
         if refine:
             rc = self._dev.L.spkhip_solve_refine(self._dev.h, nrhs, pd(B), self.n,
                                                  float(tol), int(max_iter))
-            if rc not in (0, -5):
-                check(rc, "spkhip_solve_refine")
+            self.refine_rc = int(rc)
+            _lib.check_refine(rc, "spkhip_solve_refine",
+                              self.stats().backward_error if rc == -5 else 0.0)
         else:
             check(self._dev.L.spkhip_solve(self._dev.h, nrhs, pd(B), self.n), "spkhip_solve")
         if vec:

@@ -122,8 +122,8 @@ def test_gpu_solve_backward_error(name):
     eta_o = max(backward_error(A, orc.solve(s.slvr.sym, ol, ou, op, B[:, k].copy()), B[:, k])
                 for k in range(B.shape[1]))
     eta = backward_error(A, X, B)
-    _say(f"  eta gpu {eta.max():.2e} oracle {eta_o:.2e}")
-    assert np.all(eta <= max(1e-15, 8 * eta_o))
+    _say(f"  eta gpu {eta.max():.2e} oracle {eta_o:.2e} ratio {eta.max() / eta_o:.2f}")
+    assert np.all(eta <= max(1e-15, 2 * eta_o))
     xo = orc.solve(s.slvr.sym, ol, ou, op, B[:, 0].copy())
     kappa_proxy = max(1.0, eta_o / 1e-16)
     assert np.max(np.abs(X[:, 0] - xo)) <= 1e-12 * kappa_proxy * np.max(np.abs(xo))
