@@ -5,7 +5,9 @@ Loading fails loudly when it is missing: there is no CPU fallback for the
 numeric phase.
 """
 import ctypes
+import importlib.util
 import os
+import sys
 import warnings
 
 import numpy as np
@@ -114,6 +116,31 @@ _SIGS = {
 _lib = None
 
 
+def _share_torch_hip_runtime():
+    """One HIP runtime per process. PyTorch for ROCm bundles its own
+    libamdhip64 (and ROCr), which its libraries reach under the unversioned
+    name libamdhip64.so; this library asks for libamdhip64.so.7. If this
+    library is loaded first, the dynamic linker maps /opt/rocm's runtime, a
+    later `import torch` maps torch's copy beside it, and the second runtime
+    to initialise finds no GPU. When torch is installed, its runtime is
+    therefore mapped first (by path, without importing torch): its SONAME is
+    libamdhip64.so.7, so this library binds to it, and torch later finds the
+    same file already mapped."""
+    if "torch" in sys.modules:
+        return
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    if spec is None or not spec.submodule_search_locations:
+        return
+    for d in spec.submodule_search_locations:
+        p = os.path.join(d, "lib", "libamdhip64.so")
+        if os.path.exists(p):
+            ctypes.CDLL(p, mode=ctypes.RTLD_GLOBAL)
+            return
+
+
 def lib():
     """Load libsparspak_hip.so once; raise if it has not been built."""
     global _lib
@@ -122,6 +149,7 @@ def lib():
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with "
                 "`python sparspak.jl_amd/build.py` (hipcc, gfx950)")
+        _share_torch_hip_runtime()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)
