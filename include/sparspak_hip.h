@@ -181,6 +181,23 @@ int64_t spkhip_solve(spkhip_t h, int64_t nrhs, double* b, int64_t ldb);
 /* Same with device-resident B (double, column-major). */
 int64_t spkhip_solve_device(spkhip_t h, int64_t nrhs, double* d_b,
                             int64_t ldb);
+/* No counterpart in the reference. spkhip_solve_device for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): this call makes no
+ * hipMalloc/hipFree and no other allocation. Same kernels and results as
+ * spkhip_solve_device (bit-identical), launched directly (never replayed
+ * from a captured graph). Its scratch (the permuted right-hand sides and
+ * the per-front solve vectors) is a 64 MiB array the library holds on each
+ * GPU it runs on until the process ends; it needs
+ *   round_up(n * nrhs * s, 256) + vec * nrhs * s  bytes
+ * (s = 8 for an SPKHIP_FP64 factor, 4 otherwise; vec = the handle's solve
+ * vector length, about the number of row indices of the fronts) and returns
+ * -2 when that exceeds 64 MiB (or when the factor's solve data was never
+ * sized, which spkhip_factor does). Calls on the same GPU share that array:
+ * calls that overlap in time on one GPU (other threads, other handles) race
+ * and must be serialised by the caller. */
+int64_t spkhip_solve_device_noalloc(spkhip_t h, int64_t nrhs, double* d_b,
+                                    int64_t ldb);
 
 /* Solve with iterative refinement: the residual r = b - A x and the update
  * x += d run in double on the device, the correction solve uses the last

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -110,6 +111,32 @@ struct LevelPlan {
 
 }  // namespace
 
+// Device memory of the allocation-free solve (spkhip_solve_device_noalloc):
+// the permuted right-hand sides followed by the per-front vector arena.
+// Every GPU the library runs on holds its own copy until the process ends.
+// Host code uses only the address hipGetSymbolAddress returns for it on each
+// device (scratch_address), never the array's name.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_addr;  // by device ordinal
+
+// The address of `scratch` on `device`, which must be current; looked up
+// once per device.
+unsigned char* scratch_address(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((size_t)device >= g_scratch_addr.size()) g_scratch_addr.resize((size_t)device + 1, nullptr);
+  if (!g_scratch_addr[device]) {
+    void* p = nullptr;
+    hipck(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_addr[device] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch_addr[device];
+}
+}  // namespace
+
 struct spkhip_handle {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -180,6 +207,8 @@ struct spkhip_handle {
   // for before they read caller buffers (nullptr: the legacy default stream)
   hipStream_t caller_stream = nullptr;
   hipEvent_t ev_caller = nullptr;
+  // this device's address of the allocation-free solve's scratch
+  unsigned char* scratch = nullptr;
   int refine_iters = 0;
   double backward_error = 0;
   // timing
@@ -730,7 +759,6 @@ void sync(spkhip_handle* h) { hipck(hipStreamSynchronize(h->stream)); }
 // stream (the legacy default stream unless spkhip_set_caller_stream named
 // another), without a host synchronisation.
 void wait_caller(spkhip_handle* h) {
-  if (!h->ev_caller) hipck(hipEventCreateWithFlags(&h->ev_caller, hipEventDisableTiming));
   hipck(hipEventRecord(h->ev_caller, h->caller_stream));
   hipck(hipStreamWaitEvent(h->stream, h->ev_caller, 0));
 }
@@ -1001,13 +1029,26 @@ int small_fronts_per_wg(int maxL) {
   return (int)std::max<size_t>(1, std::min<size_t>(4, 150 * 1024 / per));
 }
 
+// Device storage of the per-block solve data, sized at factor time so that
+// a solve allocates nothing for it.
+template <typename T>
+void size_solve_data(spkhip_handle* h) {
+  if (h->pdesc.empty()) return;
+  Bufs<T>::prep(h).ensure((size_t)h->prep_units * 4);
+  h->d_prepi.ensure(h->pdesc.size() * spk::kPrepInts);
+}
+template <typename T>
+bool solve_data_sized(spkhip_handle* h) {
+  return h->pdesc.empty() || (Bufs<T>::prep(h).n >= (size_t)h->prep_units * 4 &&
+                              h->d_prepi.n >= h->pdesc.size() * spk::kPrepInts);
+}
+
 // The per-block solve data of the current factor (once per factorisation).
 template <typename T>
 void solve_prep(spkhip_handle* h) {
   if (h->prepped || h->pdesc.empty()) return;
   DevBuf<T>& prep = Bufs<T>::prep(h);
-  prep.ensure((size_t)h->prep_units * 4);
-  h->d_prepi.ensure(h->pdesc.size() * spk::kPrepInts);
+  size_solve_data<T>(h);
   Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
   hipLaunchKernelGGL(spk::solve_prep_kernel<T>, dim3((unsigned)h->pdesc.size()), dim3(kWG), 0,
                      h->stream, h->ds, h->d_pdesc.p, Bufs<T>::lnz(h).p, Bufs<T>::unz(h).p,
@@ -1016,12 +1057,11 @@ void solve_prep(spkhip_handle* h) {
   h->prepped = true;
 }
 
+// vecp: the per-front vector arena, h->vec_elems * nrhs elements.
 template <typename T, int MAXW, int GW>
-void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bwd) {
+void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bwd, T* vecp) {
   hipStream_t st = h->stream;
   set_lds_attrs<T>();
-  DevBuf<T>& vec = Bufs<T>::vec(h);
-  vec.ensure((size_t)std::max<i64>(h->vec_elems * nrhs, 1));
   const T* lnz = Bufs<T>::lnz(h).p;
   const T* unz = Bufs<T>::unz(h).p;
   DevBuf<T>& prep = Bufs<T>::prep(h);
@@ -1037,7 +1077,7 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
                            dim3((lp.small_cnt + fpw - 1) / fpw), dim3(kWG),
                            (size_t)fpw * (MAXW + 1) * lp.small_maxL * sizeof(T), st, h->ds,
                            h->d_small.p + lp.small_off, lp.small_cnt, fpw, lnz, h->d_ipvt.p,
-                           rhs, ldr, nrhs, vec.p, lp.small_maxL);
+                           rhs, ldr, nrhs, vecp, lp.small_maxL);
         hipck(hipGetLastError());
       }
       if (lp.large_cnt == 0) continue;
@@ -1047,7 +1087,7 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
                            dim3((lp.items_cnt + kWG - 1) / kWG), dim3(kWG), 0, st, h->ds,
                            h->d_it_front.p + lp.items_off, h->d_it_col.p + lp.items_off,
                            h->d_it_ea.p + lp.items_off, h->d_ea_child.p, h->d_ea_t.p, rhs, ldr,
-                           nrhs, vec.p, lp.items_cnt);
+                           nrhs, vecp, lp.items_cnt);
         hipck(hipGetLastError());
       }
       for (const LevelPlan::SolveStep& S : lp.fsteps) {
@@ -1056,13 +1096,13 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
           hipLaunchKernelGGL((spk::fwd_chain_kernel<T>), dim3(S.chain_cnt), dim3(kWG),
                              (size_t)spk::kPanelElems * sizeof(T), st, h->ds,
                              h->d_sdesc.p + S.chain_off, prep.p, h->d_prepi.p, rhs, ldr, nrhs,
-                             vec.p);
+                             vecp);
           hipck(hipGetLastError());
         }
         if (S.gemv_cnt > 0) {
           Prof pr(h, SPKHIP_K_FWD_GEMV, 2.0 * S.elems * nrhs, S.elems * sizeof(T));
           hipLaunchKernelGGL(spk::fwd_bgemv_kernel<T>, dim3(S.gemv_cnt), dim3(kWG), 0, st, h->ds,
-                             h->d_sdesc.p + S.gemv_off, lnz, nrhs, vec.p);
+                             h->d_sdesc.p + S.gemv_off, lnz, nrhs, vecp);
           hipck(hipGetLastError());
         }
       }
@@ -1077,20 +1117,20 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
         hipLaunchKernelGGL(spk::bwd_gather_items_kernel<T>,
                            dim3((lp.items_cnt + kWG - 1) / kWG), dim3(kWG), 0, st, h->ds,
                            h->d_it_front.p + lp.items_off, h->d_it_col.p + lp.items_off, rhs,
-                           ldr, nrhs, vec.p, lp.items_cnt);
+                           ldr, nrhs, vecp, lp.items_cnt);
         hipck(hipGetLastError());
       }
       for (const LevelPlan::SolveStep& S : lp.bsteps) {
         if (S.gemv_cnt > 0) {
           Prof pr(h, SPKHIP_K_BWD_GEMV, 2.0 * S.elems * nrhs, S.elems * sizeof(T));
           hipLaunchKernelGGL(spk::bwd_bgemv_kernel<T>, dim3(S.gemv_cnt), dim3(kWG), 0, st, h->ds,
-                             h->d_sdesc.p + S.gemv_off, unz, nrhs, vec.p);
+                             h->d_sdesc.p + S.gemv_off, unz, nrhs, vecp);
           hipck(hipGetLastError());
         }
         Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);
         hipLaunchKernelGGL((spk::bwd_chain_kernel<T>), dim3(S.chain_cnt), dim3(kWG),
                            (size_t)(spk::kPanelElems + 128) * sizeof(T), st, h->ds,
-                           h->d_sdesc.p + S.chain_off, prep.p, rhs, ldr, nrhs, vec.p);
+                           h->d_sdesc.p + S.chain_off, prep.p, rhs, ldr, nrhs, vecp);
         hipck(hipGetLastError());
       }
       if (lp.small_cnt > 0) {
@@ -1107,11 +1147,12 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
 }
 
 template <typename T>
-void solve_levels_direct(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bwd) {
+void solve_levels_direct(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bwd,
+                         T* vecp) {
   if (h->kind == SPKHIP_KIND_LU)
-    solve_levels_w<T, 64, kGW>(h, rhs, ldr, nrhs, fwd, bwd);
+    solve_levels_w<T, 64, kGW>(h, rhs, ldr, nrhs, fwd, bwd, vecp);
   else
-    solve_levels_w<T, 96, 96>(h, rhs, ldr, nrhs, fwd, bwd);
+    solve_levels_w<T, 96, 96>(h, rhs, ldr, nrhs, fwd, bwd, vecp);
 }
 
 // Both sweeps replay a captured graph of their launches when nothing they
@@ -1123,15 +1164,15 @@ template <typename T>
 void solve_levels(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bwd) {
   const char* ge = std::getenv("SPKHIP_SOLVE_GRAPH");
   const bool use_graph = !(ge && ge[0] == '0');
+  DevBuf<T>& vec = Bufs<T>::vec(h);
+  vec.ensure((size_t)std::max<i64>(h->vec_elems * nrhs, 1));
   if (!use_graph || h->prof || !(fwd && bwd)) {
-    solve_levels_direct<T>(h, rhs, ldr, nrhs, fwd, bwd);
+    solve_levels_direct<T>(h, rhs, ldr, nrhs, fwd, bwd, vec.p);
     return;
   }
   set_lds_attrs<T>();
   small_fronts_per_wg<T, 64>(1);
   small_fronts_per_wg<T, 96>(1);
-  DevBuf<T>& vec = Bufs<T>::vec(h);
-  vec.ensure((size_t)std::max<i64>(h->vec_elems * nrhs, 1));
   solve_prep<T>(h);
   const std::vector<const void*> key = {
       (const void*)rhs, (const void*)(intptr_t)ldr, (const void*)(intptr_t)nrhs,
@@ -1140,7 +1181,7 @@ void solve_levels(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bw
   if (!h->g_exec || h->g_key != key) {
     if (h->g_seen != key) {
       h->g_seen = key;
-      solve_levels_direct<T>(h, rhs, ldr, nrhs, true, true);
+      solve_levels_direct<T>(h, rhs, ldr, nrhs, true, true, vec.p);
       return;
     }
     if (h->g_exec) {
@@ -1149,7 +1190,7 @@ void solve_levels(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bw
     }
     hipGraph_t g = nullptr;
     hipck(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    solve_levels_direct<T>(h, rhs, ldr, nrhs, true, true);
+    solve_levels_direct<T>(h, rhs, ldr, nrhs, true, true, vec.p);
     hipck(hipStreamEndCapture(h->stream, &g));
     hipck(hipGraphInstantiate(&h->g_exec, g, nullptr, nullptr, 0));
     hipck(hipGraphDestroy(g));
@@ -1170,6 +1211,19 @@ void solve_device(spkhip_handle* h, double* d_b, i64 ldb, int nrhs) {
   solve_levels<T>(h, rhs.p, h->n, nrhs, true, true);
   hipLaunchKernelGGL((spk::permute_kernel<T, double>), dim3(grid_for(h->n)), dim3(kWG), 0, st,
                      rhs.p, h->n, d_b, ldb, h->d_invp.p, h->n, nrhs);
+  hipck(hipGetLastError());
+}
+
+// The same solve with the permuted right-hand sides (n x nrhs) and the
+// vector arena in the given device memory, launched directly (no graph).
+template <typename T>
+void solve_device_in(spkhip_handle* h, double* d_b, i64 ldb, int nrhs, T* rhs, T* vecp) {
+  hipStream_t st = h->stream;
+  hipLaunchKernelGGL((spk::permute_kernel<double, T>), dim3(grid_for(h->n)), dim3(kWG), 0, st,
+                     d_b, ldb, rhs, h->n, h->d_perm.p, h->n, nrhs);
+  solve_levels_direct<T>(h, rhs, h->n, nrhs, true, true, vecp);
+  hipLaunchKernelGGL((spk::permute_kernel<T, double>), dim3(grid_for(h->n)), dim3(kWG), 0, st,
+                     rhs, h->n, d_b, ldb, h->d_invp.p, h->n, nrhs);
   hipck(hipGetLastError());
 }
 
@@ -1553,6 +1607,8 @@ int64_t spkhip_create(spkhip_t* out, int64_t device) {
       hipck(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
       hipck(hipEventCreate(&h->ev0));
       hipck(hipEventCreate(&h->ev1));
+      hipck(hipEventCreateWithFlags(&h->ev_caller, hipEventDisableTiming));
+      h->scratch = scratch_address(h->device);
     } catch (...) {
       delete h;
       throw;
@@ -1727,9 +1783,11 @@ int64_t spkhip_factor(spkhip_t h, int64_t precision) {
     if (precision == SPKHIP_FP64) {
       run_assemble<double>(h);
       run_factor<double>(h);
+      size_solve_data<double>(h);
     } else {
       run_assemble<float>(h);
       run_factor<float>(h);
+      size_solve_data<float>(h);
     }
     h->lowp = 0;
     hipck(hipEventRecord(h->ev1, h->stream));
@@ -1765,6 +1823,46 @@ int64_t spkhip_solve_device(spkhip_t h, int64_t nrhs, double* d_b, int64_t ldb) 
     hipck(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->t_solve = ms;
     prof_collect(h);
+    return 0;
+  });
+}
+
+int64_t spkhip_solve_device_noalloc(spkhip_t h, int64_t nrhs, double* d_b, int64_t ldb) {
+  if (guard(h)) return -4;
+  if (!h->factored) return -3;
+  if (nrhs < 0 || ldb < h->n || !d_b) return -4;
+  if (nrhs == 0) return 0;
+  if (!h->scratch) return -4;
+  const bool f64 = h->precision == SPKHIP_FP64;
+  const size_t es = f64 ? sizeof(double) : sizeof(float);
+  const size_t per_rhs = ((size_t)h->n + (size_t)std::max<i64>(h->vec_elems, 1)) * es;
+  if ((size_t)nrhs > kScratchBytes / per_rhs) return -2;
+  const size_t rhs_bytes = ((size_t)h->n * nrhs * es + 255) & ~size_t(255);
+  const size_t vec_bytes = (size_t)std::max<i64>(h->vec_elems, 1) * nrhs * es;
+  if (rhs_bytes + vec_bytes > kScratchBytes) return -2;
+  if (!(f64 ? solve_data_sized<double>(h) : solve_data_sized<float>(h))) return -2;
+  return wrap([&]() -> int64_t {
+    hipck(hipSetDevice(h->device));
+    // no profile events here (the profile creates events on first use)
+    struct ProfOff {
+      spkhip_handle* h;
+      bool was;
+      ~ProfOff() { h->prof = was; }
+    } po{h, h->prof};
+    h->prof = false;
+    wait_caller(h);
+    hipck(hipEventRecord(h->ev0, h->stream));
+    if (f64)
+      solve_device_in<double>(h, d_b, ldb, (int)nrhs, reinterpret_cast<double*>(h->scratch),
+                              reinterpret_cast<double*>(h->scratch + rhs_bytes));
+    else
+      solve_device_in<float>(h, d_b, ldb, (int)nrhs, reinterpret_cast<float*>(h->scratch),
+                             reinterpret_cast<float*>(h->scratch + rhs_bytes));
+    hipck(hipEventRecord(h->ev1, h->stream));
+    hipck(hipEventSynchronize(h->ev1));
+    float ms = 0;
+    hipck(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->t_solve = ms;
     return 0;
   });
 }

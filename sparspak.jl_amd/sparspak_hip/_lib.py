@@ -81,6 +81,7 @@ _SIGS = {
     "spkhip_factor": (i64, [vp, i64]),
     "spkhip_solve": (i64, [vp, i64, dp, i64]),
     "spkhip_solve_device": (i64, [vp, i64, vp, i64]),
+    "spkhip_solve_device_noalloc": (i64, [vp, i64, vp, i64]),
     "spkhip_solve_refine": (i64, [vp, i64, dp, i64, ctypes.c_double, i64]),
     "spkhip_solve_refine_device": (i64, [vp, i64, vp, i64, ctypes.c_double,
                                          i64]),
