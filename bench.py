@@ -21,9 +21,15 @@ the backward-error certificate (and refinement for fp32). The host symbolic
 phase (MMD, etree, supernodes) runs once before the timed region, as in
 sparspaklu! refactorisation, and so does the autotuner that picks each path's
 plan knobs (front split, block width, update depth) by timing its factor
-(--no-autotune keeps the defaults). Prints one JSON line.
+(--no-autotune keeps the defaults).
 
-    python bench.py [--steps K] [--warmup W] [--grid 64] [--cpu-grid 36]
+Beside the headline, the same steps run on the mixed-precision
+configuration: Q1 linear elasticity on a 40^3 element mesh (3 dof per node,
+n = 206,763, z = 0 face clamped as identity rows, body load N(0, 1)), in
+fp64, fp32_bf16x3 and fp32_bf16 (key "elasticity" of the JSON line, with the
+best mixed path's factor+solve speed-up over fp64). Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--grid 64] [--cpu-grid 36] [--elastic 40]
 """
 import argparse
 import ctypes
@@ -68,6 +74,10 @@ def main():
     ap.add_argument("--cpu-grid", type=int, default=36)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--paths", default="fp64,fp32,fp32_bf16,fp32_bf16x3")
+    ap.add_argument("--elastic", type=int, default=40,
+                    help="elements per side of the elasticity configuration measured "
+                         "beside the headline (0: skip)")
+    ap.add_argument("--elastic-paths", default="fp64,fp32_bf16x3,fp32_bf16")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default plan knobs instead of autotuning each path")
     ap.add_argument("--tuning", default=None,
@@ -92,11 +102,8 @@ def main():
         torch.cuda.synchronize()
         log(f"copied {src.numel() * 8} bytes three times")
         return
-    from sparspak_hip import SparseSolver, findorder, symbolicfactor
-    from sparspak_hip import _lib
-    from sparspak_hip.gallery import convdiff3d, uniform
+    from sparspak_hip.gallery import convdiff3d, elasticity3d, normal, uniform
 
-    dev = torch.device("cuda:0")
     g = args.grid
     t0 = time.time()
     A = convdiff3d(g)
@@ -104,104 +111,38 @@ def main():
     xtrue = uniform(2, n, -1.0, 1.0)
     b = A @ xtrue
     log(f"matrix convdiff3d {g}^3 n={n} nnz={A.nnz} ({time.time() - t0:.1f}s)")
-    t0 = time.time()
-    s = SparseSolver(A)
-    findorder(s)
-    symbolicfactor(s)
-    t_sym = time.time() - t0
-    sym = s.slvr.sym
-    F = sym.flops()
-    log(f"host symbolic {t_sym:.2f}s nsuper={sym.nsuper} nnz(L)={sym.nnzl} F_LU={F:.4e}")
-    t0 = time.time()
-    s.slvr._upload()
-    dv = s.slvr._dev
-    L = dv.L
-    h = dv.h
-    log(f"upload {time.time() - t0:.2f}s")
+    results, F, t_sym = run_workload(A, b, xtrue, args.paths.split(","), args)
 
-    d_nz = torch.from_numpy(np.ascontiguousarray(A.data, dtype=np.float64)).to(dev)
-    d_b = torch.from_numpy(np.ascontiguousarray(b)).to(dev)
-    d_x = torch.empty_like(d_b)
-    # the library orders its reads of d_nz / d_x after torch's stream
-    _lib.check(L.spkhip_set_caller_stream(
-        h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "set_caller_stream")
-    torch.cuda.synchronize()
-
-    prec = {"fp64": 0, "fp32": 1, "fp32_bf16": 2, "fp32_fp16": 3, "fp32_bf16x3": 4}
-
-    def step(path):
-        _lib.check(L.spkhip_inmatrix_device(h, ctypes.c_void_p(d_nz.data_ptr())), "inmatrix")
-        _lib.check(L.spkhip_factor(h, prec[path]), "factor")
-        d_x.copy_(d_b)
-        rc = L.spkhip_solve_refine_device(h, 1, ctypes.c_void_p(d_x.data_ptr()), n, 1e-14, 10)
-        if rc not in (0, -5):
-            _lib.check(rc, "solve")
-        st = _lib.Stats()
-        L.spkhip_stats(h, ctypes.byref(st))
-        return st
-
-    results = {}
-    for path in args.paths.split(","):
-        tuning, t_tune = None, 0.0
-        if args.tuning:
-            vals = [int(v) for v in args.tuning.split(",")]
-            tu = _lib.Tuning(*vals, 0.0)
-            _lib.check(L.spkhip_set_tuning(h, ctypes.byref(tu)), "set_tuning")
-            tuning = dict(zip(_lib.TUNING_KNOBS, vals))
-        elif not args.no_autotune:
-            # plan knobs chosen by timing this path's factorisation (setup,
-            # like the symbolic phase; outside the timed steps)
-            t0 = time.perf_counter()
-            _lib.check(L.spkhip_inmatrix_device(h, ctypes.c_void_p(d_nz.data_ptr())), "inmatrix")
-            tu = _lib.Tuning()
-            _lib.check(L.spkhip_autotune(h, prec[path], 2, ctypes.byref(tu)), "autotune")
-            t_tune = time.perf_counter() - t0
-            tuning = {f: getattr(tu, f) for f in _lib.TUNING_KNOBS}
-            log(f"{path}: autotune {t_tune:.1f}s -> {tuning} ({tu.factor_ms:.1f} ms)")
-        for _ in range(args.warmup):
-            step(path)
-        torch.cuda.synchronize()
-        tf, ts = [], []
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            st = step(path)
-            tf.append(st.t_factor_ms)
-            ts.append(st.t_refine_ms)
-        torch.cuda.synchronize()
-        wall = (time.perf_counter() - t0) / args.steps
-        x = d_x.cpu().numpy()
-        r = b - A @ x
-        eta = float(np.max(np.abs(r)) / (abs(A).sum(axis=1).max() * np.max(np.abs(x))
-                                          + np.max(np.abs(b))))
-        fwd = float(np.max(np.abs(x - xtrue)) / np.max(np.abs(xtrue)))
-        # per-kernel profile on one extra, untimed step
-        L.spkhip_profile_enable(h, 1)
-        step(path)
-        kst = {}
-        for k, name in enumerate(_lib.KCLASSES):
-            ks = _lib.KStat()
-            L.spkhip_profile_get(h, k, ctypes.byref(ks))
-            if ks.launches:
-                kst[name] = {"launches": ks.launches, "ms": ks.ms, "flops": ks.flops,
-                             "bytes": ks.bytes}
-        L.spkhip_profile_enable(h, 0)
-        t_factor = float(np.median(tf)) / 1e3
-        t_solve = float(np.median(ts)) / 1e3
-        results[path] = {
-            "factor_s": t_factor, "solve_refine_s": t_solve,
-            "factor_solve_s": t_factor + t_solve, "step_wall_s": wall,
-            "factor_tflops": F / t_factor / 1e12,
-            "refine_iters": int(st.refine_iters), "backward_error": eta,
-            "forward_error_vs_xtrue": fwd, "kernels": kst,
-            "tuning": tuning, "autotune_s": t_tune,
+    # the mixed-precision configuration: Q1 elasticity on an ne^3 mesh,
+    # 3 dof per node, z = 0 face clamped (identity rows kept), body load
+    # N(0, 1) per free dof (splitmix64 seed 3), 0 on the clamped dofs
+    elastic = None
+    if args.elastic > 0:
+        ne = args.elastic
+        t0 = time.time()
+        Ae = elasticity3d(ne)
+        ne_n = Ae.shape[0]
+        be = normal(3, ne_n)
+        be[:3 * (ne + 1) ** 2] = 0.0
+        log(f"matrix elasticity3d {ne}^3 n={ne_n} nnz={Ae.nnz} ({time.time() - t0:.1f}s)")
+        eres, eF, et_sym = run_workload(Ae, be, None, args.elastic_paths.split(","), args)
+        elastic = {
+            "workload": f"Q1 linear elasticity {ne}^3 elements, 3 dof/node, E=1, nu=0.3, "
+                        "z=0 clamped (identity rows), body load N(0,1); LU numeric phase",
+            "n": ne_n, "nnz": int(Ae.nnz), "F_LU": eF, "host_symbolic_s": et_sym,
+            "paths": {p: {k: v for k, v in eres[p].items() if k != "kernels"} for p in eres},
+            "kernels": {p: eres[p]["kernels"] for p in eres},
         }
-        log(f"{path}: factor {t_factor * 1e3:.1f} ms ({F / t_factor / 1e12:.2f} TF/s), "
-            f"solve+ir {t_solve * 1e3:.1f} ms, iters {st.refine_iters}, eta {eta:.2e}, "
-            f"wall/step {wall * 1e3:.1f} ms")
-        for k, v in kst.items():
-            log(f"   {k:16s} {v['launches']:6d} launches {v['ms']:9.3f} ms "
-                f"{v['flops'] / max(v['ms'], 1e-9) / 1e9:9.2f} TF/s "
-                f"{v['bytes'] / max(v['ms'], 1e-9) / 1e6:9.1f} GB/s")
+        if "fp64" in eres:
+            ok = [p for p in eres if p != "fp64" and eres[p]["backward_error"] <= 1e-14]
+            if ok:
+                best = min(ok, key=lambda p: eres[p]["factor_solve_s"])
+                elastic["best_mixed_path"] = best
+                elastic["speedup_vs_fp64"] = (eres["fp64"]["factor_solve_s"]
+                                              / eres[best]["factor_solve_s"])
+                log(f"elasticity: {best} factor+solve {eres[best]['factor_solve_s'] * 1e3:.1f} ms"
+                    f" vs fp64 {eres['fp64']['factor_solve_s'] * 1e3:.1f} ms "
+                    f"({elastic['speedup_vs_fp64']:.2f}x)")
 
     # headline: the fastest path that certifies eta <= 1e-14
     ok = [p for p in results if results[p]["backward_error"] <= 1e-14]
@@ -212,7 +153,13 @@ def main():
     dk = kst[dom]
     compute_bound = dom in ("large_update", "small_front", "large_diag", "large_offdiag")
     prec_name = head
-    if compute_bound and dk["flops"] > 0:
+    # a compute kernel is MFMA-bound only when its algorithmic intensity
+    # (flops per algorithmic HBM byte) times the HBM peak exceeds the matrix
+    # peak of its arithmetic; otherwise its roof is HBM
+    intensity = dk["flops"] / dk["bytes"] if dk["bytes"] > 0 else float("inf")
+    mfma_roof = compute_bound and dk["flops"] > 0 and \
+        kernel_peak(head, dom) <= intensity * HBM_PEAK / 1e3
+    if mfma_roof:
         achieved = dk["flops"] / (dk["ms"] / 1e3) / 1e12
         peak = kernel_peak(head, dom)
         roof = {"bound": "mfma", "achieved": achieved, "peak": peak,
@@ -226,6 +173,10 @@ def main():
                 "frac": achieved / HBM_PEAK, "traffic": None, "kernel": dom,
                 "launches": dk["launches"], "avg_launch_ms": dk["ms"] / dk["launches"],
                 "work_per_launch": dk["bytes"] / dk["launches"]}
+        if dk["flops"] > 0:
+            roof["flops_per_byte"] = intensity
+            roof["achieved_tflops"] = dk["flops"] / (dk["ms"] / 1e3) / 1e12
+            roof["matrix_peak_tflops"] = kernel_peak(head, dom)
 
     # measured HBM bytes per launch of the dominant kernel, from the committed
     # rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same workload (scaled
@@ -260,8 +211,125 @@ def main():
         "kernels": {p: results[p]["kernels"] for p in results},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "elasticity": elastic,
     }
     print(json.dumps(out), flush=True)
+
+
+def run_workload(A, b, xtrue, paths, args):
+    """Symbolic phase and upload once, then per precision path: autotune
+    (outside the timed steps), warmup, the timed steps (inmatrix from
+    device-resident values -> factor -> refined solve of b), and one extra
+    step under the per-launch profile. Returns (results per path, F_LU,
+    host symbolic seconds)."""
+    import torch
+    from sparspak_hip import SparseSolver, findorder, symbolicfactor
+    from sparspak_hip import _lib
+
+    dev = torch.device("cuda:0")
+    n = A.shape[0]
+    t0 = time.time()
+    s = SparseSolver(A)
+    findorder(s)
+    symbolicfactor(s)
+    t_sym = time.time() - t0
+    sym = s.slvr.sym
+    F = sym.flops()
+    log(f"host symbolic {t_sym:.2f}s nsuper={sym.nsuper} nnz(L)={sym.nnzl} F_LU={F:.4e}")
+    t0 = time.time()
+    s.slvr._upload()
+    dv = s.slvr._dev
+    L = dv.L
+    h = dv.h
+    log(f"upload {time.time() - t0:.2f}s")
+
+    d_nz = torch.from_numpy(np.ascontiguousarray(A.data, dtype=np.float64)).to(dev)
+    d_b = torch.from_numpy(np.ascontiguousarray(b)).to(dev)
+    d_x = torch.empty_like(d_b)
+    # the library orders its reads of d_nz / d_x after torch's stream
+    _lib.check(L.spkhip_set_caller_stream(
+        h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "set_caller_stream")
+    torch.cuda.synchronize()
+    anorm = abs(A).sum(axis=1).max()
+
+    prec = {"fp64": 0, "fp32": 1, "fp32_bf16": 2, "fp32_fp16": 3, "fp32_bf16x3": 4}
+
+    def step(path):
+        _lib.check(L.spkhip_inmatrix_device(h, ctypes.c_void_p(d_nz.data_ptr())), "inmatrix")
+        _lib.check(L.spkhip_factor(h, prec[path]), "factor")
+        d_x.copy_(d_b)
+        rc = L.spkhip_solve_refine_device(h, 1, ctypes.c_void_p(d_x.data_ptr()), n, 1e-14, 10)
+        if rc not in (0, -5):
+            _lib.check(rc, "solve")
+        st = _lib.Stats()
+        L.spkhip_stats(h, ctypes.byref(st))
+        return st
+
+    results = {}
+    for path in paths:
+        tuning, t_tune = None, 0.0
+        if args.tuning:
+            vals = [int(v) for v in args.tuning.split(",")]
+            tu = _lib.Tuning(*vals, 0.0)
+            _lib.check(L.spkhip_set_tuning(h, ctypes.byref(tu)), "set_tuning")
+            tuning = dict(zip(_lib.TUNING_KNOBS, vals))
+        elif not args.no_autotune:
+            # plan knobs chosen by timing this path's factorisation (setup,
+            # like the symbolic phase; outside the timed steps)
+            t0 = time.perf_counter()
+            _lib.check(L.spkhip_inmatrix_device(h, ctypes.c_void_p(d_nz.data_ptr())), "inmatrix")
+            tu = _lib.Tuning()
+            _lib.check(L.spkhip_autotune(h, prec[path], 2, ctypes.byref(tu)), "autotune")
+            t_tune = time.perf_counter() - t0
+            tuning = {f: getattr(tu, f) for f in _lib.TUNING_KNOBS}
+            log(f"{path}: autotune {t_tune:.1f}s -> {tuning} ({tu.factor_ms:.1f} ms)")
+        for _ in range(args.warmup):
+            step(path)
+        torch.cuda.synchronize()
+        tf, ts = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st = step(path)
+            tf.append(st.t_factor_ms)
+            ts.append(st.t_refine_ms)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        x = d_x.cpu().numpy()
+        r = b - A @ x
+        eta = float(np.max(np.abs(r)) / (anorm * np.max(np.abs(x)) + np.max(np.abs(b))))
+        fwd = (None if xtrue is None else
+               float(np.max(np.abs(x - xtrue)) / np.max(np.abs(xtrue))))
+        # per-kernel profile on one extra, untimed step
+        L.spkhip_profile_enable(h, 1)
+        step(path)
+        kst = {}
+        for k, name in enumerate(_lib.KCLASSES):
+            ks = _lib.KStat()
+            L.spkhip_profile_get(h, k, ctypes.byref(ks))
+            if ks.launches:
+                kst[name] = {"launches": ks.launches, "ms": ks.ms, "flops": ks.flops,
+                             "bytes": ks.bytes}
+        L.spkhip_profile_enable(h, 0)
+        t_factor = float(np.median(tf)) / 1e3
+        t_solve = float(np.median(ts)) / 1e3
+        results[path] = {
+            "factor_s": t_factor, "solve_refine_s": t_solve,
+            "factor_solve_s": t_factor + t_solve, "step_wall_s": wall,
+            "factor_tflops": F / t_factor / 1e12,
+            "refine_iters": int(st.refine_iters), "backward_error": eta,
+            "forward_error_vs_xtrue": fwd, "kernels": kst,
+            "tuning": tuning, "autotune_s": t_tune,
+        }
+        log(f"{path}: factor {t_factor * 1e3:.1f} ms ({F / t_factor / 1e12:.2f} TF/s), "
+            f"solve+ir {t_solve * 1e3:.1f} ms, iters {st.refine_iters}, eta {eta:.2e}, "
+            f"wall/step {wall * 1e3:.1f} ms")
+        for k, v in kst.items():
+            log(f"   {k:16s} {v['launches']:6d} launches {v['ms']:9.3f} ms "
+                f"{v['flops'] / max(v['ms'], 1e-9) / 1e9:9.2f} TF/s "
+                f"{v['bytes'] / max(v['ms'], 1e-9) / 1e6:9.1f} GB/s")
+    del d_nz, d_b, d_x
+    s.slvr._dev = None
+    return results, F, t_sym
 
 
 TRAFFIC_FILE = "hbm_traffic_convdiff3d64.md"

@@ -264,11 +264,19 @@ int64_t spkhip_ldltfactor_f64(int64_t n, int64_t nsuper, const int64_t* xsuper,
                               const int64_t* snode, const int64_t* xlindx,
                               const int64_t* lindx, const int64_t* xlnz,
                               double* lnz);
+int64_t spkhip_ldltfactor_f32(int64_t n, int64_t nsuper, const int64_t* xsuper,
+                              const int64_t* snode, const int64_t* xlindx,
+                              const int64_t* lindx, const int64_t* xlnz,
+                              float* lnz);
 /* SpkLDLtFactor.jl:266 _ldltsolve! */
 int64_t spkhip_ldltsolve_f64(int64_t nsuper, const int64_t* xsuper,
                              const int64_t* xlindx, const int64_t* lindx,
                              const int64_t* xlnz, const double* lnz,
                              double* rhs);
+int64_t spkhip_ldltsolve_f32(int64_t nsuper, const int64_t* xsuper,
+                             const int64_t* xlindx, const int64_t* lindx,
+                             const int64_t* xlnz, const float* lnz,
+                             float* rhs);
 
 /* Per-kernel-class profile: with profiling on, every launch of the numeric
  * phase is bracketed by HIP events on the handle's stream and its
@@ -303,7 +311,7 @@ int64_t spkhip_profile_get(spkhip_t h, int64_t kclass, spkhip_kstat_t* out);
  *                   (0..64, default 64)
  *   block_cols      widest block of consecutive pieces of an LU front, the
  *                   tile width of the diagonal, off-diagonal and update
- *                   kernels (32..128, default 128; SPD blocks are one piece)
+ *                   kernels (32..128, default 64; SPD blocks are one piece)
  *   upd_group       blocks per wide trailing update, i.e. its depth in
  *                   blocks (1..64, default 8)
  *   small_wave_min  small fronts in a level from which one wave (instead of

@@ -229,7 +229,7 @@ struct spkhip_handle {
   // range gives a correct factor, they trade launches against occupancy)
   size_t upd_group = 8;                // blocks per full trailing update
   int small_lmax = kSmallLmax;         // fronts with L <= this are factored whole in LDS
-  int block_cols = 128;                // widest block of pieces in a large LU front
+  int block_cols = 64;                 // widest block of pieces in a large LU front
   int small_wave_min = 16384;          // small fronts per level from which one wave takes one
   double tuned_ms = 0;                 // best factor time of the last autotune
   int lowp = 0;  // large-front updates: 1 bf16, 2 scaled fp16, 3 split-bf16 operands
@@ -879,6 +879,10 @@ void run_factor(spkhip_handle* h) {
   const bool two = !h->prof;
   hipStream_t st2 = two ? h->stream2 : st;
   h->xev_used = 0;
+  // LU blocks of at most 64 columns: the diagonal region in registers
+  // (SPKHIP_DIAG_SLAB=0 keeps the LDS form)
+  const char* ds_env = std::getenv("SPKHIP_DIAG_SLAB");
+  const bool slab = lu && h->block_cols <= 64 && !(ds_env && ds_env[0] == '0');
   for (const LevelPlan& lp : h->levels) {
     h->cur_level = (int)(&lp - h->levels.data());
     h->cur_grid = lp.small_cnt;
@@ -930,7 +934,11 @@ void run_factor(spkhip_handle* h) {
         {
           h->cur_grid = G.diag.wg;
           Prof pr(h, SPKHIP_K_DIAG, G.diag.flops, G.diag.elems * sizeof(T));
-          if (lu)
+          if (slab)
+            hipLaunchKernelGGL((spk::blk_diag_slab_kernel<T>), dim3(G.diag.wg), dim3(kWG), 0, st,
+                               h->ds, h->d_bdesc.p + G.diag.off, arena.p, h->d_ipvt.p,
+                               h->d_fail.p);
+          else if (lu)
             hipLaunchKernelGGL((spk::blk_diag_kernel<T, kNqLU>), dim3(G.diag.wg), dim3(kWG),
                                (size_t)G.diag.lds * sizeof(T), st, h->ds,
                                h->d_bdesc.p + G.diag.off, arena.p, h->d_ipvt.p, h->d_fail.p);
@@ -2283,12 +2291,16 @@ int64_t spkhip_luusolve_f32(int64_t n, int64_t nsuper, const int64_t* xsuper,
   return trisolve_stateless<float>(nsuper, xsuper, xlindx, lindx, xlnz, lnz, xunz, unz, nullptr,
                                    rhs, false, true, SPKHIP_KIND_LU);
 }
-int64_t spkhip_ldltsolve_f64(int64_t nsuper, const int64_t* xsuper, const int64_t* xlindx,
-                             const int64_t* lindx, const int64_t* xlnz, const double* lnz,
-                             double* rhs) {
+}  // extern "C"
+
+namespace {
+// _ldltsolve! on the device: the LDL^T panels (D on the diagonal, unit L
+// below) are turned into the LU form the solve kernels read, U11 = D L11^T
+// in the upper block and U12^T = L21 D in unz, with identity pivots.
+template <typename T>
+int64_t ldltsolve_stateless(int64_t nsuper, const int64_t* xsuper, const int64_t* xlindx,
+                            const int64_t* lindx, const int64_t* xlnz, const T* lnz, T* rhs) {
   if (nsuper <= 0 || !xsuper || !xlindx || !lindx || !xlnz || !lnz || !rhs) return -4;
-  // LDL^T panels (D on the diagonal, unit L below) -> the LU form the solve
-  // kernels read: U11 = D L11^T in the upper block, U12^T = L21 D in unz.
   const i64 n = xsuper[nsuper] - 1;
   std::vector<i64> xunz(n + 1);
   i64 up = 1;
@@ -2300,13 +2312,13 @@ int64_t spkhip_ldltsolve_f64(int64_t nsuper, const int64_t* xsuper, const int64_
     }
   }
   xunz[n] = up;
-  std::vector<double> l2(lnz, lnz + (xlnz[n] - 1)), u2(std::max<i64>(up - 1, 1));
+  std::vector<T> l2(lnz, lnz + (xlnz[n] - 1)), u2(std::max<i64>(up - 1, 1));
   for (i64 J = 0; J < nsuper; ++J) {
     i64 fj = xsuper[J] - 1, w = xsuper[J + 1] - xsuper[J], jl = xlindx[J + 1] - xlindx[J];
-    double* p = l2.data() + (xlnz[fj] - 1);
-    double* u = u2.data() + (xunz[fj] - 1);
+    T* p = l2.data() + (xlnz[fj] - 1);
+    T* u = u2.data() + (xunz[fj] - 1);
     for (i64 c = 0; c < w; ++c) {
-      const double d = p[c + c * jl];
+      const T d = p[c + c * jl];
       for (i64 k = c + 1; k < w; ++k) p[c + k * jl] = d * p[k + c * jl];
       for (i64 r = w; r < jl; ++r) u[(r - w) + c * (jl - w)] = d * p[r + c * jl];
     }
@@ -2314,8 +2326,22 @@ int64_t spkhip_ldltsolve_f64(int64_t nsuper, const int64_t* xsuper, const int64_
   std::vector<i64> piv(n);
   for (i64 J = 0; J < nsuper; ++J)
     for (i64 c = xsuper[J]; c < xsuper[J + 1]; ++c) piv[c - 1] = c - xsuper[J] + 1;
-  return trisolve_stateless<double>(nsuper, xsuper, xlindx, lindx, xlnz, l2.data(), xunz.data(),
-                                    u2.data(), piv.data(), rhs, true, true, SPKHIP_KIND_LU);
+  return trisolve_stateless<T>(nsuper, xsuper, xlindx, lindx, xlnz, l2.data(), xunz.data(),
+                               u2.data(), piv.data(), rhs, true, true, SPKHIP_KIND_LU);
+}
+}  // namespace
+
+extern "C" {
+
+int64_t spkhip_ldltsolve_f64(int64_t nsuper, const int64_t* xsuper, const int64_t* xlindx,
+                             const int64_t* lindx, const int64_t* xlnz, const double* lnz,
+                             double* rhs) {
+  return ldltsolve_stateless<double>(nsuper, xsuper, xlindx, lindx, xlnz, lnz, rhs);
+}
+int64_t spkhip_ldltsolve_f32(int64_t nsuper, const int64_t* xsuper, const int64_t* xlindx,
+                             const int64_t* lindx, const int64_t* xlnz, const float* lnz,
+                             float* rhs) {
+  return ldltsolve_stateless<float>(nsuper, xsuper, xlindx, lindx, xlnz, lnz, rhs);
 }
 
 int64_t spkhip_ldltfactor_f64(int64_t n, int64_t nsuper, const int64_t* xsuper,
@@ -2323,6 +2349,12 @@ int64_t spkhip_ldltfactor_f64(int64_t n, int64_t nsuper, const int64_t* xsuper,
                               const int64_t* xlnz, double* lnz) {
   return lufactor_stateless<double>(n, nsuper, xsuper, snode, xlindx, lindx, xlnz, lnz, nullptr,
                                     nullptr, nullptr, SPKHIP_KIND_SPD);
+}
+int64_t spkhip_ldltfactor_f32(int64_t n, int64_t nsuper, const int64_t* xsuper,
+                              const int64_t* snode, const int64_t* xlindx, const int64_t* lindx,
+                              const int64_t* xlnz, float* lnz) {
+  return lufactor_stateless<float>(n, nsuper, xsuper, snode, xlindx, lindx, xlnz, lnz, nullptr,
+                                   nullptr, nullptr, SPKHIP_KIND_SPD);
 }
 
 }  // extern "C"

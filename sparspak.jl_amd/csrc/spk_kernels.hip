@@ -1111,6 +1111,213 @@ blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ aren
   store_tile<T, 1, LD>(Fr, L, bw, bw, R);
 }
 
+// Diagonal region of one LU block of at most 64 columns per workgroup, held
+// in registers: wave w keeps columns [16 w, 16 w + 16) of the region, one
+// row per lane. The region is factored right-looking, one column at a time,
+// with the reference's pivoting confined to each piece's rows
+// (SpkLUFactor.jl:230-240, GenericBlasLapackFragments.jl:56-100). Step k,
+// a column of piece p, takes as pivot the row of p with the largest |a|
+// among those whose position is k or later (ties: the smallest position; no
+// comparable candidate: the row at position k), scales the rows below it by
+// the reciprocal of the pivot (not when it is zero) and subtracts their
+// multiple of the pivot row from the columns right of k. The rows below are
+// p's rows at later positions and every row of the later pieces: for p's
+// rows this is the piece's pivoted LU and, right of the piece, its
+// unit-lower solve U12 = L11^-1 P A12; for the later pieces' rows it is
+// L21 = A21 U11^-1 and the update of their part of the region. The
+// triangular parts receive their terms one at a time in ascending k, in the
+// order of the left-looking forms; the later pieces' part right of the
+// piece sums the piece's products from zero and subtracts the sum once,
+// as the reference's update of one supernode by another does (the sum is
+// kept in the entry itself, the entry's value aside until the piece's last
+// column). Rows never move while factoring: each lane tracks the position
+// its row holds. The wave owning column k publishes the step (multipliers,
+// pivot lane, pivot position) through LDS; the other waves apply published
+// steps to their columns as they come, without barriers (a wave only ever
+// waits for steps of columns left of its own, which the waves to its left
+// publish without waiting on it). At the end each row is stored at its
+// final position in the columns of its piece and right of it, and at its
+// own row in the columns of earlier pieces, which the reference never
+// interchanges.
+constexpr int kMaxBlkPieces = 8;  // most pieces in one block
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+blk_diag_slab_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ arena,
+                     int64_t* __restrict__ ipvt, unsigned long long* __restrict__ first_fail) {
+  constexpr int SW = 16;                  // columns per wave
+  __shared__ T sl[64][64];                // step k: the multiplier of each row
+  __shared__ T sout[64][64];              // finished column c, rows in lane order
+  __shared__ T ssv[64][64];               // entries set aside while a piece's sum forms
+  __shared__ int smeta[64];               // step k: pivot lane | position << 8 | nonzero << 16
+  __shared__ int spo[kMaxBlkPieces + 1];  // piece offsets in the region; spo[np] = bw
+  __shared__ int scp[64];                 // piece of each region column
+  __shared__ int spos[64];                // final position of each row
+  __shared__ int spiv[64];                // step k: pivot position, 1-based in its piece
+  __shared__ int sdone;                   // steps published
+  const int lane = threadIdx.x % 64, w = wave_id();
+  const BlkDesc d = desc[blockIdx.x];
+  const int f = d.front, L = s.fL[f];
+  const int64_t fc0 = s.fc0[f];
+  const int c0 = d.c0, bw = d.c1 - d.c0, np = d.np;
+  T* R = arena + s.foff[f] + c0 + (int64_t)c0 * L;
+  if ((int)threadIdx.x <= np) {
+    int o = bw;
+    if ((int)threadIdx.x < np) {
+      int64_t pc0, nq;
+      piece_cols(s, d.piece + threadIdx.x, pc0, nq);
+      o = (int)(pc0 - fc0) - c0;
+    }
+    spo[threadIdx.x] = o;
+  }
+  if (threadIdx.x == 0) sdone = 0;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    int p = 0;
+    while (p + 1 < np && (int)threadIdx.x >= spo[p + 1]) ++p;
+    scp[threadIdx.x] = p;
+  }
+  __syncthreads();
+  // this lane's row: its piece pr (np for the rows past the region)
+  const int pr = lane < bw ? scp[lane] : np;
+  const int rlo = pr < np ? spo[pr] : 0;
+  int pos = lane;
+  const int cb = SW * w;
+  // a[i] holds column c + i, c = cb while the earlier waves' steps are
+  // applied, then the column of the current step: the slab shifts left by
+  // one column after each of this wave's steps, so that the pivot column is
+  // always a[0] and the loop body indexes registers statically
+  T a[SW];
+#pragma unroll
+  for (int i = 0; i < SW; ++i)
+    a[i] = (lane < bw && cb + i < bw) ? R[lane + (int64_t)(cb + i) * L] : T(0);
+  // first step of piece pk (current column c): the later pieces' entries
+  // right of the piece are set aside and the piece's products summed from
+  // zero in their place; last step: the entries take the sum off. Called
+  // under wave-uniform branches that hold a compiler barrier, so that they
+  // stay branches instead of selects executed at every step.
+  auto piece_begin = [&](int pk, int c) {
+    __asm__ volatile("" ::: "memory");
+    const int cn = __builtin_amdgcn_readfirstlane(spo[pk + 1]);
+    const bool later = pr > pk && pr < np;
+#pragma unroll
+    for (int i = 0; i < SW; ++i)
+      if (c + i >= cn && c + i < 64) {
+        ssv[c + i][lane] = a[i];
+        if (later) a[i] = T(0);
+      }
+  };
+  auto piece_end = [&](int pk, int c) {
+    __asm__ volatile("" ::: "memory");
+    const int cn = __builtin_amdgcn_readfirstlane(spo[pk + 1]);
+    const bool later = pr > pk && pr < np;
+#pragma unroll
+    for (int i = 0; i < SW; ++i)
+      if (c + i >= cn && c + i < 64 && later) a[i] = ssv[c + i][lane] + a[i];
+  };
+  // the piece of the current step, [plo, phi): advanced at its boundaries
+  int pk = 0;
+  int plo = 0;
+  int phi = __builtin_amdgcn_readfirstlane(spo[1]);
+  // steps of the columns left of this wave's slab, as they are published
+  const int nprev = min(cb, bw);
+  for (int k = 0; k < nprev; ++k) {
+    if (k == phi) {
+      ++pk;
+      plo = phi;
+      phi = __builtin_amdgcn_readfirstlane(spo[pk + 1]);
+    }
+    for (int it = 0; it < (1 << 24) &&
+         __hip_atomic_load(&sdone, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) <= k;
+         ++it)
+      __builtin_amdgcn_s_sleep(1);
+    const int meta = __builtin_amdgcn_readfirstlane(smeta[k]);
+    const int bl = meta & 0xff;
+    const int kp = (meta >> 8) & 0xff;
+    const bool nz = (meta >> 16) & 1;
+    if (nz && kp != k) {
+      if (pos == k)
+        pos = kp;
+      else if (lane == bl)
+        pos = k;
+    }
+    const bool below = (pr == pk && pos > k) || (pr > pk && pr < np);
+    // rows not below subtract 0 u, which leaves a finite entry unchanged
+    const T lb = below ? sl[k][lane] : T(0);
+    if (k == plo) piece_begin(pk, cb);
+    T u[SW];
+#pragma unroll
+    for (int i = 0; i < SW; ++i) u[i] = rdlane(a[i], bl);
+#pragma unroll
+    for (int i = 0; i < SW; ++i) a[i] = a[i] - lb * u[i];
+    if (k == phi - 1) piece_end(pk, cb);
+  }
+  // this wave's own columns; step k is published (sdone = k + 1) at the
+  // start of step k + 1, when its LDS writes have long completed, so that
+  // the release never waits
+  const int kend = min(cb + SW, bw);
+  for (int k = cb; k < kend; ++k) {
+    if (k == phi) {
+      ++pk;
+      plo = phi;
+      phi = __builtin_amdgcn_readfirstlane(spo[pk + 1]);
+    }
+    if (k > cb)
+      __hip_atomic_store(&sdone, k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k == plo) piece_begin(pk, k);
+    const T v = a[0];
+    const int bl = pivot_lane(v, pos, pr == pk, k);
+    const int kp = __builtin_amdgcn_readlane(pos, bl);
+    const T pv = rdlane(v, bl);
+    T u[SW];
+#pragma unroll
+    for (int i = 1; i < SW; ++i) u[i] = rdlane(a[i], bl);
+    const bool nz = pv != T(0);
+    if (nz && kp != k) {
+      if (pr == pk && pos == k)
+        pos = kp;
+      else if (lane == bl)
+        pos = k;
+    }
+    const T inv = T(1) / pv;
+    const bool below = (pr == pk && pos > k) || (pr > pk && pr < np);
+    const T l = nz ? v * inv : v;
+    // (no global memory access until the end: a release would wait for it)
+    sl[k][lane] = l;
+    if (lane == 0) {
+      smeta[k] = bl | (kp << 8) | ((int)nz << 16);
+      spiv[k] = kp - plo + 1;
+    }
+    sout[k][lane] = (below && nz) ? l : v;
+    const T lb = below ? l : T(0);
+#pragma unroll
+    for (int i = 1; i < SW; ++i) a[i] = a[i] - lb * u[i];
+    if (k == phi - 1) piece_end(pk, k);
+#pragma unroll
+    for (int i = 0; i + 1 < SW; ++i) a[i] = a[i + 1];
+    a[SW - 1] = T(0);
+  }
+  if (kend > cb)
+    __hip_atomic_store(&sdone, kend, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  // the wave of the last column has seen every interchange
+  if (w == (bw - 1) / SW) spos[lane] = pos;
+  __syncthreads();
+  const int fpos = spos[lane];
+  if ((int)threadIdx.x < bw) {
+    ipvt[fc0 + c0 + threadIdx.x] = spiv[threadIdx.x];
+    // a zero pivot: the first piece that met one
+    if (!((smeta[threadIdx.x] >> 16) & 1))
+      atomicMin(first_fail, (unsigned long long)(d.piece + scp[threadIdx.x] + 1));
+  }
+  // column c of this wave: row r at its position in the columns of its
+  // piece and right of it, at its own row in earlier pieces' columns
+  if (lane < bw) {
+    for (int i = 0; i < SW; ++i) {
+      const int c = cb + i;
+      if (c < bw) R[(c >= rlo ? fpos : lane) + (int64_t)c * L] = sout[c][lane];
+    }
+  }
+}
+
 // Off-diagonal parts of one block after blk_diag_kernel (see the section
 // comment); NQ bounds the piece width. Workgroups b < d.nl take 64 rows
 // below the block, the others 64 columns right of it. Dynamic LDS:
@@ -1242,6 +1449,127 @@ __device__ inline unsigned short f2bf(float x) {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
+// The 16-bit trailing update of one 128 x 128 tile on the gfx950 bf16
+// matrix cores (v_mfma_f32_16x16x32_bf16), K-chunks of 32, double-buffered
+// in LDS as rows of kB16 halves: A(i, k) = L21 rows at As[i kB16 + k],
+// B(k, j) = U12 columns at Bs[j kB16 + k]. Global reads are four-float
+// vectors (rows of an L21 column, k of a U12 column); each thread converts
+// them and stores four bf16 per row or column (8 bytes); each lane reads its
+// MFMA operand (eight consecutive k) with one 16-byte LDS read. Split mode:
+// x = hi + lo, both bf16, products lo_b hi_a + hi_b lo_a + hi_b hi_a.
+constexpr int kB16 = 40;                       // halves per LDS row (80 bytes)
+constexpr int kB16Part = 128 * kB16;           // one operand array, halves
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+
+// p[0 .. 3], element e read only when e < nvalid (0 otherwise)
+__device__ inline f4u load4(const float* p, int nvalid) {
+  if (nvalid >= 4) return *reinterpret_cast<const f4u*>(p);
+  f4u v = {0.f, 0.f, 0.f, 0.f};
+  if (nvalid > 0) v.x = p[0];
+  if (nvalid > 1) v.y = p[1];
+  if (nvalid > 2) v.z = p[2];
+  return v;
+}
+
+__device__ inline uint2 pack_bf16(float a, float b, float c, float d) {
+  return uint2{(unsigned)f2bf(a) | ((unsigned)f2bf(b) << 16),
+               (unsigned)f2bf(c) | ((unsigned)f2bf(d) << 16)};
+}
+__device__ inline float bf_lo(float x) { return x - __uint_as_float((unsigned)f2bf(x) << 16); }
+
+__device__ void gemm_bf16_chunks(const float* __restrict__ F, int L, int k0, int bw, int r0,
+                                 int rend, int j0, int cend, bool split,
+                                 unsigned char* lds_raw, f32x4 (&acc)[4][4]) {
+  constexpr int KC = 32;
+  unsigned short* base = reinterpret_cast<unsigned short*>(lds_raw);
+  // buffer b: [A hi, B hi, A lo, B lo], each kB16Part halves
+  auto part = [&](int buf, int which) { return base + (buf * 4 + which) * kB16Part; };
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int wr = (wave % 2) * 64, wc = (wave / 2) * 64;
+  const int l16 = lane % 16, lk = lane / 16;
+  // A: rows 4 ag .. 4 ag + 3 of the tile, k = 4 aq + u; B: columns
+  // bc + 32 u, k = 4 bq .. 4 bq + 3
+  const int ag = tid % 32, aq = tid / 32;
+  const int bq = tid % 8, bc = tid / 8;
+  const int arow = r0 + 4 * ag;
+  const int anv = rend - arow;  // valid rows of this thread's A groups
+  f4u ra[4], rb[4];
+  auto gload = [&](int kc) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = kc + 4 * aq + u;
+      ra[u] = load4(F + arow + (int64_t)(k0 + k) * L, k < bw ? anv : 0);
+      const int c = j0 + bc + 32 * u;
+      const int kb = kc + 4 * bq;
+      rb[u] = load4(F + (k0 + kb) + (int64_t)c * L, c < cend ? bw - kb : 0);
+    }
+  };
+  auto lstore = [&](int buf) {
+    unsigned short* Ah = part(buf, 0);
+    unsigned short* Bh = part(buf, 1);
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      *reinterpret_cast<uint2*>(Ah + (4 * ag + e) * kB16 + 4 * aq) =
+          pack_bf16(ra[0][e], ra[1][e], ra[2][e], ra[3][e]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      *reinterpret_cast<uint2*>(Bh + (bc + 32 * u) * kB16 + 4 * bq) =
+          pack_bf16(rb[u][0], rb[u][1], rb[u][2], rb[u][3]);
+    if (split) {
+      unsigned short* Al = part(buf, 2);
+      unsigned short* Bl = part(buf, 3);
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        *reinterpret_cast<uint2*>(Al + (4 * ag + e) * kB16 + 4 * aq) =
+            pack_bf16(bf_lo(ra[0][e]), bf_lo(ra[1][e]), bf_lo(ra[2][e]), bf_lo(ra[3][e]));
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        *reinterpret_cast<uint2*>(Bl + (bc + 32 * u) * kB16 + 4 * bq) =
+            pack_bf16(bf_lo(rb[u][0]), bf_lo(rb[u][1]), bf_lo(rb[u][2]), bf_lo(rb[u][3]));
+    }
+  };
+  const int nch = (bw + KC - 1) / KC;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ++ch) {
+    const int cur = ch & 1;
+    if (ch + 1 < nch) gload((ch + 1) * KC);
+    bf16x8 ah[4], bh[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      ah[t] = *reinterpret_cast<const bf16x8*>(part(cur, 0) + (wr + t * 16 + l16) * kB16 + 8 * lk);
+      bh[t] = *reinterpret_cast<const bf16x8*>(part(cur, 1) + (wc + t * 16 + l16) * kB16 + 8 * lk);
+    }
+    if (split) {
+      bf16x8 al[4], bl[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        al[t] = *reinterpret_cast<const bf16x8*>(part(cur, 2) + (wr + t * 16 + l16) * kB16 + 8 * lk);
+        bl[t] = *reinterpret_cast<const bf16x8*>(part(cur, 3) + (wc + t * 16 + l16) * kB16 + 8 * lk);
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[c], ah[a], acc[a][c], 0, 0, 0);
+          acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[c], al[a], acc[a][c], 0, 0, 0);
+          acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[c], ah[a], acc[a][c], 0, 0, 0);
+        }
+    } else {
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[c], ah[a], acc[a][c], 0, 0, 0);
+    }
+    if (ch + 1 < nch) lstore(cur ^ 1);
+    __syncthreads();
+  }
+}
+
 // With lowp set, a float update of a front wider than 256 columns takes its
 // L21 / U12 operands in 16 bits and accumulates in float: lowp 1 rounds them
 // to bf16 for v_mfma_f32_16x16x16bf16_1k, lowp 2 scales column k of L21 up
@@ -1260,7 +1588,7 @@ blk_gemm_kernel(DevSym s, const GemmDesc* __restrict__ desc, int ndesc,
   // (the split-bf16 form stages four bf16 arrays, hi and lo of A and B, in
   // the same bytes: 2 buffers x 4 x 128 rows of 20 halves)
   constexpr int kLdsT = 2 * 2 * TM * KP * (int)sizeof(T);
-  constexpr int kLdsH = std::is_same<T, float>::value ? 2 * 4 * TM * 20 * 2 : 0;
+  constexpr int kLdsH = std::is_same<T, float>::value ? 2 * 4 * kB16Part * 2 : 0;
   __shared__ __attribute__((aligned(16))) unsigned char lds_raw[kLdsT > kLdsH ? kLdsT : kLdsH];
   T (*As)[TM * KP] = reinterpret_cast<T (*)[TM * KP]>(lds_raw);
   T (*Bs)[TM * KP] = reinterpret_cast<T (*)[TM * KP]>(lds_raw + 2 * TM * KP * sizeof(T));
@@ -1370,94 +1698,8 @@ blk_gemm_kernel(DevSym s, const GemmDesc* __restrict__ desc, int ndesc,
       }
       done = true;
     }
-    if (lowp == 3 && d.big) {
-      // split bf16: each operand x = hi + lo (hi = bf16(x), lo = bf16(x -
-      // hi)), and each product taken as lo_b hi_a + hi_b lo_a + hi_b hi_a
-      // (the lo lo term dropped): about 16 bits per operand at three times
-      // the matrix-core work of plain bf16, in a kernel bound by its operand
-      // loads
-      constexpr int KB = 20;
-      unsigned short* Ah = reinterpret_cast<unsigned short*>(lds_raw);
-      unsigned short* Al = Ah + 2 * TM * KB;
-      unsigned short* Bh = Al + 2 * TM * KB;
-      unsigned short* Bl = Bh + 2 * TM * KB;
-      auto sstore = [&](int buf) {
-#pragma unroll
-        for (int t = 0; t < PER; ++t) {
-          const unsigned short ha = f2bf(ra[t]);
-          const unsigned short la = f2bf(ra[t] - __uint_as_float((unsigned)ha << 16));
-          const unsigned short hb = f2bf(rb[t]);
-          const unsigned short lb = f2bf(rb[t] - __uint_as_float((unsigned)hb << 16));
-          const int ia = buf * TM * KB + ai * KB + ak + 2 * t;
-          const int ib = buf * TM * KB + (bj + 16 * t) * KB + bk;
-          Ah[ia] = ha;
-          Al[ia] = la;
-          Bh[ib] = hb;
-          Bl[ib] = lb;
-        }
-      };
-      gload(0);
-      sstore(0);
-      __syncthreads();
-      for (int ch = 0; ch < nch; ++ch) {
-        const int cur = ch & 1;
-        if (ch + 1 < nch) gload((ch + 1) * KC);
-        bf16x4 ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int oa = cur * TM * KB + (wr + t * 16 + l16) * KB + 4 * lk;
-          const int ob = cur * TM * KB + (wc + t * 16 + l16) * KB + 4 * lk;
-          ah[t] = *reinterpret_cast<const bf16x4*>(Ah + oa);
-          al[t] = *reinterpret_cast<const bf16x4*>(Al + oa);
-          bh[t] = *reinterpret_cast<const bf16x4*>(Bh + ob);
-          bl[t] = *reinterpret_cast<const bf16x4*>(Bl + ob);
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bl[c], ah[a], acc[a][c], 0, 0, 0);
-            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bh[c], al[a], acc[a][c], 0, 0, 0);
-            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bh[c], ah[a], acc[a][c], 0, 0, 0);
-          }
-        if (ch + 1 < nch) sstore(cur ^ 1);
-        __syncthreads();
-      }
-      done = true;
-    }
-    if (lowp == 1 && d.big) {
-      // bf16 operands in the same LDS: rows of KB = 20 halves (8-byte
-      // aligned groups of four k)
-      constexpr int KB = 20;
-      unsigned short* Ab = reinterpret_cast<unsigned short*>(&As[0][0]);
-      unsigned short* Bb = reinterpret_cast<unsigned short*>(&Bs[0][0]);
-      auto bstore = [&](int buf) {
-#pragma unroll
-        for (int t = 0; t < PER; ++t) {
-          Ab[buf * TM * KB + ai * KB + ak + 2 * t] = f2bf(ra[t]);
-          Bb[buf * TM * KB + (bj + 16 * t) * KB + bk] = f2bf(rb[t]);
-        }
-      };
-      gload(0);
-      bstore(0);
-      __syncthreads();
-      for (int ch = 0; ch < nch; ++ch) {
-        const int cur = ch & 1;
-        if (ch + 1 < nch) gload((ch + 1) * KC);
-        bf16x4 av[4], bv[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          av[t] = *reinterpret_cast<const bf16x4*>(Ab + cur * TM * KB + (wr + t * 16 + l16) * KB + 4 * lk);
-          bv[t] = *reinterpret_cast<const bf16x4*>(Bb + cur * TM * KB + (wc + t * 16 + l16) * KB + 4 * lk);
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int c = 0; c < 4; ++c)
-            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bv[c], av[a], acc[a][c], 0, 0, 0);
-        if (ch + 1 < nch) bstore(cur ^ 1);
-        __syncthreads();
-      }
+    if ((lowp == 1 || lowp == 3) && d.big) {
+      gemm_bf16_chunks(F, L, d.k0, bw, r0, rend, j0, cend, lowp == 3, lds_raw, acc);
       done = true;
     }
   }

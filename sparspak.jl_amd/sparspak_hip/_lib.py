@@ -104,7 +104,10 @@ _SIGS = {
                                   i64p, fp, fp]),
     "spkhip_ldltfactor_f64": (i64, [i64, i64, i64p, i64p, i64p, i64p, i64p,
                                     dp]),
+    "spkhip_ldltfactor_f32": (i64, [i64, i64, i64p, i64p, i64p, i64p, i64p,
+                                    fp]),
     "spkhip_ldltsolve_f64": (i64, [i64, i64p, i64p, i64p, i64p, dp, dp]),
+    "spkhip_ldltsolve_f32": (i64, [i64, i64p, i64p, i64p, i64p, fp, fp]),
     "spkhip_profile_enable": (i64, [vp, i64]),
     "spkhip_profile_get": (i64, [vp, i64, ctypes.POINTER(KStat)]),
     "spkhip_get_tuning": (i64, [vp, ctypes.POINTER(Tuning)]),

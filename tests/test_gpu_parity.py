@@ -251,6 +251,35 @@ def test_gpu_stateless_ldlt():
     assert np.max(np.abs(r1 - r2)) <= 1e-12 * np.max(np.abs(r2))
 
 
+def test_gpu_stateless_ldlt_f32():
+    """The Float32 drop-ins against the oracle's Float64 LDL^T: entries within
+    the float tolerance, solve within float accuracy of the double solve."""
+    _say("case stateless ldlt f32")
+    A = laplace2d(15, shift=0.5)
+    s = SparseSpdSolver(A)
+    findorder(s)
+    symbolicfactor(s)
+    sym = s.slvr.sym
+    n, cp, rv, nz = csc1(A)
+    lnz, _, _ = orc.inmatrix(sym, cp, rv, nz)
+    g = lnz.astype(np.float32)
+    L = _lib.lib()
+    P = _lib.p64
+    rc = L.spkhip_ldltfactor_f32(n, sym.nsuper, P(sym.xsuper), P(sym.snode), P(sym.xlindx),
+                                 P(sym.lindx), P(sym.xlnz), _lib.pf(g))
+    assert rc == 0
+    orc.ldltfactor(sym, lnz)
+    m = _spd_meaningful(sym)
+    assert _relmax(g[m].astype(np.float64), lnz[m]) < 2e-5
+    b = uniform(12, n, -1, 1)
+    r2 = b[sym.order.rperm - 1].copy()
+    r1 = r2.astype(np.float32)
+    L.spkhip_ldltsolve_f32(sym.nsuper, P(sym.xsuper), P(sym.xlindx), P(sym.lindx), P(sym.xlnz),
+                           _lib.pf(g), _lib.pf(r1))
+    orc.ldltsolve_permuted(sym, lnz, r2)
+    assert np.max(np.abs(r1 - r2)) <= 1e-4 * np.max(np.abs(r2))
+
+
 def test_gpu_zero_pivot_raises():
     _say("case zero pivot")
     A = sp.csc_matrix(np.array([[1.0, 1.0, 0.0], [1.0, 1.0, 0.0], [0.0, 0.0, 2.0]]))
