@@ -241,11 +241,15 @@ struct spkhip_handle {
   hipGraphExec_t g_exec = nullptr;
   std::vector<const void*> g_key;
   std::vector<const void*> g_seen;  // last key run directly (captured on its second use)
+  // the factorisation's launches, likewise (SPKHIP_FACTOR_GRAPH=0 disables)
+  hipGraphExec_t fg_exec = nullptr;
+  std::vector<const void*> fg_key, fg_seen;
   ~spkhip_handle() {
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
     for (hipEvent_t e : xev) (void)hipEventDestroy(e);
     if (ev_caller) (void)hipEventDestroy(ev_caller);
     if (g_exec) (void)hipGraphExecDestroy(g_exec);
+    if (fg_exec) (void)hipGraphExecDestroy(fg_exec);
   }
 };
 
@@ -995,8 +999,12 @@ void run_factor(spkhip_handle* h) {
       hipck(hipGetLastError());
     }
   }
+}
+
+// The first supernode with a zero pivot, read back after the launches.
+void read_first_fail(spkhip_handle* h) {
   unsigned long long ff = 0;
-  hipck(hipMemcpyAsync(&ff, h->d_fail.p, sizeof(ff), hipMemcpyDeviceToHost, st));
+  hipck(hipMemcpyAsync(&ff, h->d_fail.p, sizeof(ff), hipMemcpyDeviceToHost, h->stream));
   sync(h);
   h->first_fail = (ff == ~0ull) ? 0 : (i64)ff;
 }
@@ -1233,6 +1241,66 @@ void solve_device_in(spkhip_handle* h, double* d_b, i64 ldb, int nrhs, T* rhs, T
   hipLaunchKernelGGL((spk::permute_kernel<T, double>), dim3(grid_for(h->n)), dim3(kWG), 0, st,
                      rhs, h->n, d_b, ldb, h->d_invp.p, h->n, nrhs);
   hipck(hipGetLastError());
+}
+
+// Every launch of one factorisation: assembly of A's values into lnz/unz,
+// then the levels (both streams).
+template <typename T>
+void factor_launches(spkhip_handle* h) {
+  run_assemble<T>(h);
+  run_factor<T>(h);
+}
+
+// The factorisation's launches replayed from a captured HIP graph while
+// nothing they name has changed (precision, update mode, plan generation,
+// buffers); a key is captured on its second use, so that a handle factored
+// once pays no capture. The graph holds both streams (the second joins the
+// capture through the events of stream_join and is joined back before its
+// end), contains no memset (buffers are cleared by kernels) and no host
+// copy: the zero-pivot flag is read back after it. SPKHIP_FACTOR_GRAPH=0
+// launches directly; the per-launch profile always does.
+template <typename T>
+void factor_run(spkhip_handle* h) {
+  const char* ge = std::getenv("SPKHIP_FACTOR_GRAPH");
+  const bool use_graph = !(ge && ge[0] == '0') && !h->prof;
+  // every buffer sized before any capture
+  Bufs<T>::lnz(h).ensure((size_t)std::max<i64>(h->nlnz, 1));
+  Bufs<T>::unz(h).ensure((size_t)std::max<i64>(h->nunz, 1));
+  Bufs<T>::arena(h).ensure((size_t)std::max<i64>(h->arena_elems, 1));
+  h->d_ipvt.ensure((size_t)h->n);
+  h->d_fail.ensure(1);
+  const std::vector<const void*> key = {
+      (const void*)(intptr_t)sizeof(T), (const void*)(intptr_t)h->lowp,
+      (const void*)(intptr_t)h->sym_gen, Bufs<T>::lnz(h).p, Bufs<T>::unz(h).p,
+      Bufs<T>::arena(h).p, h->d_ipvt.p, h->d_fail.p, h->d_nzval.p, h->d_amap.p};
+  if (!use_graph) {
+    factor_launches<T>(h);
+  } else if (h->fg_exec && h->fg_key == key) {
+    hipck(hipGraphLaunch(h->fg_exec, h->stream));
+  } else if (h->fg_seen != key) {
+    h->fg_seen = key;
+    factor_launches<T>(h);
+  } else {
+    if (h->fg_exec) {
+      hipck(hipGraphExecDestroy(h->fg_exec));
+      h->fg_exec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    hipck(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      factor_launches<T>(h);
+    } catch (...) {
+      (void)hipStreamEndCapture(h->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    hipck(hipStreamEndCapture(h->stream, &g));
+    hipck(hipGraphInstantiate(&h->fg_exec, g, nullptr, nullptr, 0));
+    hipck(hipGraphDestroy(g));
+    h->fg_key = key;
+    hipck(hipGraphLaunch(h->fg_exec, h->stream));
+  }
+  read_first_fail(h);
 }
 
 // Storage, launch plan and their device copies: everything that depends on
@@ -1540,6 +1608,7 @@ int64_t lufactor_stateless(i64 n, i64 nsuper, const i64* xsuper, const i64* snod
       hipck(hipMemcpyAsync(du.p, unz, h->nunz * sizeof(T), hipMemcpyHostToDevice, st));
     const int prec = std::is_same<T, double>::value ? SPKHIP_FP64 : SPKHIP_FP32;
     run_factor<T>(h);
+    read_first_fail(h);
     h->precision = prec;
     h->factored = true;
     h->prepped = false;
@@ -1789,12 +1858,10 @@ int64_t spkhip_factor(spkhip_t h, int64_t precision) {
               : precision == SPKHIP_FP32_BF16X3 ? 3
                                                 : 0;
     if (precision == SPKHIP_FP64) {
-      run_assemble<double>(h);
-      run_factor<double>(h);
+      factor_run<double>(h);
       size_solve_data<double>(h);
     } else {
-      run_assemble<float>(h);
-      run_factor<float>(h);
+      factor_run<float>(h);
       size_solve_data<float>(h);
     }
     h->lowp = 0;
