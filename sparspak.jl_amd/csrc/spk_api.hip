@@ -79,6 +79,7 @@ struct DevBuf {
 
 struct LevelPlan {
   int small_off = 0, small_cnt = 0, small_maxL = 0;
+  int small_maxq = 1;               // widest piece of the level's small fronts
   int items_off = 0, items_cnt = 0;
   int wb_off = 0, wb_cnt = 0;       // write-back items: the large fronts' pivot columns
   int large_maxL = 0;               // longest index list of the level's large fronts
@@ -483,6 +484,8 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
       if (h->fL[f] <= h->small_lmax) {
         small.push_back(f);
         lp.small_maxL = std::max(lp.small_maxL, h->fL[f]);
+        for (i32 p = h->fsp0[f]; p < h->fsp0[f] + h->fnp[f]; ++p)
+          lp.small_maxq = std::max(lp.small_maxq, (int)(h->xsuper[p + 1] - h->xsuper[p]));
       } else {
         large.push_back(f);
       }
@@ -1029,10 +1032,14 @@ void run_assemble(spkhip_handle* h) {
   hipck(hipGetLastError());
 }
 
+// Right-hand sides that share one staging of a small front's panels.
+inline int small_rhs_block(int nrhs) { return std::max(1, std::min(nrhs, 4)); }
+
 // Small fronts per workgroup in the solves (one wave each): four when their
-// staging fits in 150 KB of LDS, fewer otherwise.
+// staging (a panel of at most maxq columns and qb solve vectors) fits in
+// 150 KB of LDS, fewer otherwise.
 template <typename T, int MAXW>
-int small_fronts_per_wg(int maxL) {
+int small_fronts_per_wg(int maxL, int qb = 1, int maxq = 1) {
   static bool attrs = false;
   if (!attrs) {
     hipck(hipFuncSetAttribute((const void*)spk::fwd_small_kernel<T, MAXW>,
@@ -1041,7 +1048,7 @@ int small_fronts_per_wg(int maxL) {
                               hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
     attrs = true;
   }
-  const size_t per = (size_t)(MAXW + 1) * std::max(maxL, 1) * sizeof(T);
+  const size_t per = (size_t)(maxq + qb) * std::max(maxL, 1) * sizeof(T);
   return (int)std::max<size_t>(1, std::min<size_t>(4, 150 * 1024 / per));
 }
 
@@ -1088,12 +1095,13 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
       h->cur_grid = lp.small_cnt;
       if (lp.small_cnt > 0) {
         Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
-        const int fpw = small_fronts_per_wg<T, MAXW>(lp.small_maxL);
+        const int qb = small_rhs_block(nrhs);
+        const int fpw = small_fronts_per_wg<T, MAXW>(lp.small_maxL, qb, lp.small_maxq);
         hipLaunchKernelGGL((spk::fwd_small_kernel<T, MAXW>),
                            dim3((lp.small_cnt + fpw - 1) / fpw), dim3(kWG),
-                           (size_t)fpw * (MAXW + 1) * lp.small_maxL * sizeof(T), st, h->ds,
-                           h->d_small.p + lp.small_off, lp.small_cnt, fpw, lnz, h->d_ipvt.p,
-                           rhs, ldr, nrhs, vecp, lp.small_maxL);
+                           (size_t)fpw * (lp.small_maxq + qb) * lp.small_maxL * sizeof(T), st,
+                           h->ds, h->d_small.p + lp.small_off, lp.small_cnt, fpw, qb, lnz,
+                           h->d_ipvt.p, rhs, ldr, nrhs, vecp, lp.small_maxL, lp.small_maxq);
         hipck(hipGetLastError());
       }
       if (lp.large_cnt == 0) continue;
@@ -1151,12 +1159,13 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
       }
       if (lp.small_cnt > 0) {
         Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);
-        const int fpw = small_fronts_per_wg<T, MAXW>(lp.small_maxL);
+        const int qb = small_rhs_block(nrhs);
+        const int fpw = small_fronts_per_wg<T, MAXW>(lp.small_maxL, qb, lp.small_maxq);
         hipLaunchKernelGGL((spk::bwd_small_kernel<T, MAXW>),
                            dim3((lp.small_cnt + fpw - 1) / fpw), dim3(kWG),
-                           (size_t)fpw * (MAXW + 1) * lp.small_maxL * sizeof(T), st, h->ds,
-                           h->d_small.p + lp.small_off, lp.small_cnt, fpw, lnz, unz, rhs, ldr,
-                           nrhs, lp.small_maxL);
+                           (size_t)fpw * (lp.small_maxq + qb) * lp.small_maxL * sizeof(T), st,
+                           h->ds, h->d_small.p + lp.small_off, lp.small_cnt, fpw, qb, lnz, unz,
+                           rhs, ldr, nrhs, lp.small_maxL, lp.small_maxq);
         hipck(hipGetLastError());
       }
     }

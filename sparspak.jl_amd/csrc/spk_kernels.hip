@@ -1809,8 +1809,8 @@ __device__ inline T dot_lrow(const DevSym& s, const T* __restrict__ lnz, int64_t
 
 // Unit-lower solve of one piece's diagonal block on yb (LDS) by wave 0,
 // after the piece's interchanges (glaswp! then gtrsm! 'l','l','n','u').
-template <typename T, int MAXW>
-__device__ void wave_lower_solve(T* yb, const T* D, const int* piv, int nq) {
+template <typename T>
+__device__ void wave_lower_solve(T* yb, const T* D, int ld, const int* piv, int nq) {
   const int lane = threadIdx.x % 64;
   if (lane == 0)
     for (int k = 0; k < nq; ++k) {
@@ -1825,22 +1825,22 @@ __device__ void wave_lower_solve(T* yb, const T* D, const int* piv, int nq) {
   for (int k = 0; k < nq; ++k) {
     const T yk = yb[k];
     if (yk != T(0))
-      for (int i = k + 1 + lane; i < nq; i += 64) yb[i] -= yk * D[i + k * MAXW];
+      for (int i = k + 1 + lane; i < nq; i += 64) yb[i] -= yk * D[i + k * ld];
     wave_lds_sync();
   }
 }
 
 // Upper solve (gtrsm! 'l','u','n','n') of one piece's block by wave 0.
-template <typename T, int MAXW>
-__device__ void wave_upper_solve(T* yb, const T* D, int nq) {
+template <typename T>
+__device__ void wave_upper_solve(T* yb, const T* D, int ld, int nq) {
   const int lane = threadIdx.x % 64;
   for (int k = nq - 1; k >= 0; --k) {
     T yk = yb[k];
     if (yk != T(0)) {
-      yk /= D[k + k * MAXW];
+      yk /= D[k + k * ld];
       wave_lds_sync();
       if (lane == 0) yb[k] = yk;
-      for (int i = lane; i < k; i += 64) yb[i] -= yk * D[i + k * MAXW];
+      for (int i = lane; i < k; i += 64) yb[i] -= yk * D[i + k * ld];
     }
     wave_lds_sync();
   }
@@ -1859,32 +1859,44 @@ __device__ void wave_upper_solve(T* yb, const T* D, int nq) {
 // panel (MAXW x maxL, maxL the level's longest small front) and yW[maxL].
 template <typename T, int MAXW>
 __global__ void __launch_bounds__(kWG)
-fwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw,
+fwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw, int qb,
                  const T* __restrict__ lnz, const int64_t* __restrict__ ipvt,
-                 T* __restrict__ rhs, int64_t ldr, int nrhs, T* __restrict__ vec, int maxL) {
+                 T* __restrict__ rhs, int64_t ldr, int nrhs, T* __restrict__ vec, int maxL,
+                 int maxq) {
   extern __shared__ unsigned char smem_raw[];
   __shared__ int spiv_all[4][MAXW];
   const int lane = threadIdx.x % 64, wave = wave_id();
   const int item = blockIdx.x * fpw + wave;
   if (wave >= fpw || item >= cnt) return;
-  T* D = reinterpret_cast<T*>(smem_raw) + (int64_t)wave * (MAXW + 1) * maxL;
-  T* yW = D + MAXW * maxL;
+  // the staged panel: rows of stride maxL, at most maxq columns (the level's
+  // widest piece); then the qb solve vectors
+  T* D = reinterpret_cast<T*>(smem_raw) + (int64_t)wave * (maxq + qb) * maxL;
+  T* Y = D + (int64_t)maxq * maxL;
   int* spiv = spiv_all[wave];
   const int f = list[item];
   const int L = s.fL[f], W = s.fW[f];
   const int64_t fc0 = s.fc0[f];
   const int sp0 = s.fsp0[f], np = s.fnp[f];
-  for (int q = 0; q < nrhs; ++q) {
-    T* yg = vec + s.fvec[f] * nrhs + (int64_t)q * L;
-    T* x = rhs + (int64_t)q * ldr;
-    for (int r = lane; r < L; r += 64) yW[r] = (r < W) ? x[fc0 + r] : T(0);
+  // right-hand sides in blocks of qb (y of the block's right-hand side qq
+  // at yW + qq maxL), each piece's panel staged once per block; every
+  // right-hand side receives the same operations in the same order
+  for (int q0 = 0; q0 < nrhs; q0 += qb) {
+    const int nq_b = min(qb, nrhs - q0);
+    for (int qq = 0; qq < nq_b; ++qq) {
+      T* yW = Y + qq * maxL;
+      const T* x = rhs + (int64_t)(q0 + qq) * ldr;
+      for (int r = lane; r < L; r += 64) yW[r] = (r < W) ? x[fc0 + r] : T(0);
+    }
     wave_lds_sync();
     for (int ci = s.fchp[f]; ci < s.fchp[f + 1]; ++ci) {
       const int c = s.fch[ci];
       const int Lc = s.fL[c], Wc = s.fW[c];
       const int32_t* rel = s.relmap + s.frel[c];
-      const T* uc = vec + s.fvec[c] * nrhs + (int64_t)q * Lc + Wc;
-      for (int t = lane; t < Lc - Wc; t += 64) yW[rel[t]] += uc[t];
+      for (int qq = 0; qq < nq_b; ++qq) {
+        T* yW = Y + qq * maxL;
+        const T* uc = vec + s.fvec[c] * nrhs + (int64_t)(q0 + qq) * Lc + Wc;
+        for (int t = lane; t < Lc - Wc; t += 64) yW[rel[t]] += uc[t];
+      }
       wave_lds_sync();
     }
     for (int p = sp0; p < sp0 + np; ++p) {
@@ -1896,22 +1908,30 @@ fwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw,
       const T* lp = lnz + (s.xlnz[pc0] - 1);
       for (int idx = lane; idx < jl * nq; idx += 64) {
         const int i = idx % jl, kk = idx / jl;
-        D[i + kk * MAXW] = lp[i + (int64_t)kk * jl];
+        D[i + kk * maxL] = lp[i + (int64_t)kk * jl];
       }
       for (int i = lane; i < nq; i += 64) spiv[i] = (int)ipvt[pc0 + i] - 1;
       wave_lds_sync();
-      wave_lower_solve<T, MAXW>(yW + cq, D, spiv, nq);
-      wave_lds_sync();
-      for (int r = cq + nq + lane; r < L; r += 64) {
-        const T* dr = D + (r - cq);
-        T acc = T(0);
-        for (int kk = 0; kk < nq; ++kk) acc += dr[kk * MAXW] * yW[cq + kk];
-        yW[r] -= acc;
+      for (int qq = 0; qq < nq_b; ++qq) {
+        T* yW = Y + qq * maxL;
+        wave_lower_solve<T>(yW + cq, D, maxL, spiv, nq);
+        wave_lds_sync();
+        for (int r = cq + nq + lane; r < L; r += 64) {
+          const T* dr = D + (r - cq);
+          T acc = T(0);
+          for (int kk = 0; kk < nq; ++kk) acc += dr[kk * maxL] * yW[cq + kk];
+          yW[r] -= acc;
+        }
+        wave_lds_sync();
       }
-      wave_lds_sync();
     }
-    for (int r = W + lane; r < L; r += 64) yg[r] = yW[r];
-    for (int r = lane; r < W; r += 64) x[fc0 + r] = yW[r];
+    for (int qq = 0; qq < nq_b; ++qq) {
+      const T* yW = Y + qq * maxL;
+      T* yg = vec + s.fvec[f] * nrhs + (int64_t)(q0 + qq) * L;
+      T* x = rhs + (int64_t)(q0 + qq) * ldr;
+      for (int r = W + lane; r < L; r += 64) yg[r] = yW[r];
+      for (int r = lane; r < W; r += 64) x[fc0 + r] = yW[r];
+    }
     wave_lds_sync();
   }
 }
@@ -1922,23 +1942,31 @@ fwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw,
 // LDS per wave: the staged rows (MAXW x maxL) and yW[maxL].
 template <typename T, int MAXW>
 __global__ void __launch_bounds__(kWG)
-bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw,
+bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw, int qb,
                  const T* __restrict__ lnz, const T* __restrict__ unz, T* __restrict__ rhs,
-                 int64_t ldr, int nrhs, int maxL) {
+                 int64_t ldr, int nrhs, int maxL, int maxq) {
   extern __shared__ unsigned char smem_raw[];
   const int lane = threadIdx.x % 64, wave = wave_id();
   const int item = blockIdx.x * fpw + wave;
   if (wave >= fpw || item >= cnt) return;
-  T* D = reinterpret_cast<T*>(smem_raw) + (int64_t)wave * (MAXW + 1) * maxL;
-  T* yW = D + MAXW * maxL;
+  // the staged U rows: at most maxq rows (the level's widest piece) of
+  // stride maxq, maxL columns; then the qb solve vectors
+  T* D = reinterpret_cast<T*>(smem_raw) + (int64_t)wave * (maxq + qb) * maxL;
+  T* Y = D + (int64_t)maxq * maxL;
   const int f = list[item];
   const int L = s.fL[f], W = s.fW[f];
   const int64_t fc0 = s.fc0[f];
   const int sp0 = s.fsp0[f], np = s.fnp[f];
   const int32_t* I = s.lindx + (s.xlindx[sp0] - 1);
-  for (int q = 0; q < nrhs; ++q) {
-    T* x = rhs + (int64_t)q * ldr;
-    for (int r = lane; r < L; r += 64) yW[r] = x[I[r] - 1];
+  // right-hand sides in blocks of qb (y of the block's right-hand side qq
+  // at yW + qq maxL), each piece's U rows staged once per block
+  for (int q0 = 0; q0 < nrhs; q0 += qb) {
+    const int nq_b = min(qb, nrhs - q0);
+    for (int qq = 0; qq < nq_b; ++qq) {
+      T* yW = Y + qq * maxL;
+      const T* x = rhs + (int64_t)(q0 + qq) * ldr;
+      for (int r = lane; r < L; r += 64) yW[r] = x[I[r] - 1];
+    }
     wave_lds_sync();
     for (int p = sp0 + np - 1; p >= sp0; --p) {
       int64_t pc0, nq64;
@@ -1952,20 +1980,27 @@ bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw,
       const T* lp = lnz + (s.xlnz[pc0] - 1);
       for (int idx = lane; idx < nq * jl; idx += 64) {
         const int c = idx % nq, j = idx / nq;
-        D[c + j * MAXW] = (j < nq) ? lp[c + (int64_t)j * jl] : unz[s.ucol[pc0 + c] + cq + j];
+        D[c + j * maxq] = (j < nq) ? lp[c + (int64_t)j * jl] : unz[s.ucol[pc0 + c] + cq + j];
       }
       wave_lds_sync();
-      for (int c = 0; c < nq; ++c) {
-        T acc = T(0);
-        for (int j = cq + nq + lane; j < L; j += 64) acc += D[c + (j - cq) * MAXW] * yW[j];
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-        if (lane == 0) yW[cq + c] -= acc;
+      for (int qq = 0; qq < nq_b; ++qq) {
+        T* yW = Y + qq * maxL;
+        for (int c = 0; c < nq; ++c) {
+          T acc = T(0);
+          for (int j = cq + nq + lane; j < L; j += 64) acc += D[c + (j - cq) * maxq] * yW[j];
+          for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+          if (lane == 0) yW[cq + c] -= acc;
+        }
+        wave_lds_sync();
+        wave_upper_solve<T>(yW + cq, D, maxq, nq);
+        wave_lds_sync();
       }
-      wave_lds_sync();
-      wave_upper_solve<T, MAXW>(yW + cq, D, nq);
-      wave_lds_sync();
     }
-    for (int c = lane; c < W; c += 64) x[fc0 + c] = yW[c];
+    for (int qq = 0; qq < nq_b; ++qq) {
+      const T* yW = Y + qq * maxL;
+      T* x = rhs + (int64_t)(q0 + qq) * ldr;
+      for (int c = lane; c < W; c += 64) x[fc0 + c] = yW[c];
+    }
     wave_lds_sync();
   }
 }
@@ -2208,11 +2243,12 @@ fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   const int ng = (bw + 3) / 4, bwp = 4 * ng;
   const bool in0 = lane < bw, in1 = lane + 64 < bw;
   const int32_t* pi = prepi + (int64_t)d.r0 * kPrepInts;
-  // wave 0: permutation and first right-hand side, in flight with the panels
+  // every wave: the permutation and its first right-hand side (wave w
+  // takes right-hand sides w, w + 4, ...), in flight with the panels
   int s0 = lane, s1 = lane + 64;
   T y0r = T(0), y1r = T(0);
-  const T* yv = vec + d.fvec * nrhs;
-  if (wave == 0) {
+  const T* yv = vec + d.fvec * nrhs + (int64_t)wave * L;
+  if (wave < nrhs) {
     if (in0) {
       s0 = pi[lane];
       y0r = yv[c0 + lane];
@@ -2225,7 +2261,7 @@ fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   load_panels(reinterpret_cast<const V*>(prep + 4 * (int64_t)d.aux), smem_v4,
               2 * ng * (ng + 1));
   __syncthreads();
-  if (wave != 0) return;
+  if (wave >= nrhs) return;
   const int gA = min(ng, 16);
   // panel values of step g (rows clamped into the panel)
   auto load = [&](int g, FwdStep<T>& o, bool both) {
@@ -2239,10 +2275,10 @@ fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
     if (both) o.a = *(const V*)(P + 4 * (min(max(lane, k), bwp - 1) - k));
     o.b = *(const V*)(P + 4 * (min(max(lane + 64, k), bwp - 1) - k));
   };
-  for (int q = 0; q < nrhs; ++q) {
+  for (int q = wave; q < nrhs; q += 4) {
     T* yg = vec + d.fvec * nrhs + (int64_t)q * L;
     T* x = rhs + (int64_t)q * ldr;
-    if (q > 0) {
+    if (q > wave) {
       y0r = in0 ? yg[c0 + lane] : T(0);
       y1r = in1 ? yg[c0 + 64 + lane] : T(0);
     }
@@ -2317,18 +2353,19 @@ fwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   const int r = c1 + d.r0 + lane;
   const int cb = bw * wave / 4, ce = bw * (wave + 1) / 4;  // at most 32 columns
   if (tid < bw) lc[tid] = s.lcol[fc0 + c0 + tid];
+  __syncthreads();
+  // the block's entries of this tile's rows, read once for every rhs
+  T lv[32];
+#pragma unroll
+  for (int j = 0; j < 32; ++j) lv[j] = (r < L && cb + j < ce) ? lnz[lc[cb + j] + r] : T(0);
   for (int q = 0; q < nrhs; ++q) {
     T* yg = vec + d.fvec * nrhs + (int64_t)q * L;
     if (tid < bw) zs[tid] = yg[c0 + tid];
-    // this tile's rows of y, requested with the block's entries (no other
-    // workgroup touches them)
+    // this tile's rows of y (no other workgroup touches them)
     const T yr = (wave == 0 && r < L) ? yg[r] : T(0);
     __syncthreads();
     T acc = T(0);
     if (r < L) {
-      T lv[32];
-#pragma unroll
-      for (int j = 0; j < 32; ++j) lv[j] = (cb + j < ce) ? lnz[lc[cb + j] + r] : T(0);
 #pragma unroll
       for (int j = 0; j < 32; ++j)
         if (cb + j < ce) acc += lv[j] * zs[cb + j];
@@ -2351,7 +2388,8 @@ template <typename T>
 __global__ void __launch_bounds__(kWG)
 bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ unz,
                  int nrhs, T* __restrict__ vec) {
-  __shared__ T red[4];
+  constexpr int QB = 8;  // right-hand sides sharing one read of the row
+  __shared__ T red[QB][4];
   const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
   const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
@@ -2359,24 +2397,46 @@ bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   const int L = d.L;
   const int64_t fc0 = d.fc0;
   const T* u = unz + d.ucol;  // the row's U entries (its unz offset rides in the descriptor)
-  for (int q = 0; q < nrhs; ++q) {
-    T* yg = vec + d.fvec * nrhs + (int64_t)q * L;
-    T acc = T(0);
+  for (int q0 = 0; q0 < nrhs; q0 += QB) {
+    const int nq = min(QB, nrhs - q0);
+    T acc[QB];
+#pragma unroll
+    for (int qq = 0; qq < QB; ++qq) acc[qq] = T(0);
     for (int j0 = d.c1 + tid; j0 < L; j0 += 16 * kWG) {
-      T uv[16], xv[16];
+      T uv[16];
 #pragma unroll
       for (int t = 0; t < 16; ++t) {
         const int j = j0 + kWG * t;
         uv[t] = (j < L) ? u[j] : T(0);
-        xv[t] = (j < L) ? yg[j] : T(0);
       }
 #pragma unroll
-      for (int t = 0; t < 16; ++t) acc += uv[t] * xv[t];
+      for (int qq = 0; qq < QB; ++qq) {
+        if (qq < nq) {
+          const T* yg = vec + d.fvec * nrhs + (int64_t)(q0 + qq) * L;
+          T xv[16];
+#pragma unroll
+          for (int t = 0; t < 16; ++t) {
+            const int j = j0 + kWG * t;
+            xv[t] = (j < L) ? yg[j] : T(0);
+          }
+#pragma unroll
+          for (int t = 0; t < 16; ++t) acc[qq] += uv[t] * xv[t];
+        }
+      }
     }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane == 0) red[wave] = acc;
+#pragma unroll
+    for (int qq = 0; qq < QB; ++qq) {
+      if (qq < nq) {
+        T a = acc[qq];
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+        if (lane == 0) red[qq][wave] = a;
+      }
+    }
     __syncthreads();
-    if (tid == 0) yg[c] = ((red[0] + red[1]) + red[2]) + red[3];
+    if (tid < nq) {
+      T* yg = vec + d.fvec * nrhs + (int64_t)(q0 + tid) * L;
+      yg[c] = ((red[tid][0] + red[tid][1]) + red[tid][2]) + red[tid][3];
+    }
     __syncthreads();
   }
 }
@@ -2427,9 +2487,11 @@ bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   const int ng = (bw + 3) / 4;
   const bool in0 = lane < bw, in1 = lane + 64 < bw, sub = c1 < L;
   T x0r = T(0), x1r = T(0), t0r = T(0), t1r = T(0);
-  if (wave == 0) {
-    const T* x = rhs + fc0 + c0;
-    const T* yg = vec + d.fvec * nrhs + c0;
+  // wave w takes right-hand sides w, w + 4, ...; its first one in flight
+  // with the panels
+  if (wave < nrhs) {
+    const T* x = rhs + (int64_t)wave * ldr + fc0 + c0;
+    const T* yg = vec + d.fvec * nrhs + (int64_t)wave * L + c0;
     if (in0) {
       x0r = x[lane];
       if (sub) t0r = yg[lane];
@@ -2443,7 +2505,7 @@ bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   load_panels(reinterpret_cast<const V*>(prep + 4 * (int64_t)d.aux + npan), smem_v4,
               2 * ng * (ng + 1) + ng);
   __syncthreads();
-  if (wave != 0) return;
+  if (wave >= nrhs) return;
   const T* Rinv = R + npan;
   const int gA = min(ng, 16);
   auto load = [&](int g, BwdStep<T>& o, bool both) {
@@ -2459,10 +2521,10 @@ bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
     o.a = *(const V*)(P + 4 * min(lane, k + 3));
     if (both) o.b = *(const V*)(P + 4 * min(lane + 64, k + 3));
   };
-  for (int q = 0; q < nrhs; ++q) {
+  for (int q = wave; q < nrhs; q += 4) {
     T* yg = vec + d.fvec * nrhs + (int64_t)q * L;
     T* x = rhs + (int64_t)q * ldr;
-    if (q > 0) {
+    if (q > wave) {
       x0r = in0 ? x[fc0 + c0 + lane] : T(0);
       x1r = in1 ? x[fc0 + c0 + 64 + lane] : T(0);
       t0r = (in0 && sub) ? yg[c0 + lane] : T(0);

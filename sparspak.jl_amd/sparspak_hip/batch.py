@@ -168,6 +168,34 @@ class BatchedSolver:
             out.append(x[:, 0] if np.ndim(b) == 1 else x)
         return out
 
+    def solve_device(self, d_b, nrhs=1, refine=None, tol=1e-14, max_iter=10):
+        """The stacked right-hand sides already in device memory, overwritten
+        by the solutions: d_b is a device address (int) or an object with
+        data_ptr() (a torch tensor), float64, column-major n x nrhs. The
+        library orders its reads after the caller's stream (the legacy
+        default stream unless set otherwise)."""
+        ptr = ctypes.c_void_p(d_b.data_ptr() if hasattr(d_b, "data_ptr") else int(d_b))
+        if refine is None:
+            refine = self.precision != FP64
+        L = self._dev.L
+        if refine:
+            rc = L.spkhip_solve_refine_device(self._dev.h, int(nrhs), ptr, self.n, float(tol),
+                                              int(max_iter))
+            _lib.check_refine(rc, "spkhip_solve_refine_device",
+                              self.stats().backward_error if rc == -5 else 0.0)
+        else:
+            check(L.spkhip_solve_device(self._dev.h, int(nrhs), ptr, self.n),
+                  "spkhip_solve_device")
+
+    def autotune(self, precision=FP64, reps=2):
+        """Plan knobs picked by timing this batch's factorisation (spkhip_autotune);
+        leaves the batch factored with the winner. Returns the knobs."""
+        self.precision = _PREC[precision]
+        t = _lib.Tuning()
+        check(self._dev.L.spkhip_autotune(self._dev.h, self.precision, int(reps),
+                                          ctypes.byref(t)), "spkhip_autotune")
+        return {k: getattr(t, k) for k in _lib.TUNING_KNOBS}
+
     def stats(self):
         st = _lib.Stats()
         check(self._dev.L.spkhip_stats(self._dev.h, ctypes.byref(st)), "spkhip_stats")
