@@ -208,6 +208,16 @@ int64_t spkhip_solve_refine(spkhip_t h, int64_t nrhs, double* b, int64_t ldb,
                             double tol, int64_t max_iter);
 int64_t spkhip_solve_refine_device(spkhip_t h, int64_t nrhs, double* d_b,
                                    int64_t ldb, double tol, int64_t max_iter);
+/* 1-norm condition estimate of an SPD factor: the reference's commented
+ * sparsespdainverseonenorm / sparsespdcondest (SpkSparseSpdSolver.jl:346-459),
+ * Hager's method with at most five iterations, on device solves with the
+ * current factor (double or float). cond1 = ||A||_1 * est(||A^-1||_1), with
+ * ||A||_1 the largest row sum of |a_ij| of the matrix given to
+ * spkhip_symbolic_upload (the full symmetric matrix); ainv_norm1 the
+ * estimate of ||A^-1||_1, a lower bound. -4 for an LU handle: the method
+ * needs solves with A^T, which equal the solves with A only for a symmetric
+ * factor; -3 before a factor. */
+int64_t spkhip_condest(spkhip_t h, double* cond1, double* ainv_norm1);
 /* Independent systems stacked in one factor (a block-diagonal batch): the
  * refinement's backward error is then taken per system, over the rows
  * [offsets[i], offsets[i+1]) (0-based, original numbering, offsets[0] = 0,

@@ -169,3 +169,42 @@ def factor_csc(sym, colptr, rowval, nzval, dtype=np.float64):
         return lnz, unz, ipiv, rc, ff
     rc = ldltfactor(sym, lnz)
     return lnz, None, None, rc, 0
+
+
+def ainverse_onenorm(solve, n, itmax=5):
+    """Hager's 1-norm estimate of A^-1 for a symmetric A, restated from the
+    reference's commented sparsespdainverseonenorm
+    (SpkSparseSpdSolver.jl:346-431): solve(x) returns A^-1 x (which is also
+    A^-T x). Plain numpy; test infrastructure."""
+    def sgn(v):
+        return np.where(v < 0.0, -1.0, 1.0)
+    x = solve(np.full(n, 1.0 / n))
+    if n == 1:
+        return abs(x[0])
+    est = float(np.sum(np.abs(x)))
+    x = sgn(x)
+    isgn = x.astype(np.int64)
+    x = solve(x)
+    j = int(np.argmax(np.abs(x)))
+    it = 1
+    while True:
+        it += 1
+        x = np.zeros(n)
+        x[j] = 1.0
+        x = solve(x)
+        estold, est = est, float(np.sum(np.abs(x)))
+        if np.all(sgn(x).astype(np.int64) == isgn):
+            break
+        if est <= estold:
+            break
+        x = sgn(x)
+        isgn = x.astype(np.int64)
+        x = solve(x)
+        jlast, j = j, int(np.argmax(np.abs(x)))
+        if x[jlast] == abs(x[j]) or it >= itmax:
+            break
+    i = np.arange(n, dtype=np.float64)
+    x = np.where(np.arange(n) % 2 == 0, 1.0, -1.0) * (1.0 + i / (n - 1))
+    x = solve(x)
+    temp = 2.0 * float(np.sum(np.abs(x))) / (3.0 * n)
+    return max(temp, est)

@@ -198,6 +198,11 @@ struct spkhip_handle {
   i64 first_fail = 0;
   // solve / refine work
   DevBuf<double> d_b, d_x, d_r, d_tmp64, d_norms, d_bm;
+  // condition estimate: its vector, sign vector, partial reductions
+  DevBuf<double> d_cx, d_cpsum, d_cpmax;
+  DevBuf<int32_t> d_cisgn;
+  DevBuf<int64_t> d_cpidx;
+  DevBuf<unsigned int> d_cchg;
   std::vector<i64> seg{0};            // row segments for the backward error
   // row chunks of the segments (device copies), rebuilt when seg changes
   DevBuf<i64> d_clo;
@@ -2085,9 +2090,146 @@ int64_t refine_impl(spkhip_t h, const double* d_bm, double* d_xo, int nrhs, doub
   if (!std::isfinite(eta_max)) return -6;
   return (eta_max <= tol) ? 0 : -5;
 }
+// Hager's 1-norm estimate of A^-1 as the reference's commented
+// sparsespdainverseonenorm (SpkSparseSpdSolver.jl:346-431), at most five
+// iterations, for a symmetric factor (the transposed solves it calls are the
+// solves themselves). The vector steps (fills, solves with the current
+// factor, signs, sums, the largest entry) run on the device; the host takes
+// the scalar decisions from per-block partial reductions summed in block
+// order.
+struct CondEst {
+  spkhip_handle* h;
+  int64_t n, nb, chunk;
+  std::vector<double> ps, pm;
+  std::vector<int64_t> pi;
+  explicit CondEst(spkhip_handle* hh) : h(hh), n(hh->n) {
+    chunk = std::max<int64_t>(kWG, (n + 1023) / 1024);
+    nb = (n + chunk - 1) / chunk;
+    h->d_cx.ensure((size_t)n);
+    h->d_cisgn.ensure((size_t)n);
+    h->d_cpsum.ensure((size_t)nb);
+    h->d_cpmax.ensure((size_t)nb);
+    h->d_cpidx.ensure((size_t)nb);
+    h->d_cchg.ensure(1);
+    ps.resize(nb);
+    pm.resize(nb);
+    pi.resize(nb);
+  }
+  void fill(int mode, int64_t j = 0) {
+    hipLaunchKernelGGL(spk::condest_fill_kernel, dim3(grid_for(n)), dim3(kWG), 0, h->stream,
+                       h->d_cx.p, n, mode, j);
+  }
+  void solve() {
+    if (h->precision == SPKHIP_FP64)
+      solve_device<double>(h, h->d_cx.p, n, 1);
+    else
+      solve_device<float>(h, h->d_cx.p, n, 1);
+  }
+  // sasum and isamax (0-based first index of the largest |x|)
+  void reduce(double* asum, int64_t* imax, double* vmax) {
+    hipLaunchKernelGGL(spk::condest_reduce_kernel, dim3((unsigned)nb), dim3(kWG), 0, h->stream,
+                       h->d_cx.p, n, chunk, h->d_cpsum.p, h->d_cpmax.p, h->d_cpidx.p);
+    hipck(hipMemcpyAsync(ps.data(), h->d_cpsum.p, nb * sizeof(double), hipMemcpyDeviceToHost,
+                         h->stream));
+    hipck(hipMemcpyAsync(pm.data(), h->d_cpmax.p, nb * sizeof(double), hipMemcpyDeviceToHost,
+                         h->stream));
+    hipck(hipMemcpyAsync(pi.data(), h->d_cpidx.p, nb * sizeof(int64_t), hipMemcpyDeviceToHost,
+                         h->stream));
+    sync(h);
+    double s = 0.0, m = -1.0;
+    int64_t k = 0;
+    for (int64_t b = 0; b < nb; ++b) {
+      s += ps[b];
+      if (pm[b] > m) {
+        m = pm[b];
+        k = pi[b];
+      }
+    }
+    *asum = s;
+    *imax = k;
+    *vmax = m;
+  }
+  double at(int64_t i) {
+    double v = 0;
+    hipck(hipMemcpyAsync(&v, h->d_cx.p + i, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    sync(h);
+    return v;
+  }
+  // x = sign(x), isgn = sign(x); or (compare) whether any sign differs
+  bool signs(bool compare) {
+    unsigned int chg = 0;
+    if (compare)
+      hipLaunchKernelGGL(spk::fill_kernel<unsigned int>, dim3(1), dim3(kWG), 0, h->stream,
+                         h->d_cchg.p, (int64_t)1, 0u);
+    hipLaunchKernelGGL(spk::condest_sign_kernel, dim3(grid_for(n)), dim3(kWG), 0, h->stream,
+                       h->d_cx.p, h->d_cisgn.p, n, compare ? 1 : 0, h->d_cchg.p);
+    if (!compare) return false;
+    hipck(hipMemcpyAsync(&chg, h->d_cchg.p, sizeof(chg), hipMemcpyDeviceToHost, h->stream));
+    sync(h);
+    return chg != 0;
+  }
+  double ainv_norm1() {
+    const int itmax = 5;
+    double est = 0.0, asum = 0.0, vmax = 0.0;
+    int64_t j = 0;
+    fill(0);
+    solve();
+    if (n == 1) return std::fabs(at(0));
+    reduce(&est, &j, &vmax);
+    signs(false);
+    solve();  // the transposed solve of a symmetric factor
+    reduce(&asum, &j, &vmax);
+    int iter = 1;
+    for (;;) {
+      ++iter;
+      fill(1, j);
+      solve();
+      const double estold = est;
+      int64_t jx = 0;
+      double vx = 0.0;
+      reduce(&est, &jx, &vx);  // (only the sum: j changes after the transposed solve)
+      if (!signs(true)) break;  // repeated sign vector: converged
+      if (est <= estold) break;  // cycling
+      signs(false);
+      solve();
+      const int64_t jlast = j;
+      reduce(&asum, &j, &vmax);
+      if (at(jlast) == vmax || iter >= itmax) break;
+    }
+    fill(2);
+    solve();
+    reduce(&asum, &j, &vmax);
+    const double temp = 2.0 * asum / (3.0 * (double)n);
+    return temp > est ? temp : est;
+  }
+};
+
 }  // namespace
 
 extern "C" {
+
+int64_t spkhip_condest(spkhip_t h, double* cond1, double* ainv_norm1) {
+  if (guard(h)) return -4;
+  if (h->kind != SPKHIP_KIND_SPD) return -4;
+  if (!h->factored) return -3;
+  return wrap([&]() -> int64_t {
+    hipck(hipSetDevice(h->device));
+    CondEst ce(h);
+    const double ainv = ce.ainv_norm1();
+    // ||A||_1 of the symmetric matrix: its largest row sum of |a_ij|
+    hipLaunchKernelGGL(spk::condest_reduce_kernel, dim3((unsigned)ce.nb), dim3(kWG), 0,
+                       h->stream, h->d_rowabs.p, h->n, ce.chunk, h->d_cpsum.p, h->d_cpmax.p,
+                       h->d_cpidx.p);
+    hipck(hipMemcpyAsync(ce.pm.data(), h->d_cpmax.p, ce.nb * sizeof(double),
+                         hipMemcpyDeviceToHost, h->stream));
+    sync(h);
+    double anorm = 0.0;
+    for (double v : ce.pm) anorm = std::max(anorm, v);
+    if (ainv_norm1) *ainv_norm1 = ainv;
+    if (cond1) *cond1 = anorm * ainv;
+    return 0;
+  });
+}
 
 int64_t spkhip_solve_refine_device(spkhip_t h, int64_t nrhs, double* d_b, int64_t ldb, double tol,
                                    int64_t max_iter) {

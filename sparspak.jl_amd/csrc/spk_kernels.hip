@@ -2689,6 +2689,83 @@ __global__ void axpy_kernel(double* __restrict__ x, int64_t ldx,
   for (int q = 0; q < nrhs; ++q) x[i + q * ldx] += (double)d[i + q * ldd];
 }
 
+// ---------------------------------------------------------------------------
+// The 1-norm estimate of A^-1 (Hager's method, the reference's commented
+// sparsespdainverseonenorm, SpkSparseSpdSolver.jl:346-431): its vector steps.
+// ---------------------------------------------------------------------------
+// mode 0: x_i = 1/n; mode 1: the unit vector e_j; mode 2: x_i = (-1)^i (1 +
+// i/(n-1)) (0-based i: the reference's altsgn vector).
+__global__ void condest_fill_kernel(double* __restrict__ x, int64_t n, int mode, int64_t j) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v;
+  if (mode == 0)
+    v = 1.0 / (double)n;
+  else if (mode == 1)
+    v = (i == j) ? 1.0 : 0.0;
+  else
+    v = ((i & 1) ? -1.0 : 1.0) * (1.0 + (double)i / (double)(n - 1));
+  x[i] = v;
+}
+
+// Per-block partial results over [b chunk, (b+1) chunk): the sum of |x|
+// (a fixed tree order), the largest |x| and its first index.
+__global__ void condest_reduce_kernel(const double* __restrict__ x, int64_t n, int64_t chunk,
+                                      double* __restrict__ psum, double* __restrict__ pmax,
+                                      int64_t* __restrict__ pidx) {
+  __shared__ double ss[kWG], sm[kWG];
+  __shared__ int64_t si[kWG];
+  const int64_t lo = (int64_t)blockIdx.x * chunk, hi = min(n, lo + chunk);
+  double sum = 0.0, mx = -1.0;
+  int64_t mi = lo;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kWG) {
+    const double a = fabs(x[i]);
+    sum += a;
+    if (a > mx) {
+      mx = a;
+      mi = i;
+    }
+  }
+  ss[threadIdx.x] = sum;
+  sm[threadIdx.x] = mx;
+  si[threadIdx.x] = mi;
+  __syncthreads();
+  for (int off = kWG / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      ss[threadIdx.x] += ss[threadIdx.x + off];
+      const double m2 = sm[threadIdx.x + off];
+      const int64_t i2 = si[threadIdx.x + off];
+      if (m2 > sm[threadIdx.x] || (m2 == sm[threadIdx.x] && i2 < si[threadIdx.x])) {
+        sm[threadIdx.x] = m2;
+        si[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    psum[blockIdx.x] = ss[0];
+    pmax[blockIdx.x] = sm[0];
+    pidx[blockIdx.x] = si[0];
+  }
+}
+
+// Signs s_i = sign(1, x_i) (+1 for x_i >= 0): compare mode counts the
+// entries whose sign differs from isgn; otherwise x and isgn become the
+// signs.
+__global__ void condest_sign_kernel(double* __restrict__ x, int32_t* __restrict__ isgn,
+                                    int64_t n, int compare,
+                                    unsigned int* __restrict__ changed) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int sg = (x[i] < 0.0) ? -1 : 1;
+  if (compare) {
+    if (sg != isgn[i]) atomicAdd(changed, 1u);
+  } else {
+    x[i] = (double)sg;
+    isgn[i] = sg;
+  }
+}
+
 // Factor conversions for download.
 template <typename TI, typename TO>
 __global__ void convert_kernel(const TI* __restrict__ in, TO* __restrict__ out,

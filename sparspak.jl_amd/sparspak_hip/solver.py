@@ -203,6 +203,12 @@ class _SparseBase:
         self.precision = _PREC[precision]
         return {f: getattr(t, f) for f in _lib.TUNING_KNOBS + ("factor_ms",)}
 
+    def _condest(self):
+        c, a = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        check(self._dev.L.spkhip_condest(self._dev.h, ctypes.byref(c), ctypes.byref(a)),
+              "spkhip_condest")
+        return c.value, a.value
+
     def stats(self):
         st = _lib.Stats()
         check(self._dev.L.spkhip_stats(self._dev.h, ctypes.byref(st)), "spkhip_stats")
@@ -371,6 +377,22 @@ def autotune(s, precision=FP32, reps=2):
     s.precision = _PREC[precision]
     s._factordone = True
     return best
+
+
+def condest(s):
+    """sparsespdcondest / sparsespdainverseonenorm (SpkSparseSpdSolver.jl:346-459,
+    commented Fortran in the reference): for a factored SparseSpdSolver, the
+    1-norm condition number estimate ||A||_1 est(||A^-1||_1) with est from
+    Hager's method (at most five iterations) on device solves. Returns
+    (estimate of cond_1(A), estimate of ||A^-1||_1, a lower bound)."""
+    if s.kind != "spd":
+        raise ValueError("condest needs a SparseSpdSolver (its solves with A^T are "
+                         "the solves with A)")
+    if not s._factordone:
+        raise RuntimeError("Sequence error. Factorization not done yet.")
+    cond, ainv = s.slvr._condest()
+    s._condestdone = True
+    return cond, ainv
 
 
 def triangularsolve(s, rhs=None, refine=None, tol=1e-14, max_iter=10):

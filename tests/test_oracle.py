@@ -223,3 +223,23 @@ def test_ldlt_wide_supernodes():
     b = np.ones(130)
     x = orc.solve(s, lnz, None, None, b.copy())
     assert backward_error(A, x, b) < 1e-14
+
+
+def test_oracle_hager_estimate_pins():
+    """The oracle's Hager restatement (ainverse_onenorm): exact on a diagonal
+    matrix (||A^-1||_1 = max 1/|d_i|, found at the first unit-vector step),
+    and a lower bound within 3x of the exact ||A^-1||_1 on symmetric
+    matrices (dense inverses as the solve)."""
+    import oracle.oracle as orc
+    d = np.array([4.0, -0.5, 2.0, 8.0, 0.25, 3.0])
+    est = orc.ainverse_onenorm(lambda b: b / d, d.size)
+    assert est == pytest.approx(4.0, rel=1e-15)
+    rng = np.random.default_rng(11)
+    for n in (5, 17, 40):
+        M = rng.uniform(-1, 1, (n, n))
+        A = M + M.T + np.diag(np.abs(M).sum(axis=1) * 2 + 0.1)
+        Ai = np.linalg.inv(A)
+        exact = np.abs(Ai).sum(axis=0).max()
+        est = orc.ainverse_onenorm(lambda b: Ai @ b, n)
+        assert est <= exact * (1 + 1e-12)
+        assert est >= exact / 3.0
