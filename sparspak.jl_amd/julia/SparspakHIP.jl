@@ -5,6 +5,8 @@
 # include/sparspak_hip.h one to one.
 module SparspakHIP
 
+using Sparspak.SpkProblem: Problem
+
 const libspk = get(ENV, "SPARSPAK_HIP_LIB",
                    joinpath(@__DIR__, "..", "sparspak_hip", "libsparspak_hip.so"))
 
@@ -115,6 +117,51 @@ end
 inmatrix!(h::Handle, m) = GC.@preserve m check(
     ccall((:spkhip_inmatrix, libspk), Int64, (Ptr{Cvoid}, Ptr{Float64}), h.ptr, m.nzval),
     "spkhip_inmatrix")
+
+# A reference Problem (SpkProblem.jl:105-139) as the CSC arrays the upload
+# takes: each column's linked list from head[c] through link, rows already
+# ascending (inaij! keeps them sorted), values as stored.
+function problem_csc(p)
+    n = p.ncols
+    colptr = ones(Int64, n + 1)
+    rowval = Int64[]
+    nzval = Float64[]
+    sizehint!(rowval, p.nedges)
+    sizehint!(nzval, p.nedges)
+    for c in 1:n
+        e = p.head[c]
+        while e > 0
+            push!(rowval, p.rowsubs[e])
+            push!(nzval, p.values[e])
+            e = p.link[e]
+        end
+        colptr[c + 1] = length(rowval) + 1
+    end
+    return colptr, rowval, nzval
+end
+
+# SparseSolver(p::Problem) / SparseSpdSolver(p::Problem) after
+# symbolicfactor!: the symbolic factor and the Problem's pattern to the device
+function upload!(h::Handle, s, p::Problem; kind=SPKHIP_KIND_LU)
+    colptr, rowval, _ = problem_csc(p)
+    upload!(h, s, (colptr=colptr, rowval=rowval); kind=kind)
+end
+
+# the Problem's values (the lists walked in the same order as the upload)
+function inmatrix!(h::Handle, p::Problem)
+    _, _, nzval = problem_csc(p)
+    inmatrix!(h, (nzval=nzval,))
+end
+
+# sparsespdcondest (SpkSparseSpdSolver.jl:433-459, commented in the
+# reference): (estimate of cond_1(A), estimate of ||A^-1||_1) of an SPD factor
+function condest(h::Handle)
+    c = Ref(0.0)
+    a = Ref(0.0)
+    check(ccall((:spkhip_condest, libspk), Int64, (Ptr{Cvoid}, Ref{Float64}, Ref{Float64}),
+                h.ptr, c, a), "spkhip_condest")
+    return c[], a[]
+end
 
 factor!(h::Handle; precision=SPKHIP_FP64) = check(
     ccall((:spkhip_factor, libspk), Int64, (Ptr{Cvoid}, Int64), h.ptr, precision),
