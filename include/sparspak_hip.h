@@ -102,6 +102,21 @@ int64_t spk_front_maps(int64_t n, int64_t nsuper, const int64_t* xsuper,
                        const int64_t* lindx, int64_t* fsp0, int64_t* fnp,
                        int64_t* fparent, int64_t* frel, int64_t* relmap);
 
+/* The map from each entry of A (CSC, original numbering, nnz = colptr[n]-1)
+ * to its slot, as spkhip_symbolic_upload builds it for _inmatrix!
+ * (SpkSparseBase.jl:302-372, SparseCSCInterface.jl:112-163, SPD
+ * SpkSparseSpdBase.jl:274-282): amap[k] >= 0 is the 0-based lnz slot,
+ * amap[k] <= -2 the unz slot -(amap[k] + 2), -1 an entry not stored (the
+ * index list ran out, or an SPD entry above the diagonal). Returns -1 ("No
+ * space for matrix element": a larger index met first), else 0. nthreads:
+ * host threads (0: SPKHIP_HOST_THREADS or the hardware's, at most 16); any
+ * count gives the same map. No device is used. */
+int64_t spk_amap(int64_t n, int64_t kind, int64_t nsuper, const int64_t* xsuper,
+                 const int64_t* snode, const int64_t* xlindx, const int64_t* lindx,
+                 const int64_t* xlnz, const int64_t* xunz, const int64_t* invp,
+                 const int64_t* colptr, const int64_t* rowval, int64_t nthreads,
+                 int64_t* amap);
+
 /* ======================================================================
  * Device numeric phase.
  * ====================================================================== */

@@ -11,6 +11,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -781,8 +782,17 @@ void wait_caller(spkhip_handle* h) {
 // entry ignored. The index lists are sorted, so a binary search replaces the
 // linear scan. SPD keeps only entries on or below the diagonal
 // (SpkSparseSpdBase.jl:274-282) and puts them in the lower triangle.
+// Host threads for the A -> slot map: SPKHIP_HOST_THREADS, else the
+// hardware's, at most 16; one thread below 2^18 entries.
+int host_threads(i64 work) {
+  if (work < (1 << 18)) return 1;
+  const char* e = std::getenv("SPKHIP_HOST_THREADS");
+  int t = e ? std::atoi(e) : (int)std::thread::hardware_concurrency();
+  return std::max(1, std::min(t, 16));
+}
+
 void build_amap(spkhip_handle* h, const i64* colptr, const i64* rowval,
-                std::vector<i64>& amap) {
+                std::vector<i64>& amap, int nthreads = 0) {
   const i64 n = h->n;
   amap.assign(h->nnz, -1);
   h->pattern_error = false;
@@ -800,29 +810,58 @@ void build_amap(spkhip_handle* h, const i64* colptr, const i64* rowval,
     loc = it - b;
     return 1;
   };
-  for (i64 i = 1; i <= n; ++i) {
-    for (i64 iptr = colptr[i - 1]; iptr <= colptr[i] - 1; ++iptr) {
-      i64 k = iptr - 1;
-      i64 orow = rowval[k];
-      if (h->kind == SPKHIP_KIND_SPD && orow < i) continue;
-      i64 inew = h->invp[orow - 1];
-      i64 jnew = h->invp[i - 1];
-      if (h->kind == SPKHIP_KIND_SPD && inew < jnew) std::swap(inew, jnew);
-      i64 loc = 0;
-      i64 js = h->snode[jnew - 1];
-      if (inew >= h->xsuper[js - 1]) {
-        int r = search(js - 1, 0, inew, loc);
-        if (r < 0) h->pattern_error = true;
-        if (r > 0) amap[k] = h->xlnz[jnew - 1] - 1 + loc;
-      } else {
-        i64 is = h->snode[inew - 1];
-        i64 width = h->xsuper[is] - h->xsuper[is - 1];
-        int r = search(is - 1, width, jnew, loc);
-        if (r < 0) h->pattern_error = true;
-        if (r > 0) amap[k] = -(h->xunz[inew - 1] - 1 + loc) - 2;
+  // columns [c0, c1): every entry's slot depends on that entry alone, so the
+  // columns split across threads give the same map as one pass
+  auto columns = [&](i64 c0, i64 c1, bool* err) {
+    for (i64 i = c0; i < c1; ++i) {
+      for (i64 iptr = colptr[i - 1]; iptr <= colptr[i] - 1; ++iptr) {
+        i64 k = iptr - 1;
+        i64 orow = rowval[k];
+        if (h->kind == SPKHIP_KIND_SPD && orow < i) continue;
+        i64 inew = h->invp[orow - 1];
+        i64 jnew = h->invp[i - 1];
+        if (h->kind == SPKHIP_KIND_SPD && inew < jnew) std::swap(inew, jnew);
+        i64 loc = 0;
+        i64 js = h->snode[jnew - 1];
+        if (inew >= h->xsuper[js - 1]) {
+          int r = search(js - 1, 0, inew, loc);
+          if (r < 0) *err = true;
+          if (r > 0) amap[k] = h->xlnz[jnew - 1] - 1 + loc;
+        } else {
+          i64 is = h->snode[inew - 1];
+          i64 width = h->xsuper[is] - h->xsuper[is - 1];
+          int r = search(is - 1, width, jnew, loc);
+          if (r < 0) *err = true;
+          if (r > 0) amap[k] = -(h->xunz[inew - 1] - 1 + loc) - 2;
+        }
       }
     }
+  };
+  const int nt = nthreads > 0 ? nthreads : host_threads(h->nnz);
+  if (nt <= 1) {
+    bool err = false;
+    columns(1, n + 1, &err);
+    h->pattern_error = err;
+    return;
   }
+  // column ranges of about nnz / nt entries each
+  std::vector<i64> cut(nt + 1, n + 1);
+  cut[0] = 1;
+  for (int t = 1, i = 1; t < nt; ++t) {
+    const i64 target = 1 + (h->nnz * t) / nt;
+    while (i <= n && colptr[i - 1] < target) ++i;
+    cut[t] = std::max<i64>(i, cut[t - 1]);
+  }
+  std::vector<char> errs(nt, 0);
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; ++t)
+    pool.emplace_back([&, t]() {
+      bool e = false;
+      columns(cut[t], cut[t + 1], &e);
+      errs[t] = e ? 1 : 0;
+    });
+  for (auto& th : pool) th.join();
+  for (char e : errs) h->pattern_error = h->pattern_error || e;
 }
 
 template <typename T>
@@ -2327,6 +2366,38 @@ int64_t spk_front_maps(int64_t n, int64_t nsuper, const int64_t* xsuper, const i
     frel[hh.nfront] = nrel + 1;
     for (i64 k = 0; k < nrel; ++k) relmap[k] = hh.relmap[k];
     return hh.nfront;
+  });
+}
+
+int64_t spk_amap(int64_t n, int64_t kind, int64_t nsuper, const int64_t* xsuper,
+                 const int64_t* snode, const int64_t* xlindx, const int64_t* lindx,
+                 const int64_t* xlnz, const int64_t* xunz, const int64_t* invp,
+                 const int64_t* colptr, const int64_t* rowval, int64_t nthreads,
+                 int64_t* amap) {
+  if (n <= 0 || nsuper <= 0 || !xsuper || !snode || !xlindx || !lindx || !xlnz || !invp ||
+      !colptr || !rowval || !amap || (kind == SPKHIP_KIND_LU && !xunz) || nthreads < 0)
+    return -4;
+  if (kind != SPKHIP_KIND_LU && kind != SPKHIP_KIND_SPD) return -4;
+  return wrap([&]() -> int64_t {
+    // the map spkhip_symbolic_upload builds, on a host-only handle
+    spkhip_handle hh;
+    hh.n = n;
+    hh.kind = kind;
+    hh.nsuper = nsuper;
+    hh.xsuper.assign(xsuper, xsuper + nsuper + 1);
+    hh.snode.assign(snode, snode + n);
+    hh.xlindx.assign(xlindx, xlindx + nsuper + 1);
+    hh.nsub = hh.xlindx[nsuper] - 1;
+    hh.lindx.resize(std::max<i64>(hh.nsub, 1));
+    for (i64 k = 0; k < hh.nsub; ++k) hh.lindx[k] = (i32)lindx[k];
+    hh.xlnz.assign(xlnz, xlnz + n + 1);
+    if (kind == SPKHIP_KIND_LU) hh.xunz.assign(xunz, xunz + n + 1);
+    hh.invp.assign(invp, invp + n);
+    hh.nnz = colptr[n] - 1;
+    std::vector<i64> m;
+    build_amap(&hh, colptr, rowval, m, (int)nthreads);
+    for (i64 k = 0; k < hh.nnz; ++k) amap[k] = m[k];
+    return hh.pattern_error ? -1 : 0;
   });
 }
 

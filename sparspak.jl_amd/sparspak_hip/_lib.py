@@ -70,6 +70,8 @@ _SIGS = {
     "spk_supernode_tree": (None, [i64, i64p, i64p, i64p, i64p, i64p]),
     "spk_front_maps": (i64, [i64, i64, i64p, i64p, i64p, i64p, i64p, i64p, i64p, i64p,
                              i64p]),
+    "spk_amap": (i64, [i64, i64, i64, i64p, i64p, i64p, i64p, i64p, i64p, i64p, i64p,
+                       i64p, i64, i64p]),
     # device numeric phase
     "spkhip_create": (i64, [ctypes.POINTER(vp), i64]),
     "spkhip_destroy": (i64, [vp]),
