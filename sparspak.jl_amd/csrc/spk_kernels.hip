@@ -2226,16 +2226,14 @@ struct FwdStep {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(kWG)
-fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ prep,
-                 const int32_t* __restrict__ prepi, T* __restrict__ rhs, int64_t ldr, int nrhs,
-                 T* __restrict__ vec) {
+__device__ __attribute__((always_inline)) void fwd_chain_body(
+    const DevSym& s, const SolveDesc& d, const T* __restrict__ prep,
+    const int32_t* __restrict__ prepi, T* __restrict__ rhs, int64_t ldr, int nrhs,
+    T* __restrict__ vec, unsigned char* smem_a32) {
   using V = typename Vec4<T>::type;
-  extern __shared__ __attribute__((aligned(32))) unsigned char smem_a32[];
   V* smem_v4 = reinterpret_cast<V*>(smem_a32);
   const T* R = reinterpret_cast<const T*>(smem_a32);
   const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
-  const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
   const int L = d.L;
   const int64_t fc0 = d.fc0;
@@ -2332,20 +2330,28 @@ fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ prep,
+                 const int32_t* __restrict__ prepi, T* __restrict__ rhs, int64_t ldr, int nrhs,
+                 T* __restrict__ vec) {
+  extern __shared__ __attribute__((aligned(32))) unsigned char smem_a32[];
+  fwd_chain_body<T>(s, desc[blockIdx.x], prep, prepi, rhs, ldr, nrhs, vec, smem_a32);
+}
+
 // Forward: y[c1 + r0 + lane] -= L[., c0:c1] y[c0:c1]; wave w takes a
 // quarter of the block's columns, partial sums added in a fixed order. The
 // block's column offsets and y[c0:c1] are staged in LDS first, then every
 // thread issues all of its L loads at once.
 template <typename T>
-__global__ void __launch_bounds__(kWG)
-fwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ lnz,
-                 int nrhs, T* __restrict__ vec) {
+__device__ __attribute__((always_inline)) void fwd_bgemv_body(
+    const DevSym& s, const SolveDesc& d, const T* __restrict__ lnz, int nrhs,
+    T* __restrict__ vec) {
   __shared__ T red[4][64];
   __shared__ int64_t lc[128];
   __shared__ T zs[128];
   const int tid = threadIdx.x, lane = tid % 64;
   const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
-  const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
   const int L = d.L;
   const int64_t fc0 = d.fc0;
@@ -2378,6 +2384,13 @@ fwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
 }
 
 
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+fwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ lnz,
+                 int nrhs, T* __restrict__ vec) {
+  fwd_bgemv_body<T>(s, desc[blockIdx.x], lnz, nrhs, vec);
+}
+
 // Backward: t[c] = U[c, c1:L] x[c1:L] for row c = c0 + r0 of the block, one
 // workgroup per row, stored in y[c]. x[c1:L] is read from the front's vector:
 // its update rows from bwd_gather_items_kernel, its own columns as the backward
@@ -2385,13 +2398,13 @@ fwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
 // + 256 m, sixteen loads in flight; per-thread sums are added over the wave,
 // then the four waves in a fixed order.
 template <typename T>
-__global__ void __launch_bounds__(kWG)
-bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ unz,
-                 int nrhs, T* __restrict__ vec) {
-  constexpr int QB = 8;  // right-hand sides sharing one read of the row
+__device__ __attribute__((always_inline)) void bwd_bgemv_body(
+    const DevSym& s, const SolveDesc& d, const T* __restrict__ unz, int nrhs,
+    T* __restrict__ vec) {
+  constexpr int QB = 8;   // right-hand sides sharing one read of the row
+  constexpr int CH = 16;  // columns per thread and pass
   __shared__ T red[QB][4];
   const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
-  const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
   const int c = d.c0 + d.r0;
   const int L = d.L;
@@ -2402,10 +2415,10 @@ bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
     T acc[QB];
 #pragma unroll
     for (int qq = 0; qq < QB; ++qq) acc[qq] = T(0);
-    for (int j0 = d.c1 + tid; j0 < L; j0 += 16 * kWG) {
-      T uv[16];
+    for (int j0 = d.c1 + tid; j0 < L; j0 += CH * kWG) {
+      T uv[CH];
 #pragma unroll
-      for (int t = 0; t < 16; ++t) {
+      for (int t = 0; t < CH; ++t) {
         const int j = j0 + kWG * t;
         uv[t] = (j < L) ? u[j] : T(0);
       }
@@ -2413,14 +2426,14 @@ bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
       for (int qq = 0; qq < QB; ++qq) {
         if (qq < nq) {
           const T* yg = vec + d.fvec * nrhs + (int64_t)(q0 + qq) * L;
-          T xv[16];
+          T xv[CH];
 #pragma unroll
-          for (int t = 0; t < 16; ++t) {
+          for (int t = 0; t < CH; ++t) {
             const int j = j0 + kWG * t;
             xv[t] = (j < L) ? yg[j] : T(0);
           }
 #pragma unroll
-          for (int t = 0; t < 16; ++t) acc[qq] += uv[t] * xv[t];
+          for (int t = 0; t < CH; ++t) acc[qq] += uv[t] * xv[t];
         }
       }
     }
@@ -2439,6 +2452,13 @@ bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
     }
     __syncthreads();
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ unz,
+                 int nrhs, T* __restrict__ vec) {
+  bwd_bgemv_body<T>(s, desc[blockIdx.x], unz, nrhs, vec);
 }
 
 // b / u from r = 1 / u (correctly rounded): q0 = b r, then one correction
@@ -2471,15 +2491,13 @@ struct BwdStep {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(kWG)
-bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ prep,
-                 T* __restrict__ rhs, int64_t ldr, int nrhs, T* __restrict__ vec) {
+__device__ __attribute__((always_inline)) void bwd_chain_body(
+    const DevSym& s, const SolveDesc& d, const T* __restrict__ prep, T* __restrict__ rhs,
+    int64_t ldr, int nrhs, T* __restrict__ vec, unsigned char* smem_a32) {
   using V = typename Vec4<T>::type;
-  extern __shared__ __attribute__((aligned(32))) unsigned char smem_a32[];
   V* smem_v4 = reinterpret_cast<V*>(smem_a32);
   const T* R = reinterpret_cast<const T*>(smem_a32);
   const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
-  const SolveDesc d = desc[blockIdx.x];
   const int f = d.front;
   const int L = d.L;
   const int64_t fc0 = d.fc0;
@@ -2592,6 +2610,14 @@ bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
       yg[c0 + 64 + lane] = y1;
     }
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ prep,
+                 T* __restrict__ rhs, int64_t ldr, int nrhs, T* __restrict__ vec) {
+  extern __shared__ __attribute__((aligned(32))) unsigned char smem_a32[];
+  bwd_chain_body<T>(s, desc[blockIdx.x], prep, rhs, ldr, nrhs, vec, smem_a32);
 }
 
 // ---------------------------------------------------------------------------
