@@ -107,6 +107,11 @@ struct LevelPlan {
     double elems = 0;               // factor entries read by the step
   };
   std::vector<SolveStep> fsteps, bsteps;
+  // explicit-inverse block steps of the float solves (ranges of inverse
+  // items): one launch per forward step, per backward step, and the
+  // backward sweep's finishing launch
+  std::vector<std::pair<int, int>> ifw, ibw;
+  std::pair<int, int> ifin{0, 0};
   std::vector<Group> groups;
   int solve_off = 0, solve_cnt = 0;
 };
@@ -180,6 +185,13 @@ struct spkhip_handle {
   DevBuf<float> d_prep32;
   DevBuf<i32> d_prepi;
   bool prepped = false;
+  // explicit inverses of the diagonal blocks (float solves): two bwp x bwp
+  // matrices per block, built after the other solve data
+  std::vector<spk::InvItem> iitems;
+  DevBuf<spk::InvItem> d_iitems;
+  i64 inv_elems = 0;
+  DevBuf<float> d_inv32;
+  bool inv_prepped = false;
   DevSym ds{};
   // matrix
   i64 nnz = 0;
@@ -471,10 +483,13 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
                 std::vector<i32>& ut_piece, std::vector<i32>& ut_r0) {
   const i32 nf = h->nfront;
   h->pdesc.clear();
+  h->iitems.clear();
   h->wb_front.clear();
   h->wb_col.clear();
   h->prep_units = 0;
+  h->inv_elems = 0;
   h->prepped = false;
+  h->inv_prepped = false;
   std::vector<std::vector<i32>> bylev(h->nlevel);
   for (i32 f = 0; f < nf; ++f) bylev[h->flev[f]].push_back(f);
   h->levels.assign(h->nlevel, LevelPlan{});
@@ -707,10 +722,12 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
     std::vector<std::vector<spk::SolveDesc>> bdata(large.size());
     for (size_t fi = 0; fi < large.size(); ++fi)
       for (const Blk& B : sblocks[fi]) {
-        const spk::SolveDesc D = sd(large[fi], B, (i32)h->pdesc.size(), (i32)h->prep_units);
+        spk::SolveDesc D = sd(large[fi], B, (i32)h->pdesc.size(), (i32)h->prep_units);
+        const i64 ng = (B.c1 - B.c0 + 3) / 4;
+        D.inv = h->inv_elems;
+        h->inv_elems += 2 * (4 * ng) * (4 * ng);
         h->pdesc.push_back(D);
         bdata[fi].push_back(D);
-        const i64 ng = (B.c1 - B.c0 + 3) / 4;
         h->prep_units += 4 * ng * (ng + 1) + ng;
       }
     for (size_t gi = 0; gi < maxgs; ++gi) {
@@ -720,26 +737,35 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         if (gi < sblocks[fi].size()) sdesc.push_back(bdata[fi][gi]);
       S.chain_cnt = (int)sdesc.size() - S.chain_off;
       S.gemv_off = (int)sdesc.size();
+      lp.ifw.emplace_back((int)h->iitems.size(), 0);
       for (size_t fi = 0; fi < large.size(); ++fi)
         if (gi < sblocks[fi].size()) {
           const Blk& B = sblocks[fi][gi];
           const i32 t = h->fL[large[fi]] - B.c1;
-          for (i32 r0 = 0; r0 < t; r0 += 64)
+          const i32 blk = bdata[fi][gi].r0;
+          if (t <= 0) h->iitems.push_back(spk::InvItem{-1, blk, 1, 0});
+          for (i32 r0 = 0; r0 < t; r0 += 64) {
+            h->iitems.push_back(spk::InvItem{(i32)sdesc.size(), blk, r0 == 0, 0});
             sdesc.push_back(sd(large[fi], B, r0, 0));
+          }
           S.elems += (double)(B.c1 - B.c0) * (h->fL[large[fi]] - B.c0);
         }
+      lp.ifw.back().second = (int)h->iitems.size() - lp.ifw.back().first;
       S.gemv_cnt = (int)sdesc.size() - S.gemv_off;
       lp.fsteps.push_back(S);
     }
     for (size_t si = 0; si < maxgs; ++si) {
       LevelPlan::SolveStep S;
       S.gemv_off = (int)sdesc.size();
+      lp.ibw.emplace_back((int)h->iitems.size(), 0);
       for (size_t fi = 0; fi < large.size(); ++fi) {
         const i64 bi = (i64)sblocks[fi].size() - 1 - (i64)si;
         if (bi < 0) continue;
         const Blk& B = sblocks[fi][bi];
         if (B.c1 >= h->fL[large[fi]]) continue;
+        const i32 nxt = bi + 1 < (i64)sblocks[fi].size() ? bdata[fi][bi + 1].r0 : -1;
         for (i32 r0 = 0; r0 < B.c1 - B.c0; ++r0) {
+          h->iitems.push_back(spk::InvItem{(i32)sdesc.size(), nxt, r0 == 0 && nxt >= 0, 0});
           spk::SolveDesc D = sd(large[fi], B, r0, 0);
           // unz offset of the row in front coordinates (as DevSym::ucol)
           const i64 g = h->fc0[large[fi]] + B.c0 + r0;
@@ -750,6 +776,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         }
       }
       S.gemv_cnt = (int)sdesc.size() - S.gemv_off;
+      lp.ibw.back().second = (int)h->iitems.size() - lp.ibw.back().first;
       S.chain_off = (int)sdesc.size();
       for (size_t fi = 0; fi < large.size(); ++fi) {
         const i64 bi = (i64)sblocks[fi].size() - 1 - (i64)si;
@@ -761,10 +788,21 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
       S.chain_cnt = (int)sdesc.size() - S.chain_off;
       lp.bsteps.push_back(S);
     }
+    lp.ifin.first = (int)h->iitems.size();
+    for (size_t fi = 0; fi < large.size(); ++fi)
+      h->iitems.push_back(spk::InvItem{-1, bdata[fi][0].r0, 0, 1});
+    lp.ifin.second = (int)large.size();
   }
 }
 
 void sync(spkhip_handle* h) { hipck(hipStreamSynchronize(h->stream)); }
+
+// Float solves use the explicit inverses of the diagonal blocks
+// (SPKHIP_SOLVE_INV=0: substitution chains, as the double solves).
+bool inv_solves() {
+  const char* e = std::getenv("SPKHIP_SOLVE_INV");
+  return !(e && e[0] == '0');
+}
 
 // The handle's streams do not synchronise with other streams by themselves
 // (non-blocking): before an entry point reads a caller's device buffer, the
@@ -908,6 +946,9 @@ void set_lds_attrs() {
   hipck(hipFuncSetAttribute((const void*)spk::bwd_chain_kernel<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize,
                             (spk::kPanelElems + 128) * (int)sizeof(T)));
+  if constexpr (std::is_same<T, float>::value)
+    hipck(hipFuncSetAttribute((const void*)spk::solve_inv_prep_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, spk::kInvPrepLds));
   done = true;
 }
 
@@ -1103,28 +1144,47 @@ void size_solve_data(spkhip_handle* h) {
   if (h->pdesc.empty()) return;
   Bufs<T>::prep(h).ensure((size_t)h->prep_units * 4);
   h->d_prepi.ensure(h->pdesc.size() * spk::kPrepInts);
+  if (std::is_same<T, float>::value && inv_solves()) h->d_inv32.ensure((size_t)h->inv_elems);
 }
 template <typename T>
 bool solve_data_sized(spkhip_handle* h) {
-  return h->pdesc.empty() || (Bufs<T>::prep(h).n >= (size_t)h->prep_units * 4 &&
-                              h->d_prepi.n >= h->pdesc.size() * spk::kPrepInts);
+  return h->pdesc.empty() ||
+         (Bufs<T>::prep(h).n >= (size_t)h->prep_units * 4 &&
+          h->d_prepi.n >= h->pdesc.size() * spk::kPrepInts &&
+          !(std::is_same<T, float>::value && inv_solves() &&
+            h->d_inv32.n < (size_t)h->inv_elems));
 }
 
-// The per-block solve data of the current factor (once per factorisation).
+// The per-block solve data of the current factor (once per factorisation),
+// and for float factors in inverse mode the blocks' explicit inverses.
 template <typename T>
 void solve_prep(spkhip_handle* h) {
-  if (h->prepped || h->pdesc.empty()) return;
+  if (h->pdesc.empty()) return;
   DevBuf<T>& prep = Bufs<T>::prep(h);
-  size_solve_data<T>(h);
-  Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
-  hipLaunchKernelGGL(spk::solve_prep_kernel<T>, dim3((unsigned)h->pdesc.size()), dim3(kWG), 0,
-                     h->stream, h->ds, h->d_pdesc.p, Bufs<T>::lnz(h).p, Bufs<T>::unz(h).p,
-                     h->d_ipvt.p, prep.p, h->d_prepi.p);
-  hipck(hipGetLastError());
-  h->prepped = true;
+  if (!h->prepped) {
+    size_solve_data<T>(h);
+    Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
+    hipLaunchKernelGGL(spk::solve_prep_kernel<T>, dim3((unsigned)h->pdesc.size()), dim3(kWG), 0,
+                       h->stream, h->ds, h->d_pdesc.p, Bufs<T>::lnz(h).p, Bufs<T>::unz(h).p,
+                       h->d_ipvt.p, prep.p, h->d_prepi.p);
+    hipck(hipGetLastError());
+    h->prepped = true;
+  }
+  if constexpr (std::is_same<T, float>::value) {
+    if (inv_solves() && !h->inv_prepped) {
+      size_solve_data<T>(h);
+      Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
+      hipLaunchKernelGGL(spk::solve_inv_prep_kernel, dim3((unsigned)h->pdesc.size()), dim3(kWG),
+                         (size_t)spk::kInvPrepLds, h->stream, h->d_pdesc.p, prep.p,
+                         h->d_prepi.p, h->d_inv32.p);
+      hipck(hipGetLastError());
+      h->inv_prepped = true;
+    }
+  }
 }
 
-// vecp: the per-front vector arena, h->vec_elems * nrhs elements.
+// vecp: the per-front vector arena, h->vec_elems * nrhs elements. Float
+// factors take the explicit-inverse block steps unless SPKHIP_SOLVE_INV=0.
 template <typename T, int MAXW, int GW>
 void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bwd, T* vecp) {
   hipStream_t st = h->stream;
@@ -1133,6 +1193,7 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
   const T* unz = Bufs<T>::unz(h).p;
   DevBuf<T>& prep = Bufs<T>::prep(h);
   solve_prep<T>(h);
+  const bool inv = std::is_same<T, float>::value && inv_solves();
   if (fwd)
     for (const LevelPlan& lp : h->levels) {
       h->cur_level = (int)(&lp - h->levels.data());
@@ -1157,6 +1218,20 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
                            h->d_it_ea.p + lp.items_off, h->d_ea_child.p, h->d_ea_t.p, rhs, ldr,
                            nrhs, vecp, lp.items_cnt);
         hipck(hipGetLastError());
+      }
+      if constexpr (std::is_same<T, float>::value) {
+        if (inv) {
+          for (size_t k = 0; k < lp.ifw.size(); ++k) {
+            const double el = lp.fsteps[k].elems;
+            Prof pr(h, SPKHIP_K_FWD_GEMV, 2.0 * el * nrhs, el * sizeof(T));
+            hipLaunchKernelGGL(spk::fwd_inv_kernel, dim3(lp.ifw[k].second), dim3(kWG), 0, st,
+                               h->ds, h->d_sdesc.p, h->d_pdesc.p,
+                               h->d_iitems.p + lp.ifw[k].first, h->d_inv32.p, lnz, rhs, ldr,
+                               nrhs, vecp);
+            hipck(hipGetLastError());
+          }
+          continue;
+        }
       }
       for (const LevelPlan::SolveStep& S : lp.fsteps) {
         {
@@ -1188,7 +1263,29 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
                            ldr, nrhs, vecp, lp.items_cnt);
         hipck(hipGetLastError());
       }
+      bool done = false;
+      if constexpr (std::is_same<T, float>::value) {
+        if (inv && lp.large_cnt > 0) {
+          for (size_t k = 0; k < lp.ibw.size(); ++k) {
+            if (lp.ibw[k].second == 0) continue;
+            const double el = lp.bsteps[k].elems;
+            Prof pr(h, SPKHIP_K_BWD_GEMV, 2.0 * el * nrhs, el * sizeof(T));
+            hipLaunchKernelGGL(spk::bwd_inv_kernel, dim3(lp.ibw[k].second), dim3(kWG), 0, st,
+                               h->ds, h->d_sdesc.p, h->d_pdesc.p,
+                               h->d_iitems.p + lp.ibw[k].first, h->d_inv32.p, unz, rhs, ldr,
+                               nrhs, vecp);
+            hipck(hipGetLastError());
+          }
+          Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);
+          hipLaunchKernelGGL(spk::bwd_inv_kernel, dim3(lp.ifin.second), dim3(kWG), 0, st, h->ds,
+                             h->d_sdesc.p, h->d_pdesc.p, h->d_iitems.p + lp.ifin.first,
+                             h->d_inv32.p, unz, rhs, ldr, nrhs, vecp);
+          hipck(hipGetLastError());
+          done = true;
+        }
+      }
       for (const LevelPlan::SolveStep& S : lp.bsteps) {
+        if (done) break;
         if (S.gemv_cnt > 0) {
           Prof pr(h, SPKHIP_K_BWD_GEMV, 2.0 * S.elems * nrhs, S.elems * sizeof(T));
           hipLaunchKernelGGL(spk::bwd_bgemv_kernel<T>, dim3(S.gemv_cnt), dim3(kWG), 0, st, h->ds,
@@ -1246,7 +1343,8 @@ void solve_levels(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bw
   const std::vector<const void*> key = {
       (const void*)rhs, (const void*)(intptr_t)ldr, (const void*)(intptr_t)nrhs,
       (const void*)(intptr_t)sizeof(T), (const void*)(intptr_t)h->sym_gen, vec.p,
-      Bufs<T>::prep(h).p, h->d_prepi.p, Bufs<T>::lnz(h).p, Bufs<T>::unz(h).p, h->d_ipvt.p};
+      Bufs<T>::prep(h).p, h->d_prepi.p, Bufs<T>::lnz(h).p, Bufs<T>::unz(h).p, h->d_ipvt.p,
+      (const void*)(intptr_t)inv_solves(), h->d_inv32.p};
   if (!h->g_exec || h->g_key != key) {
     if (h->g_seen != key) {
       h->g_seen = key;
@@ -1414,6 +1512,7 @@ void replan(spkhip_handle* h) {
   h->d_wb_col.upload(h->wb_col, st);
   h->d_sdesc.upload(sdesc, st);
   if (!h->pdesc.empty()) h->d_pdesc.upload(h->pdesc, st);
+  if (!h->iitems.empty()) h->d_iitems.upload(h->iitems, st);
   DevSym& d = h->ds;
   d.foff = h->d_foff.p;
   d.fcb = h->d_fcb.p;
@@ -1423,6 +1522,7 @@ void replan(spkhip_handle* h) {
   h->sym_gen += 1;
   h->factored = false;
   h->prepped = false;
+  h->inv_prepped = false;
 }
 
 void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i64* xsuper,
@@ -1665,6 +1765,7 @@ int64_t lufactor_stateless(i64 n, i64 nsuper, const i64* xsuper, const i64* snod
     h->precision = prec;
     h->factored = true;
     h->prepped = false;
+    h->inv_prepped = false;
     download_impl<T>(h, lnz, kind == SPKHIP_KIND_LU ? unz : nullptr, ipvt);
     return h->first_fail ? -1 : 0;
   });
@@ -1929,6 +2030,7 @@ int64_t spkhip_factor(spkhip_t h, int64_t precision) {
     h->mode = (int)precision;
     h->factored = true;
     h->prepped = false;
+    h->inv_prepped = false;
     return h->first_fail ? -1 : 0;
   });
 }

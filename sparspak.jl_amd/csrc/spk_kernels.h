@@ -101,6 +101,16 @@ struct SolveDesc {
   int64_t fc0;     // the front's first column
   int64_t fvec;    // the front's offset in the solve vector arena
   int64_t ucol;    // backward GEMV: s.ucol of its row (unz offset)
+  int64_t inv;     // solve data (chain descriptors): element offset of the
+                   // block's two explicit inverses (float factors)
+};
+
+// One workgroup of an explicit-inverse block step (fwd_inv_kernel,
+// bwd_inv_kernel): the GEMV descriptor it applies (tile < 0: none) and the
+// solve-data descriptor of the block whose new entries it forms from that
+// block's inverse (blk < 0: none); write: it stores those entries.
+struct InvItem {
+  int32_t tile, blk, write, pad;
 };
 
 }  // namespace spk

@@ -2621,6 +2621,265 @@ bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
 }
 
 // ---------------------------------------------------------------------------
+// Float solves with explicit inverses of the diagonal blocks. The chains are
+// the solves' serial critical path (one column group after another); with
+// ML = L11^-1 P and MU = U11^-1 formed once per factorisation, every block
+// step becomes GEMV work that all workgroups of the step do in parallel, one
+// launch per step instead of a chain launch and a GEMV launch. (The double
+// factor keeps substitution, the reference's operation order.)
+// ---------------------------------------------------------------------------
+
+// ML and MU of one block (one workgroup per block), each bwp x bwp
+// row-major and zero outside bw x bw, from the block's panels: thread j
+// forms column j by substitution on e_j in LDS (X[r][j]), the lower inverse
+// first (its columns then permuted by the block's interchanges), then the
+// upper one.
+constexpr int kInvPrepLds = (128 * 128 + kPanelElems) * 4;  // float
+__global__ void __launch_bounds__(kWG)
+solve_inv_prep_kernel(const SolveDesc* __restrict__ desc, const float* __restrict__ prep,
+                      const int32_t* __restrict__ prepi, float* __restrict__ inv) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_inv[];
+  float* X = reinterpret_cast<float*>(smem_inv);  // X[r * 128 + j]
+  float* P = X + 128 * 128;                       // the panels of one factor
+  __shared__ int iperm[128];
+  const SolveDesc d = desc[blockIdx.x];
+  const int bw = d.c1 - d.c0, ng = (bw + 3) / 4, bwp = 4 * ng;
+  const int tid = threadIdx.x;
+  const int npan = 8 * ng * (ng + 1);
+  const float* lo = prep + 4 * (int64_t)d.aux;
+  const float* up = lo + npan;
+  const float* rinv = up + npan;
+  const int32_t* pi = prepi + (int64_t)d.r0 * kPrepInts;
+  float* ML = inv + d.inv;
+  float* MU = ML + (int64_t)bwp * bwp;
+  for (int i = tid; i < bw; i += kWG) iperm[pi[i]] = i;
+  for (int i = tid; i < npan; i += kWG) P[i] = lo[i];
+  for (int i = tid; i < 128 * 128; i += kWG) X[i] = 0.0f;
+  __syncthreads();
+  if (tid < bw) {
+    const int j = tid;
+    X[j * 128 + j] = 1.0f;
+    for (int c = j; c < bw; ++c) {
+      const float xc = X[c * 128 + j];
+      const float* Pc = P + lo_panel(c / 4, ng) + (c % 4) - 4 * 4 * (c / 4);
+      for (int r = c + 1; r < bw; ++r) X[r * 128 + j] -= Pc[4 * r] * xc;
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < bwp * bwp; idx += kWG) {
+    const int i = idx / bwp, jj = idx % bwp;
+    ML[idx] = (i < bw && jj < bw) ? X[i * 128 + iperm[jj]] : 0.0f;
+  }
+  __syncthreads();
+  for (int i = tid; i < npan; i += kWG) P[i] = up[i];
+  for (int i = tid; i < 128 * 128; i += kWG) X[i] = 0.0f;
+  __syncthreads();
+  if (tid < bw) {
+    const int j = tid;
+    X[j * 128 + j] = 1.0f;
+    for (int c = j; c >= 0; --c) {
+      const float xc = X[c * 128 + j] * rinv[c];
+      X[c * 128 + j] = xc;
+      const float* Pc = P + up_panel(c / 4) + (c % 4);
+      for (int r = 0; r < c; ++r) X[r * 128 + j] -= Pc[4 * r] * xc;
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < bwp * bwp; idx += kWG) {
+    const int i = idx / bwp, jj = idx % bwp;
+    MU[idx] = (i < bw && jj < bw) ? X[i * 128 + jj] : 0.0f;
+  }
+}
+
+// Row i of a block's inverse, columns 64h .. 64h+63, as 16 four-vectors
+// (zero outside the block).
+__device__ __attribute__((always_inline)) inline void load_inv_row(
+    const float* __restrict__ M, int bw, int bwp, int i, int h, f32x4 (&mv)[16]) {
+  const f32x4* R = reinterpret_cast<const f32x4*>(M + (int64_t)i * bwp);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int j = 64 * h + 4 * k;
+    mv[k] = (i < bw && j < bwp) ? R[j / 4] : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  }
+}
+
+// v = M yb for a block of up to 128 entries: thread (i = tid % 128, h = tid /
+// 128) sums its 64 columns in order, the two halves are added in order; yb
+// (128 entries, zero past the block) in LDS, v to LDS.
+__device__ __attribute__((always_inline)) inline void inv_apply(const f32x4 (&mv)[16],
+                                                                const float* yb, float (*zp)[128],
+                                                                float* v) {
+  const int tid = threadIdx.x, i = tid & 127, h = tid >> 7;
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int j = 64 * h + 4 * k;
+    acc += mv[k].x * yb[j];
+    acc += mv[k].y * yb[j + 1];
+    acc += mv[k].z * yb[j + 2];
+    acc += mv[k].w * yb[j + 3];
+  }
+  zp[h][i] = acc;
+  __syncthreads();
+  if (tid < 128) v[tid] = zp[0][tid] + zp[1][tid];
+  __syncthreads();
+}
+
+// Forward block step: every workgroup forms z = ML y[c0:c1] of its block;
+// the one marked write stores z as the block's solution entries (rhs only:
+// the front's vector keeps y, which the step's other workgroups read); a
+// tile workgroup then updates its 64 rows, y[c1 + r0 + lane] -= L[., c0:c1]
+// z, with fwd_bgemv_body's operations.
+__global__ void __launch_bounds__(kWG)
+fwd_inv_kernel(DevSym s, const SolveDesc* __restrict__ sdesc, const SolveDesc* __restrict__ pdesc,
+               const InvItem* __restrict__ items, const float* __restrict__ inv,
+               const float* __restrict__ lnz, float* __restrict__ rhs, int64_t ldr, int nrhs,
+               float* __restrict__ vec) {
+  __shared__ float yb[128];
+  __shared__ float zp[2][128];
+  __shared__ float zs[128];
+  __shared__ float red[4][64];
+  __shared__ int64_t lc[128];
+  const int tid = threadIdx.x, lane = tid % 64;
+  const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+  const InvItem it = items[blockIdx.x];
+  const SolveDesc b = pdesc[it.blk];
+  const int L = b.L, c0 = b.c0, bw = b.c1 - b.c0, bwp = 4 * ((bw + 3) / 4);
+  f32x4 mv[16];
+  load_inv_row(inv + b.inv, bw, bwp, tid & 127, tid >> 7, mv);
+  const bool tile = it.tile >= 0;
+  const int r0 = tile ? sdesc[it.tile].r0 : 0;
+  const int r = b.c1 + r0 + lane;
+  const int cb = bw * wave / 4, ce = bw * (wave + 1) / 4;
+  if (tile && tid < bw) lc[tid] = s.lcol[b.fc0 + c0 + tid];
+  __syncthreads();
+  float lv[32];
+#pragma unroll
+  for (int j = 0; j < 32; ++j)
+    lv[j] = (tile && r < L && cb + j < ce) ? lnz[lc[cb + j] + r] : 0.0f;
+  for (int q = 0; q < nrhs; ++q) {
+    float* yg = vec + b.fvec * nrhs + (int64_t)q * L;
+    if (tid < 128) yb[tid] = (tid < bw) ? yg[c0 + tid] : 0.0f;
+    const float yr = (tile && wave == 0 && r < L) ? yg[r] : 0.0f;
+    __syncthreads();
+    inv_apply(mv, yb, zp, zs);
+    if (it.write && tid < bw) rhs[(int64_t)q * ldr + b.fc0 + c0 + tid] = zs[tid];
+    float acc = 0.0f;
+    if (tile && r < L) {
+#pragma unroll
+      for (int j = 0; j < 32; ++j)
+        if (cb + j < ce) acc += lv[j] * zs[cb + j];
+    }
+    red[wave][lane] = acc;
+    __syncthreads();
+    if (tile && wave == 0 && r < L)
+      yg[r] = yr - (((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]);
+    __syncthreads();
+  }
+}
+
+// Backward block step. Before it, the right-hand side holds y = z - t for
+// the rows whose t is done. A row workgroup of block b first forms x of the
+// next block, x = MU y[c0':c1'] (blk), then its row's t = U[c, c1:L] x[c1:L]
+// (x of the next block from LDS, of later blocks from the front's vector,
+// the update rows as gathered) with bwd_bgemv_body's operations, and leaves
+// y_c = z_c - t_c in the right-hand side. The workgroup marked write stores
+// x of the next block in the front's vector (read by later steps only). A
+// finishing workgroup (pad != 0, one per front) forms x of the front's first
+// block into the right-hand side and copies the other blocks' x there.
+__global__ void __launch_bounds__(kWG)
+bwd_inv_kernel(DevSym s, const SolveDesc* __restrict__ sdesc, const SolveDesc* __restrict__ pdesc,
+               const InvItem* __restrict__ items, const float* __restrict__ inv,
+               const float* __restrict__ unz, float* __restrict__ rhs, int64_t ldr, int nrhs,
+               float* __restrict__ vec) {
+  constexpr int QB = 4;   // right-hand sides sharing one read of the row
+  constexpr int CH = 16;  // columns per thread and pass
+  __shared__ float yb[128];
+  __shared__ float zp[2][128];
+  __shared__ float xs[QB][128];
+  __shared__ float red[QB][4];
+  const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
+  const InvItem it = items[blockIdx.x];
+  const bool hasx = it.blk >= 0, row = it.tile >= 0, fin = it.pad != 0;
+  const SolveDesc d = row ? sdesc[it.tile] : pdesc[it.blk];
+  const SolveDesc b = hasx ? pdesc[it.blk] : d;
+  const int L = d.L;
+  const int bbw = b.c1 - b.c0, bwp = 4 * ((bbw + 3) / 4);
+  f32x4 mv[16];
+  if (hasx) load_inv_row(inv + b.inv + (int64_t)bwp * bwp, bbw, bwp, tid & 127, tid >> 7, mv);
+  const int xlo = hasx ? b.c0 : 0, xhi = hasx ? b.c1 : 0;  // x columns held in LDS
+  for (int q0 = 0; q0 < nrhs; q0 += QB) {
+    const int nq = min(QB, nrhs - q0);
+    if (hasx) {
+      for (int qq = 0; qq < nq; ++qq) {
+        const float* y = rhs + (int64_t)(q0 + qq) * ldr + b.fc0 + b.c0;
+        if (tid < 128) yb[tid] = (tid < bbw) ? y[tid] : 0.0f;
+        __syncthreads();
+        inv_apply(mv, yb, zp, xs[qq]);
+      }
+      if (it.write || fin) {
+        for (int qq = 0; qq < nq; ++qq)
+          if (tid < bbw) {
+            if (fin)
+              rhs[(int64_t)(q0 + qq) * ldr + b.fc0 + b.c0 + tid] = xs[qq][tid];
+            else
+              vec[b.fvec * nrhs + (int64_t)(q0 + qq) * L + b.c0 + tid] = xs[qq][tid];
+          }
+      }
+      if (fin) {
+        for (int qq = 0; qq < nq; ++qq)
+          for (int j = b.c1 + tid; j < b.W; j += kWG)
+            rhs[(int64_t)(q0 + qq) * ldr + b.fc0 + j] =
+                vec[b.fvec * nrhs + (int64_t)(q0 + qq) * L + j];
+      }
+    }
+    if (row) {
+      const int c = d.c0 + d.r0;
+      const float* u = unz + d.ucol;
+      float acc[QB];
+#pragma unroll
+      for (int qq = 0; qq < QB; ++qq) acc[qq] = 0.0f;
+      for (int j0 = d.c1 + tid; j0 < L; j0 += CH * kWG) {
+        float uv[CH];
+#pragma unroll
+        for (int t = 0; t < CH; ++t) {
+          const int j = j0 + kWG * t;
+          uv[t] = (j < L) ? u[j] : 0.0f;
+        }
+#pragma unroll
+        for (int qq = 0; qq < QB; ++qq) {
+          if (qq < nq) {
+            const float* yg = vec + d.fvec * nrhs + (int64_t)(q0 + qq) * L;
+            float xv[CH];
+#pragma unroll
+            for (int t = 0; t < CH; ++t) {
+              const int j = j0 + kWG * t;
+              xv[t] = (j >= L) ? 0.0f : (j < xhi && j >= xlo) ? xs[qq][j - xlo] : yg[j];
+            }
+#pragma unroll
+            for (int t = 0; t < CH; ++t) acc[qq] += uv[t] * xv[t];
+          }
+        }
+      }
+#pragma unroll
+      for (int qq = 0; qq < QB; ++qq) {
+        if (qq < nq) {
+          float a = acc[qq];
+          for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+          if (lane == 0) red[qq][wave] = a;
+        }
+      }
+      __syncthreads();
+      if (tid < nq) {
+        float* y = rhs + (int64_t)(q0 + tid) * ldr + d.fc0 + c;
+        *y = *y - (((red[tid][0] + red[tid][1]) + red[tid][2]) + red[tid][3]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Permutation, residual and refinement helpers (double unless noted).
 // pb[i] = b[perm[i]-1] (rhs = b[rperm], SpkSparseBase.jl:407) and
 // x[j] = pb[invp[j]-1] (solution = rhs[rinvp], :413).
