@@ -742,10 +742,14 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         if (gi < sblocks[fi].size()) {
           const Blk& B = sblocks[fi][gi];
           const i32 t = h->fL[large[fi]] - B.c1;
-          const i32 blk = bdata[fi][gi].r0;
-          if (t <= 0) h->iitems.push_back(spk::InvItem{-1, blk, 1, 0});
+          const spk::SolveDesc& Db = bdata[fi][gi];
+          auto fitem = [&](i32 r0, i32 write) {
+            return spk::InvItem{Db.fc0, Db.fvec, Db.inv, 0, Db.L, Db.W, B.c0, B.c1, r0, -1, 0,
+                                write};
+          };
+          if (t <= 0) h->iitems.push_back(fitem(-1, 1));
           for (i32 r0 = 0; r0 < t; r0 += 64) {
-            h->iitems.push_back(spk::InvItem{(i32)sdesc.size(), blk, r0 == 0, 0});
+            h->iitems.push_back(fitem(r0, r0 == 0 ? 1 : 0));
             sdesc.push_back(sd(large[fi], B, r0, 0));
           }
           S.elems += (double)(B.c1 - B.c0) * (h->fL[large[fi]] - B.c0);
@@ -763,15 +767,18 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         if (bi < 0) continue;
         const Blk& B = sblocks[fi][bi];
         if (B.c1 >= h->fL[large[fi]]) continue;
-        const i32 nxt = bi + 1 < (i64)sblocks[fi].size() ? bdata[fi][bi + 1].r0 : -1;
+        const bool nxt = bi + 1 < (i64)sblocks[fi].size();
+        const spk::SolveDesc& Dn = bdata[fi][nxt ? bi + 1 : bi];
         for (i32 r0 = 0; r0 < B.c1 - B.c0; ++r0) {
-          h->iitems.push_back(spk::InvItem{(i32)sdesc.size(), nxt, r0 == 0 && nxt >= 0, 0});
           spk::SolveDesc D = sd(large[fi], B, r0, 0);
           // unz offset of the row in front coordinates (as DevSym::ucol)
           const i64 g = h->fc0[large[fi]] + B.c0 + r0;
           const i64 pp = h->snode[g] - 1;
           const i64 pc0 = h->xsuper[pp] - 1, nq = h->xsuper[pp + 1] - h->xsuper[pp];
           D.ucol = h->xunz[g] - 1 - (pc0 - h->fc0[large[fi]] + nq);
+          h->iitems.push_back(spk::InvItem{D.fc0, D.fvec, nxt ? Dn.inv : 0, D.ucol, D.L, D.W,
+                                           nxt ? Dn.c0 : 0, nxt ? Dn.c1 : 0, 0, B.c0 + r0, B.c1,
+                                           (r0 == 0 && nxt) ? 1 : 0});
           sdesc.push_back(D);
         }
       }
@@ -789,8 +796,11 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
       lp.bsteps.push_back(S);
     }
     lp.ifin.first = (int)h->iitems.size();
-    for (size_t fi = 0; fi < large.size(); ++fi)
-      h->iitems.push_back(spk::InvItem{-1, bdata[fi][0].r0, 0, 1});
+    for (size_t fi = 0; fi < large.size(); ++fi) {
+      const spk::SolveDesc& D0 = bdata[fi][0];
+      h->iitems.push_back(
+          spk::InvItem{D0.fc0, D0.fvec, D0.inv, 0, D0.L, D0.W, D0.c0, D0.c1, 0, -1, 0, 2});
+    }
     lp.ifin.second = (int)large.size();
   }
 }
@@ -1225,9 +1235,8 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
             const double el = lp.fsteps[k].elems;
             Prof pr(h, SPKHIP_K_FWD_GEMV, 2.0 * el * nrhs, el * sizeof(T));
             hipLaunchKernelGGL(spk::fwd_inv_kernel, dim3(lp.ifw[k].second), dim3(kWG), 0, st,
-                               h->ds, h->d_sdesc.p, h->d_pdesc.p,
-                               h->d_iitems.p + lp.ifw[k].first, h->d_inv32.p, lnz, rhs, ldr,
-                               nrhs, vecp);
+                               h->ds, h->d_iitems.p + lp.ifw[k].first, h->d_inv32.p, lnz, rhs,
+                               ldr, nrhs, vecp);
             hipck(hipGetLastError());
           }
           continue;
@@ -1271,15 +1280,14 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
             const double el = lp.bsteps[k].elems;
             Prof pr(h, SPKHIP_K_BWD_GEMV, 2.0 * el * nrhs, el * sizeof(T));
             hipLaunchKernelGGL(spk::bwd_inv_kernel, dim3(lp.ibw[k].second), dim3(kWG), 0, st,
-                               h->ds, h->d_sdesc.p, h->d_pdesc.p,
-                               h->d_iitems.p + lp.ibw[k].first, h->d_inv32.p, unz, rhs, ldr,
-                               nrhs, vecp);
+                               h->ds, h->d_iitems.p + lp.ibw[k].first, h->d_inv32.p, unz, rhs,
+                               ldr, nrhs, vecp);
             hipck(hipGetLastError());
           }
           Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);
           hipLaunchKernelGGL(spk::bwd_inv_kernel, dim3(lp.ifin.second), dim3(kWG), 0, st, h->ds,
-                             h->d_sdesc.p, h->d_pdesc.p, h->d_iitems.p + lp.ifin.first,
-                             h->d_inv32.p, unz, rhs, ldr, nrhs, vecp);
+                             h->d_iitems.p + lp.ifin.first, h->d_inv32.p, unz, rhs, ldr, nrhs,
+                             vecp);
           hipck(hipGetLastError());
           done = true;
         }

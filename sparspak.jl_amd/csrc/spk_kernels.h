@@ -106,11 +106,17 @@ struct SolveDesc {
 };
 
 // One workgroup of an explicit-inverse block step (fwd_inv_kernel,
-// bwd_inv_kernel): the GEMV descriptor it applies (tile < 0: none) and the
-// solve-data descriptor of the block whose new entries it forms from that
-// block's inverse (blk < 0: none); write: it stores those entries.
+// bwd_inv_kernel), with every field it needs (no dependent descriptor load).
+// The block whose new entries it forms from that block's inverse: columns
+// [x0, x1) (x1 == x0: none), inverse at element inv. Forward: the tile's
+// first row below the block, c1 + r0 (r0 < 0: no tile). Backward: the row
+// c of its GEMV (c < 0: none), its U entries from ucol, summed from column
+// rc1. write: it stores the entries it formed (forward: the right-hand
+// side; backward: the front's vector); fin: backward, it stores them in the
+// right-hand side and copies the front's other columns [x1, W) there.
 struct InvItem {
-  int32_t tile, blk, write, pad;
+  int64_t fc0, fvec, inv, ucol;
+  int32_t L, W, x0, x1, r0, c, rc1, flags;  // flags: 1 write, 2 fin
 };
 
 }  // namespace spk

@@ -2725,14 +2725,13 @@ __device__ __attribute__((always_inline)) inline void inv_apply(const f32x4 (&mv
   __syncthreads();
 }
 
-// Forward block step: every workgroup forms z = ML y[c0:c1] of its block;
+// Forward block step: every workgroup forms z = ML y[x0:x1] of its block;
 // the one marked write stores z as the block's solution entries (rhs only:
 // the front's vector keeps y, which the step's other workgroups read); a
-// tile workgroup then updates its 64 rows, y[c1 + r0 + lane] -= L[., c0:c1]
+// tile workgroup then updates its 64 rows, y[x1 + r0 + lane] -= L[., x0:x1]
 // z, with fwd_bgemv_body's operations.
 __global__ void __launch_bounds__(kWG)
-fwd_inv_kernel(DevSym s, const SolveDesc* __restrict__ sdesc, const SolveDesc* __restrict__ pdesc,
-               const InvItem* __restrict__ items, const float* __restrict__ inv,
+fwd_inv_kernel(DevSym s, const InvItem* __restrict__ items, const float* __restrict__ inv,
                const float* __restrict__ lnz, float* __restrict__ rhs, int64_t ldr, int nrhs,
                float* __restrict__ vec) {
   __shared__ float yb[128];
@@ -2743,27 +2742,31 @@ fwd_inv_kernel(DevSym s, const SolveDesc* __restrict__ sdesc, const SolveDesc* _
   const int tid = threadIdx.x, lane = tid % 64;
   const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
   const InvItem it = items[blockIdx.x];
-  const SolveDesc b = pdesc[it.blk];
-  const int L = b.L, c0 = b.c0, bw = b.c1 - b.c0, bwp = 4 * ((bw + 3) / 4);
+  const int L = it.L, c0 = it.x0, bw = it.x1 - it.x0, bwp = 4 * ((bw + 3) / 4);
   f32x4 mv[16];
-  load_inv_row(inv + b.inv, bw, bwp, tid & 127, tid >> 7, mv);
-  const bool tile = it.tile >= 0;
-  const int r0 = tile ? sdesc[it.tile].r0 : 0;
-  const int r = b.c1 + r0 + lane;
+  load_inv_row(inv + it.inv, bw, bwp, tid & 127, tid >> 7, mv);
+  const bool tile = it.r0 >= 0;
+  const int r = it.x1 + it.r0 + lane;
   const int cb = bw * wave / 4, ce = bw * (wave + 1) / 4;
-  if (tile && tid < bw) lc[tid] = s.lcol[b.fc0 + c0 + tid];
+  if (tile && tid < bw) lc[tid] = s.lcol[it.fc0 + c0 + tid];
+  float* const yf = vec + it.fvec * nrhs;
+  // the first right-hand side's y, in flight with the column offsets
+  if (tid < 128) yb[tid] = (tid < bw) ? yf[c0 + tid] : 0.0f;
+  float yr = (tile && wave == 0 && r < L) ? yf[r] : 0.0f;
   __syncthreads();
   float lv[32];
 #pragma unroll
   for (int j = 0; j < 32; ++j)
     lv[j] = (tile && r < L && cb + j < ce) ? lnz[lc[cb + j] + r] : 0.0f;
   for (int q = 0; q < nrhs; ++q) {
-    float* yg = vec + b.fvec * nrhs + (int64_t)q * L;
-    if (tid < 128) yb[tid] = (tid < bw) ? yg[c0 + tid] : 0.0f;
-    const float yr = (tile && wave == 0 && r < L) ? yg[r] : 0.0f;
-    __syncthreads();
+    float* yg = yf + (int64_t)q * L;
+    if (q > 0) {
+      if (tid < 128) yb[tid] = (tid < bw) ? yg[c0 + tid] : 0.0f;
+      yr = (tile && wave == 0 && r < L) ? yg[r] : 0.0f;
+      __syncthreads();
+    }
     inv_apply(mv, yb, zp, zs);
-    if (it.write && tid < bw) rhs[(int64_t)q * ldr + b.fc0 + c0 + tid] = zs[tid];
+    if ((it.flags & 1) && tid < bw) rhs[(int64_t)q * ldr + it.fc0 + c0 + tid] = zs[tid];
     float acc = 0.0f;
     if (tile && r < L) {
 #pragma unroll
@@ -2779,17 +2782,16 @@ fwd_inv_kernel(DevSym s, const SolveDesc* __restrict__ sdesc, const SolveDesc* _
 }
 
 // Backward block step. Before it, the right-hand side holds y = z - t for
-// the rows whose t is done. A row workgroup of block b first forms x of the
-// next block, x = MU y[c0':c1'] (blk), then its row's t = U[c, c1:L] x[c1:L]
-// (x of the next block from LDS, of later blocks from the front's vector,
-// the update rows as gathered) with bwd_bgemv_body's operations, and leaves
-// y_c = z_c - t_c in the right-hand side. The workgroup marked write stores
-// x of the next block in the front's vector (read by later steps only). A
-// finishing workgroup (pad != 0, one per front) forms x of the front's first
-// block into the right-hand side and copies the other blocks' x there.
+// the rows whose t is done. A row workgroup first forms x of the block
+// after its row's block, x = MU y[x0:x1], then its row's t = U[c, rc1:L]
+// x[rc1:L] (x[x0:x1] from LDS, of later blocks from the front's vector, the
+// update rows as gathered) with bwd_bgemv_body's operations, and leaves y_c
+// = z_c - t_c in the right-hand side. The workgroup marked write stores x of
+// that block in the front's vector (read by later steps only). A finishing
+// workgroup (one per front) forms x of the front's first block into the
+// right-hand side and copies the other blocks' x there.
 __global__ void __launch_bounds__(kWG)
-bwd_inv_kernel(DevSym s, const SolveDesc* __restrict__ sdesc, const SolveDesc* __restrict__ pdesc,
-               const InvItem* __restrict__ items, const float* __restrict__ inv,
+bwd_inv_kernel(DevSym s, const InvItem* __restrict__ items, const float* __restrict__ inv,
                const float* __restrict__ unz, float* __restrict__ rhs, int64_t ldr, int nrhs,
                float* __restrict__ vec) {
   constexpr int QB = 4;   // right-hand sides sharing one read of the row
@@ -2800,46 +2802,44 @@ bwd_inv_kernel(DevSym s, const SolveDesc* __restrict__ sdesc, const SolveDesc* _
   __shared__ float red[QB][4];
   const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
   const InvItem it = items[blockIdx.x];
-  const bool hasx = it.blk >= 0, row = it.tile >= 0, fin = it.pad != 0;
-  const SolveDesc d = row ? sdesc[it.tile] : pdesc[it.blk];
-  const SolveDesc b = hasx ? pdesc[it.blk] : d;
-  const int L = d.L;
-  const int bbw = b.c1 - b.c0, bwp = 4 * ((bbw + 3) / 4);
+  const bool row = it.c >= 0, fin = (it.flags & 2) != 0;
+  const int L = it.L;
+  const int xlo = it.x0, xhi = it.x1, xbw = it.x1 - it.x0;  // x columns formed here
+  const bool hasx = xbw > 0;
+  const int bwp = 4 * ((xbw + 3) / 4);
   f32x4 mv[16];
-  if (hasx) load_inv_row(inv + b.inv + (int64_t)bwp * bwp, bbw, bwp, tid & 127, tid >> 7, mv);
-  const int xlo = hasx ? b.c0 : 0, xhi = hasx ? b.c1 : 0;  // x columns held in LDS
+  if (hasx) load_inv_row(inv + it.inv + (int64_t)bwp * bwp, xbw, bwp, tid & 127, tid >> 7, mv);
+  float* const yf = vec + it.fvec * nrhs;
   for (int q0 = 0; q0 < nrhs; q0 += QB) {
     const int nq = min(QB, nrhs - q0);
     if (hasx) {
       for (int qq = 0; qq < nq; ++qq) {
-        const float* y = rhs + (int64_t)(q0 + qq) * ldr + b.fc0 + b.c0;
-        if (tid < 128) yb[tid] = (tid < bbw) ? y[tid] : 0.0f;
+        const float* y = rhs + (int64_t)(q0 + qq) * ldr + it.fc0 + xlo;
+        if (tid < 128) yb[tid] = (tid < xbw) ? y[tid] : 0.0f;
         __syncthreads();
         inv_apply(mv, yb, zp, xs[qq]);
       }
-      if (it.write || fin) {
+      if (it.flags & 3) {
         for (int qq = 0; qq < nq; ++qq)
-          if (tid < bbw) {
+          if (tid < xbw) {
             if (fin)
-              rhs[(int64_t)(q0 + qq) * ldr + b.fc0 + b.c0 + tid] = xs[qq][tid];
+              rhs[(int64_t)(q0 + qq) * ldr + it.fc0 + xlo + tid] = xs[qq][tid];
             else
-              vec[b.fvec * nrhs + (int64_t)(q0 + qq) * L + b.c0 + tid] = xs[qq][tid];
+              yf[(int64_t)(q0 + qq) * L + xlo + tid] = xs[qq][tid];
           }
       }
       if (fin) {
         for (int qq = 0; qq < nq; ++qq)
-          for (int j = b.c1 + tid; j < b.W; j += kWG)
-            rhs[(int64_t)(q0 + qq) * ldr + b.fc0 + j] =
-                vec[b.fvec * nrhs + (int64_t)(q0 + qq) * L + j];
+          for (int j = xhi + tid; j < it.W; j += kWG)
+            rhs[(int64_t)(q0 + qq) * ldr + it.fc0 + j] = yf[(int64_t)(q0 + qq) * L + j];
       }
     }
     if (row) {
-      const int c = d.c0 + d.r0;
-      const float* u = unz + d.ucol;
+      const float* u = unz + it.ucol;
       float acc[QB];
 #pragma unroll
       for (int qq = 0; qq < QB; ++qq) acc[qq] = 0.0f;
-      for (int j0 = d.c1 + tid; j0 < L; j0 += CH * kWG) {
+      for (int j0 = it.rc1 + tid; j0 < L; j0 += CH * kWG) {
         float uv[CH];
 #pragma unroll
         for (int t = 0; t < CH; ++t) {
@@ -2849,7 +2849,7 @@ bwd_inv_kernel(DevSym s, const SolveDesc* __restrict__ sdesc, const SolveDesc* _
 #pragma unroll
         for (int qq = 0; qq < QB; ++qq) {
           if (qq < nq) {
-            const float* yg = vec + d.fvec * nrhs + (int64_t)(q0 + qq) * L;
+            const float* yg = yf + (int64_t)(q0 + qq) * L;
             float xv[CH];
 #pragma unroll
             for (int t = 0; t < CH; ++t) {
@@ -2871,7 +2871,7 @@ bwd_inv_kernel(DevSym s, const SolveDesc* __restrict__ sdesc, const SolveDesc* _
       }
       __syncthreads();
       if (tid < nq) {
-        float* y = rhs + (int64_t)(q0 + tid) * ldr + d.fc0 + c;
+        float* y = rhs + (int64_t)(q0 + tid) * ldr + it.fc0 + it.c;
         *y = *y - (((red[tid][0] + red[tid][1]) + red[tid][2]) + red[tid][3]);
       }
     }
