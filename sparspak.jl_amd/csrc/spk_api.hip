@@ -97,6 +97,9 @@ struct LevelPlan {
     // (which must wait for the previous gemm2)
     Launch gemm2;
     bool w1 = false;
+    // the previous step's update within its block group, launched with this
+    // step's diagonal regions (blk_diag_border_kernel)
+    Launch bin;
   };
   double small_flops = 0, small_elems = 0, init_elems = 0, wb_elems = 0, solve_elems = 0;
   int max_w = 0;                    // widest front of the level
@@ -249,6 +252,9 @@ struct spkhip_handle {
   size_t upd_group = 8;                // blocks per full trailing update
   int small_lmax = kSmallLmax;         // fronts with L <= this are factored whole in LDS
   int block_cols = 64;                 // widest block of pieces in a large LU front
+  // the plan runs LU diagonal regions in registers (blk_diag_slab_kernel)
+  // and launches each in-group update with the next block's regions
+  bool slab_plan = false, merge_plan = false;
   int small_wave_min = 16384;          // small fronts per level from which one wave takes one
   double tuned_ms = 0;                 // best factor time of the last autotune
   int lowp = 0;  // large-front updates: 1 bf16, 2 scaled fp16, 3 split-bf16 operands
@@ -482,6 +488,16 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
                 std::vector<spk::SolveDesc>& sdesc, std::vector<i32>& ut_front,
                 std::vector<i32>& ut_piece, std::vector<i32>& ut_r0) {
   const i32 nf = h->nfront;
+  {
+    // LU blocks of at most 64 columns: the diagonal region in registers
+    // (SPKHIP_DIAG_SLAB=0 keeps the LDS form), and each in-group update
+    // beside the next block's regions (SPKHIP_MERGE_BORDER=0: after them)
+    const char* ds = std::getenv("SPKHIP_DIAG_SLAB");
+    const char* mb = std::getenv("SPKHIP_MERGE_BORDER");
+    h->slab_plan =
+        h->kind == SPKHIP_KIND_LU && h->block_cols <= 64 && !(ds && ds[0] == '0');
+    h->merge_plan = h->slab_plan && !(mb && mb[0] == '0');
+  }
   h->pdesc.clear();
   h->iitems.clear();
   h->wb_front.clear();
@@ -624,6 +640,11 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
       maxg = std::max(maxg, fblocks.back().size());
       maxgs = std::max(maxgs, sblocks.back().size());
     }
+    // in-group updates of the previous step (launched with this step's
+    // diagonal regions) and, per front, the previous block to fold in
+    std::vector<spk::GemmDesc> carry;
+    LevelPlan::Launch carryL;
+    std::vector<std::pair<i32, i32>> fold(large.size(), {0, 0}), fold_next(large.size(), {0, 0});
     for (size_t gi = 0; gi < maxg; ++gi) {
       LevelPlan::Group G;
       G.diag.off = (int)bdesc.size();
@@ -631,7 +652,8 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         if (gi >= fblocks[fi].size()) continue;
         const Blk& B = fblocks[fi][gi];
         const i32 f = large[fi];
-        bdesc.push_back(BlkDesc{f, B.p0, B.np, B.c0, B.c1, G.diag.wg, 0, 0});
+        bdesc.push_back(
+            BlkDesc{f, B.p0, B.np, B.c0, B.c1, G.diag.wg, 0, 0, fold[fi].first, fold[fi].second});
         G.diag.wg += 1;
         const i32 bw = B.c1 - B.c0;
         G.diag.lds = std::max(G.diag.lds, bw * spk::kRLD);
@@ -643,6 +665,13 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         G.diag.elems += 2.0 * bw * bw;
       }
       G.diag.cnt = (int)bdesc.size() - G.diag.off;
+      G.bin = carryL;
+      G.bin.off = (int)gdesc.size();
+      G.bin.cnt = (int)carry.size();
+      for (const spk::GemmDesc& D : carry) gdesc.push_back(D);
+      carry.clear();
+      carryL = LevelPlan::Launch{};
+      std::fill(fold_next.begin(), fold_next.end(), std::pair<i32, i32>{0, 0});
       G.offd.off = (int)bdesc.size();
       for (size_t fi = 0; fi < large.size(); ++fi) {
         if (gi >= fblocks[fi].size()) continue;
@@ -667,11 +696,13 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
       // both blocks; an unpaired last block -> the rest with itself
       G.gemm.off = (int)gdesc.size();
       std::vector<spk::GemmDesc> g2desc;
-      auto rect = [&](i32 f, i32 k0, i32 k1, i32 r0, i32 r1, i32 j0, i32 j1, bool second) {
+      // dest 0: this step's launch on the main stream, 1: the second stream,
+      // 2: with the next step's diagonal regions
+      auto rect = [&](i32 f, i32 k0, i32 k1, i32 r0, i32 r1, i32 j0, i32 j1, int dest) {
         if (r1 <= r0 || j1 <= j0 || k1 <= k0) return;
-        LevelPlan::Launch& Lc = second ? G.gemm2 : G.gemm;
+        LevelPlan::Launch& Lc = dest == 1 ? G.gemm2 : dest == 2 ? carryL : G.gemm;
         const i32 ntr = (r1 - r0 + 127) / 128, ntc = (j1 - j0 + 127) / 128;
-        (second ? g2desc : gdesc)
+        (dest == 1 ? g2desc : dest == 2 ? carry : gdesc)
             .push_back(spk::GemmDesc{f, k0, k1, r0, r1 - r0, j0, j1 - j0, Lc.wg, ntr,
                                      h->fW[f] > 256 ? 1 : 0});
         Lc.wg += ntr * ntc;
@@ -689,8 +720,18 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         if (gi + 1 < ge) {
           // border of the group's remaining blocks [B.c1, E) x ..., E = their end
           const i32 E = fblocks[fi][ge - 1].c1;
-          rect(f, B.c0, B.c1, B.c1, L, B.c1, E, false);
-          rect(f, B.c0, B.c1, B.c1, E, E, L, false);
+          if (h->merge_plan) {
+            // all of it but the next block's diagonal region, which the
+            // next step's diagonal kernel takes itself, beside the rest
+            const i32 cn = fblocks[fi][gi + 1].c1;
+            rect(f, B.c0, B.c1, cn, L, B.c1, E, 2);
+            rect(f, B.c0, B.c1, B.c1, cn, cn, E, 2);
+            rect(f, B.c0, B.c1, B.c1, E, E, L, 2);
+            fold_next[fi] = {B.c0, B.c1};
+          } else {
+            rect(f, B.c0, B.c1, B.c1, L, B.c1, E, 0);
+            rect(f, B.c0, B.c1, B.c1, E, E, L, 0);
+          }
         } else {
           // last block of the group: the rest with the whole group, split at
           // E2 = the end of the next group: the next group's rows and
@@ -699,9 +740,9 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
           const i32 E = B.c1, k0 = fblocks[fi][gs].c0;
           const size_t nb = fblocks[fi].size();
           const i32 E2 = ge < nb ? fblocks[fi][std::min(ge + h->upd_group, nb) - 1].c1 : E;
-          rect(f, k0, E, E, L, E, E2, false);
-          rect(f, k0, E, E, E2, E2, L, false);
-          rect(f, k0, E, E2, L, E2, L, true);
+          rect(f, k0, E, E, L, E, E2, 0);
+          rect(f, k0, E, E, E2, E2, L, 0);
+          rect(f, k0, E, E2, L, E2, L, 1);
           if (E2 > E) G.w1 = true;
         }
       }
@@ -710,7 +751,9 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
       for (spk::GemmDesc& D : g2desc) gdesc.push_back(D);
       G.gemm2.cnt = (int)g2desc.size();
       lp.groups.push_back(G);
+      fold.swap(fold_next);
     }
+    if (!carry.empty()) throw ApiError{-4};  // a last step never updates in its group
     // blocked solve steps over the same blocks; each block's solve data
     // (index r0, offset 4 aux) is built once per factorisation
     // (the front's sizes and offsets ride in every descriptor, so that the
@@ -981,10 +1024,7 @@ void run_factor(spkhip_handle* h) {
   const bool two = !h->prof;
   hipStream_t st2 = two ? h->stream2 : st;
   h->xev_used = 0;
-  // LU blocks of at most 64 columns: the diagonal region in registers
-  // (SPKHIP_DIAG_SLAB=0 keeps the LDS form)
-  const char* ds_env = std::getenv("SPKHIP_DIAG_SLAB");
-  const bool slab = lu && h->block_cols <= 64 && !(ds_env && ds_env[0] == '0');
+  const bool slab = h->slab_plan;
   for (const LevelPlan& lp : h->levels) {
     h->cur_level = (int)(&lp - h->levels.data());
     h->cur_grid = lp.small_cnt;
@@ -1033,7 +1073,16 @@ void run_factor(spkhip_handle* h) {
       }
       bool pending = false;  // second-stream work not yet joined
       for (const LevelPlan::Group& G : lp.groups) {
-        {
+        if (G.bin.cnt > 0) {
+          h->cur_grid = G.diag.wg + G.bin.wg;
+          Prof pr(h, SPKHIP_K_DIAG, G.diag.flops + G.bin.flops,
+                  (G.diag.elems + G.bin.elems) * sizeof(T));
+          hipLaunchKernelGGL((spk::blk_diag_border_kernel<T>), dim3(G.diag.wg + G.bin.wg),
+                             dim3(kWG), 0, st, h->ds, h->d_bdesc.p + G.diag.off, G.diag.wg,
+                             h->d_gdesc.p + G.bin.off, G.bin.cnt, arena.p, h->d_ipvt.p,
+                             h->d_fail.p, h->lowp);
+          hipck(hipGetLastError());
+        } else {
           h->cur_grid = G.diag.wg;
           Prof pr(h, SPKHIP_K_DIAG, G.diag.flops, G.diag.elems * sizeof(T));
           if (slab)

@@ -69,6 +69,8 @@ struct BlkDesc {
   int32_t wg_begin;  // prefix sum of workgroups over the launch
   int32_t nl;        // off-diagonal kernel: workgroups taking rows below
   int32_t pad;
+  int32_t fk0, fk1;  // diagonal kernel: previous block's columns folded in
+                     // first (fk1 > fk0), its update running alongside
 };
 
 // Per-launch descriptor of the trailing-update kernel: one rectangle of one

@@ -1139,15 +1139,24 @@ blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ aren
 // final position in the columns of its piece and right of it, and at its
 // own row in the columns of earlier pieces, which the reference never
 // interchanges.
+// When d.fk1 > d.fk0 the region first takes the product of the previous
+// block (columns [fk0, fk1)) with its own columns, summed from zero over
+// those columns in order and subtracted once (the trailing update's form),
+// so that the rest of that block's update can run beside this kernel.
+// lds: kSlabLds<T> bytes (the three 64 x 64 arrays below).
 constexpr int kMaxBlkPieces = 8;  // most pieces in one block
 template <typename T>
-__global__ void __launch_bounds__(kWG)
-blk_diag_slab_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ arena,
-                     int64_t* __restrict__ ipvt, unsigned long long* __restrict__ first_fail) {
+constexpr int kSlabLds = 3 * 64 * 64 * (int)sizeof(T);
+template <typename T>
+__device__ __attribute__((always_inline)) void blk_diag_slab_body(
+    const DevSym& s, const BlkDesc& d, T* __restrict__ arena, int64_t* __restrict__ ipvt,
+    unsigned long long* __restrict__ first_fail, unsigned char* lds) {
   constexpr int SW = 16;                  // columns per wave
-  __shared__ T sl[64][64];                // step k: the multiplier of each row
-  __shared__ T sout[64][64];              // finished column c, rows in lane order
-  __shared__ T ssv[64][64];               // entries set aside while a piece's sum forms
+  // step k: the multiplier of each row; finished column c, rows in lane
+  // order; entries set aside while a piece's sum forms
+  T (*sl)[64] = reinterpret_cast<T (*)[64]>(lds);
+  T (*sout)[64] = reinterpret_cast<T (*)[64]>(lds + 64 * 64 * sizeof(T));
+  T (*ssv)[64] = reinterpret_cast<T (*)[64]>(lds + 2 * 64 * 64 * sizeof(T));
   __shared__ int smeta[64];               // step k: pivot lane | position << 8 | nonzero << 16
   __shared__ int spo[kMaxBlkPieces + 1];  // piece offsets in the region; spo[np] = bw
   __shared__ int scp[64];                 // piece of each region column
@@ -1155,7 +1164,6 @@ blk_diag_slab_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__
   __shared__ int spiv[64];                // step k: pivot position, 1-based in its piece
   __shared__ int sdone;                   // steps published
   const int lane = threadIdx.x % 64, w = wave_id();
-  const BlkDesc d = desc[blockIdx.x];
   const int f = d.front, L = s.fL[f];
   const int64_t fc0 = s.fc0[f];
   const int c0 = d.c0, bw = d.c1 - d.c0, np = d.np;
@@ -1190,6 +1198,29 @@ blk_diag_slab_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__
 #pragma unroll
   for (int i = 0; i < SW; ++i)
     a[i] = (lane < bw && cb + i < bw) ? R[lane + (int64_t)(cb + i) * L] : T(0);
+  const int fkw = d.fk1 - d.fk0;
+  if (fkw > 0) {
+    // L(c0 + r, fk0 + k) at sout[k][r], U(fk0 + k, c0 + c) at sl[k][c]
+    const T* Fb = arena + s.foff[f];
+    for (int idx = threadIdx.x; idx < 64 * fkw; idx += kWG) {
+      const int r = idx % 64, k = idx / 64;
+      sout[k][r] = (r < bw) ? Fb[c0 + r + (int64_t)(d.fk0 + k) * L] : T(0);
+      sl[k][r] = (r < bw) ? Fb[d.fk0 + k + (int64_t)(c0 + r) * L] : T(0);
+    }
+    __syncthreads();
+    T acc[SW];
+#pragma unroll
+    for (int i = 0; i < SW; ++i) acc[i] = T(0);
+    for (int k = 0; k < fkw; ++k) {
+      const T lv = sout[k][lane];
+#pragma unroll
+      for (int i = 0; i < SW; ++i) acc[i] += lv * sl[k][cb + i];
+    }
+#pragma unroll
+    for (int i = 0; i < SW; ++i)
+      if (lane < bw && cb + i < bw) a[i] = a[i] - acc[i];
+    __syncthreads();
+  }
   // first step of piece pk (current column c): the later pieces' entries
   // right of the piece are set aside and the piece's products summed from
   // zero in their place; last step: the entries take the sum off. Called
@@ -1316,6 +1347,14 @@ blk_diag_slab_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__
       if (c < bw) R[(c >= rlo ? fpos : lane) + (int64_t)c * L] = sout[c][lane];
     }
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+blk_diag_slab_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ arena,
+                     int64_t* __restrict__ ipvt, unsigned long long* __restrict__ first_fail) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[kSlabLds<T>];
+  blk_diag_slab_body<T>(s, desc[blockIdx.x], arena, ipvt, first_fail, lds);
 }
 
 // Off-diagonal parts of one block after blk_diag_kernel (see the section
@@ -1577,25 +1616,30 @@ __device__ void gemm_bf16_chunks(const float* __restrict__ F, int L, int k0, int
 // for v_mfma_f32_16x16x16f16 (operand k = 4 (l / 16) .. + 3 per lane, the
 // accumulator layout of the f32 form); refinement in double recovers the
 // accuracy.
+// double-buffered: As[buf][i * KP + k] = L(r0 + i, k0 + k),
+//                  Bs[buf][j * KP + k] = U(k0 + k, j0 + j)
+// (the split-bf16 form stages four bf16 arrays, hi and lo of A and B, in
+// the same bytes: 2 buffers x 4 x 128 rows of 20 halves)
 template <typename T>
-__global__ void __launch_bounds__(kWG, 2)
-blk_gemm_kernel(DevSym s, const GemmDesc* __restrict__ desc, int ndesc,
-                T* __restrict__ arena, int lowp) {
+constexpr int kGemmLds = (std::is_same<T, float>::value && 2 * 4 * kB16Part * 2 >
+                                                              2 * 2 * 128 * 17 * (int)sizeof(T))
+                             ? 2 * 4 * kB16Part * 2
+                             : 2 * 2 * 128 * 17 * (int)sizeof(T);
+
+// Workgroup bid of a trailing-update launch over desc[0, ndesc); lds_raw:
+// kGemmLds<T> bytes.
+template <typename T>
+__device__ __attribute__((always_inline)) void blk_gemm_body(
+    const DevSym& s, const GemmDesc* __restrict__ desc, int ndesc, T* __restrict__ arena,
+    int lowp, int bid, unsigned char* lds_raw) {
   constexpr int TM = 128, KC = 16, KP = KC + 1, PER = TM * KC / kWG;  // 8 loads each
   typedef typename Mfma16<T>::v4 v4;
-  // double-buffered: As[buf][i * KP + k] = L(r0 + i, k0 + k),
-  //                  Bs[buf][j * KP + k] = U(k0 + k, j0 + j)
-  // (the split-bf16 form stages four bf16 arrays, hi and lo of A and B, in
-  // the same bytes: 2 buffers x 4 x 128 rows of 20 halves)
-  constexpr int kLdsT = 2 * 2 * TM * KP * (int)sizeof(T);
-  constexpr int kLdsH = std::is_same<T, float>::value ? 2 * 4 * kB16Part * 2 : 0;
-  __shared__ __attribute__((aligned(16))) unsigned char lds_raw[kLdsT > kLdsH ? kLdsT : kLdsH];
   T (*As)[TM * KP] = reinterpret_cast<T (*)[TM * KP]>(lds_raw);
   T (*Bs)[TM * KP] = reinterpret_cast<T (*)[TM * KP]>(lds_raw + 2 * TM * KP * sizeof(T));
   const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
-  const int di = find_blk(desc, ndesc, blockIdx.x);
+  const int di = find_blk(desc, ndesc, bid);
   const GemmDesc d = desc[di];
-  const int b = blockIdx.x - d.wg_begin;
+  const int b = bid - d.wg_begin;
   const int L = s.fL[d.front];
   T* F = arena + s.foff[d.front];
   const int bw = d.k1 - d.k0;
@@ -1760,6 +1804,33 @@ blk_gemm_kernel(DevSym s, const GemmDesc* __restrict__ desc, int ndesc,
         }
     }
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kWG, 2)
+blk_gemm_kernel(DevSym s, const GemmDesc* __restrict__ desc, int ndesc,
+                T* __restrict__ arena, int lowp) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds_raw[kGemmLds<T>];
+  blk_gemm_body<T>(s, desc, ndesc, arena, lowp, blockIdx.x, lds_raw);
+}
+
+// The diagonal regions of one block step (workgroups b < ndiag, with the
+// previous block's product folded in) beside the rest of the previous
+// block's update within its block group (the others): the two touch
+// disjoint parts of each front, so one launch holds both and the update
+// leaves the step's critical path.
+template <typename T>
+__global__ void __launch_bounds__(kWG)
+blk_diag_border_kernel(DevSym s, const BlkDesc* __restrict__ ddesc, int ndiag,
+                       const GemmDesc* __restrict__ gdesc, int ngemm, T* __restrict__ arena,
+                       int64_t* __restrict__ ipvt, unsigned long long* __restrict__ first_fail,
+                       int lowp) {
+  constexpr int kL = kSlabLds<T> > kGemmLds<T> ? kSlabLds<T> : kGemmLds<T>;
+  __shared__ __attribute__((aligned(16))) unsigned char lds[kL];
+  if ((int)blockIdx.x < ndiag)
+    blk_diag_slab_body<T>(s, ddesc[blockIdx.x], arena, ipvt, first_fail, lds);
+  else
+    blk_gemm_body<T>(s, gdesc, ngemm, arena, lowp, (int)blockIdx.x - ndiag, lds);
 }
 
 // ---------------------------------------------------------------------------
