@@ -151,7 +151,8 @@ def main():
     kst = R["kernels"]
     dom = max(kst, key=lambda k: kst[k]["ms"])
     dk = kst[dom]
-    compute_bound = dom in ("large_update", "small_front", "large_diag", "large_offdiag")
+    compute_bound = dom in ("large_update", "small_front", "large_diag", "large_offdiag",
+                            "large_diag_border")
     prec_name = head
     # a compute kernel is MFMA-bound only when its algorithmic intensity
     # (flops per algorithmic HBM byte) times the HBM peak exceeds the matrix

@@ -320,7 +320,9 @@ int64_t spkhip_ldltsolve_f32(int64_t nsuper, const int64_t* xsuper,
 #define SPKHIP_K_OFFDIAG 9    /* large front: L rows below, U rows right */
 #define SPKHIP_K_FWD_GEMV 10  /* forward solve: rows below a block       */
 #define SPKHIP_K_BWD_GEMV 11  /* backward solve: U rows right of a block */
-#define SPKHIP_KCLASS_COUNT 12
+#define SPKHIP_K_DIAG_BORDER 12 /* diagonal regions beside the previous  */
+                                /* block's update within its group       */
+#define SPKHIP_KCLASS_COUNT 13
 typedef struct spkhip_kstat {
   int64_t launches;
   double ms, flops, bytes;

@@ -1075,7 +1075,7 @@ void run_factor(spkhip_handle* h) {
       for (const LevelPlan::Group& G : lp.groups) {
         if (G.bin.cnt > 0) {
           h->cur_grid = G.diag.wg + G.bin.wg;
-          Prof pr(h, SPKHIP_K_DIAG, G.diag.flops + G.bin.flops,
+          Prof pr(h, SPKHIP_K_DIAG_BORDER, G.diag.flops + G.bin.flops,
                   (G.diag.elems + G.bin.elems) * sizeof(T));
           hipLaunchKernelGGL((spk::blk_diag_border_kernel<T>), dim3(G.diag.wg + G.bin.wg),
                              dim3(kWG), 0, st, h->ds, h->d_bdesc.p + G.diag.off, G.diag.wg,

@@ -50,7 +50,7 @@ TUNING_KNOBS = ("small_lmax", "block_cols", "upd_group", "small_wave_min")
 
 KCLASSES = ["assemble", "small_front", "large_init", "large_diag", "large_update",
             "large_writeback", "fwd_solve", "bwd_solve", "residual", "large_offdiag",
-            "fwd_gemv", "bwd_gemv"]
+            "fwd_gemv", "bwd_gemv", "large_diag_border"]
 
 # name -> (restype, argtypes)
 _SIGS = {
