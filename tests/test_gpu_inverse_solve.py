@@ -1,10 +1,11 @@
 """Float solves with the explicit inverses of the diagonal blocks (the
-default for float factors) against the same solves by substitution
-(SPKHIP_SOLVE_INV=0, the reference's operation order, SpkLUFactor.jl:269-377):
-fronts of one block and of several, with and without rows below the blocks
-and update rows, LU and SPD, one and many right-hand sides. The two differ
-only by float rounding (the inverses are formed in float); the refined
-solutions meet the double backward-error bar."""
+default for float factors and up to three right-hand sides) against the same
+solves by substitution (SPKHIP_SOLVE_INV=0, the reference's operation order,
+SpkLUFactor.jl:269-377): fronts of one block and of several, with and
+without rows below the blocks and update rows, LU and SPD, one and three
+right-hand sides. The two differ only by float rounding (the inverses are
+formed in float); the refined solutions meet the double backward-error bar.
+Six right-hand sides take the substitution kernels in both settings."""
 import os
 
 import numpy as np
@@ -55,7 +56,7 @@ def _solve(s, B, inv, refine):
 
 
 @pytest.mark.parametrize("name", list(CASES))
-@pytest.mark.parametrize("nrhs", [1, 6])
+@pytest.mark.parametrize("nrhs", [1, 3, 6])
 def test_inverse_solves_match_substitution(name, nrhs):
     gen, kind = CASES[name]
     print(f"case inverse solve {name} nrhs={nrhs}", flush=True)
@@ -70,12 +71,15 @@ def test_inverse_solves_match_substitution(name, nrhs):
     B = np.asfortranarray(np.stack([uniform(70 + q, n, -1.0, 1.0) for q in range(nrhs)], axis=1))
     Xs = _solve(s, B, inv=False, refine=False)
     Xi = _solve(s, B, inv=True, refine=False)
+    if nrhs > 3:
+        assert np.array_equal(Xi, Xs), "more than three right-hand sides: substitution"
     for q in range(nrhs):
         rel = np.max(np.abs(Xi[:, q] - Xs[:, q])) / np.max(np.abs(Xs[:, q]))
         assert rel <= 1e-4, f"column {q}: inverse and substitution solves differ by {rel:.2e}"
-    # each column of a many-column solve is that column's own solve, bit for bit
-    if nrhs > 1:
-        x0 = _solve(s, np.asfortranarray(B[:, :1]), inv=True, refine=False)
+    # each column of a three-column inverse solve is that column's own solve,
+    # bit for bit
+    if nrhs == 3:
+        x0 = _solve(s, np.array(B[:, :1], order="F"), inv=True, refine=False)
         assert np.array_equal(x0[:, 0], Xi[:, 0])
     X = _solve(s, B, inv=True, refine=True)
     assert max(backward_error(A, X[:, q], B[:, q]) for q in range(nrhs)) <= 1e-14
