@@ -1480,11 +1480,6 @@ blk_offdiag_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
 // transposed: lane l keeps row l % 16 of four columns, and each group of 16
 // lanes writes 16 contiguous rows of the column-major front.
 
-// float -> bf16, round to nearest even (finite inputs).
-__device__ inline unsigned short f2bf(float x) {
-  const unsigned u = __float_as_uint(x);
-  return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-}
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 
@@ -1512,11 +1507,15 @@ __device__ inline f4u load4(const float* p, int nvalid) {
   return v;
 }
 
+// four floats to bf16 (round to nearest even), two per v_cvt_pk_bf16_f32
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 __device__ inline uint2 pack_bf16(float a, float b, float c, float d) {
-  return uint2{(unsigned)f2bf(a) | ((unsigned)f2bf(b) << 16),
-               (unsigned)f2bf(c) | ((unsigned)f2bf(d) << 16)};
+  const bf16x2 p = {(__bf16)a, (__bf16)b}, q = {(__bf16)c, (__bf16)d};
+  return uint2{__builtin_bit_cast(unsigned, p), __builtin_bit_cast(unsigned, q)};
 }
-__device__ inline float bf_lo(float x) { return x - __uint_as_float((unsigned)f2bf(x) << 16); }
+// the part of x below its bf16 rounding (exact: x and its rounding are floats
+// within a factor of two of each other)
+__device__ inline float bf_lo(float x) { return x - (float)(__bf16)x; }
 
 __device__ void gemm_bf16_chunks(const float* __restrict__ F, int L, int k0, int bw, int r0,
                                  int rend, int j0, int cend, bool split,
