@@ -206,7 +206,8 @@ int64_t spkhip_solve_device(spkhip_t h, int64_t nrhs, double* d_b,
  * GPU it runs on until the process ends; it needs
  *   round_up(n * nrhs * s, 256) + vec * nrhs * s  bytes
  * (s = 8 for an SPKHIP_FP64 factor, 4 otherwise; vec = the handle's solve
- * vector length, about the number of row indices of the fronts) and returns
+ * vector length, about the number of row indices of the fronts, rounded up
+ * to 64; both terms times the number of value sets) and returns
  * -2 when that exceeds 64 MiB (or when the factor's solve data was never
  * sized, which spkhip_factor does). Calls on the same GPU share that array:
  * calls that overlap in time on one GPU (other threads, other handles) race
@@ -239,6 +240,25 @@ int64_t spkhip_condest(spkhip_t h, double* cond1, double* ainv_norm1);
  * offsets[nseg] = n). Default: one segment. No counterpart in the reference
  * (it factors one system at a time). */
 int64_t spkhip_set_segments(spkhip_t h, int64_t nseg, const int64_t* offsets);
+/* Value sets: nvs matrices of the uploaded pattern (same perm, same
+ * symbolic factor, different values) factored and solved together, each
+ * launch covering every set. After this call, and until the next symbolic
+ * upload (which returns to one set):
+ *   spkhip_inmatrix[_device]: nzval holds nvs * nnz values, set v's at v*nnz;
+ *   spkhip_factor: factors every set (a zero pivot in any set returns -1;
+ *     stats name the lowest failing supernode over the sets);
+ *   spkhip_solve*, spkhip_solve_refine*: B has n * nvs rows (ldb >= n*nvs),
+ *     set v's system on rows [v*n, (v+1)*n); the backward error is taken per
+ *     set and per segment of spkhip_set_segments;
+ *   spkhip_download*: set v's lnz at v*nnz_lnz, unz at v*nnz_unz, ipvt at
+ *     v*n of the host arrays.
+ * Every set's result equals, bit for bit, that of a handle holding that set
+ * alone. Values and factor held before the call are dropped (-3 until the
+ * next spkhip_inmatrix). spkhip_condest takes one set only (-4). nvs in
+ * [1, 65535]. The reference factors one matrix at a time; its
+ * refactorisation with new values of one pattern (sparspaklu!,
+ * SparseCSCInterface.jl:247-268) is what this batches. */
+int64_t spkhip_set_value_sets(spkhip_t h, int64_t nvs);
 /* The stream (a hipStream_t, NULL = the legacy default stream) on which the
  * caller produces the device buffers it passes to the _device entry points;
  * they order themselves after that stream's earlier work. No counterpart in

@@ -160,6 +160,10 @@ struct spkhip_handle {
   bool have_sym = false;
   int64_t kind = SPKHIP_KIND_LU;
   i64 n = 0, nsuper = 0, nsub = 0, nlnz = 0, nunz = 0;
+  // value sets: nvs matrices of this one pattern factored and solved
+  // together (set v = blockIdx.y of every per-set launch; its data at the
+  // DevSym vs_* strides; its rows of the caller's right-hand sides at v * n)
+  int nvs = 1;
   std::vector<i64> xsuper, snode, xlnz, xunz, xlindx, perm, invp;
   std::vector<i32> lindx;
   // fronts
@@ -280,6 +284,26 @@ struct spkhip_handle {
 
 namespace {
 
+// Elements of a buffer holding every value set's copy at the given stride.
+inline size_t per_sets(const spkhip_handle* h, int64_t stride) {
+  return (size_t)std::max<int64_t>(stride, 1) * (size_t)h->nvs;
+}
+
+// The strides between value sets' copies of the per-set buffers: each copy
+// rounded up to 64 elements, so that every set's data starts as aligned as
+// the first set's.
+void set_vs_strides(spkhip_handle* h) {
+  auto up = [](i64 x) { return std::max<i64>(64, (x + 63) / 64 * 64); };
+  DevSym& d = h->ds;
+  d.vs_lnz = up(h->nlnz);
+  d.vs_unz = up(h->nunz);
+  d.vs_arena = up(h->arena_elems);
+  d.vs_prep = up(h->prep_units * 4);
+  d.vs_prepi = up((i64)h->pdesc.size() * spk::kPrepInts);
+  d.vs_inv = up(h->inv_elems);
+  d.vs_vec = up(h->vec_elems);
+}
+
 using Clock = std::chrono::steady_clock;
 inline double ms_since(Clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
@@ -300,7 +324,8 @@ struct Prof {
   int cls;
   double flops, bytes;
   size_t ea = 0;
-  Prof(spkhip_handle* hh, int c, double fl, double by) : h(hh), cls(c), flops(fl), bytes(by) {
+  Prof(spkhip_handle* hh, int c, double fl, double by)
+      : h(hh), cls(c), flops(fl * hh->nvs), bytes(by * hh->nvs) {
     if (h->prof) {
       ea = prof_event(h);
       hipck(hipEventRecord(h->evpool[ea], h->stream));
@@ -1021,8 +1046,8 @@ void run_factor(spkhip_handle* h) {
   DevBuf<T>& lnz = Bufs<T>::lnz(h);
   DevBuf<T>& unz = Bufs<T>::unz(h);
   DevBuf<T>& arena = Bufs<T>::arena(h);
-  arena.ensure((size_t)std::max<i64>(h->arena_elems, 1));
-  h->d_ipvt.ensure((size_t)h->n);
+  arena.ensure(per_sets(h, h->ds.vs_arena));
+  h->d_ipvt.ensure((size_t)h->n * h->nvs);
   h->d_fail.ensure(1);
   hipLaunchKernelGGL(spk::fill_kernel<unsigned long long>, dim3(1), dim3(kWG), 0, st,
                      h->d_fail.p, (int64_t)1, ~0ull);
@@ -1045,12 +1070,12 @@ void run_factor(spkhip_handle* h) {
       Prof pr(h, SPKHIP_K_SMALL, lp.small_flops, lp.small_elems * sizeof(T));
       if (lp.small_cnt >= h->small_wave_min) {
         const int fpw = (int)std::max<size_t>(1, std::min<size_t>(4, 150 * 1024 / per));
-        hipLaunchKernelGGL((spk::small_front_kernel<T, 64>), dim3((lp.small_cnt + fpw - 1) / fpw),
+        hipLaunchKernelGGL((spk::small_front_kernel<T, 64>), dim3((lp.small_cnt + fpw - 1) / fpw, h->nvs),
                            dim3(kWG), fpw * per, st, h->ds, h->d_small.p + lp.small_off,
                            lp.small_cnt, fpw, lp.small_maxL, lnz.p, unz.p, h->d_ipvt.p, arena.p,
                            h->d_fail.p);
       } else {
-        hipLaunchKernelGGL((spk::small_front_kernel<T, kWG>), dim3(lp.small_cnt), dim3(kWG), per,
+        hipLaunchKernelGGL((spk::small_front_kernel<T, kWG>), dim3(lp.small_cnt, h->nvs), dim3(kWG), per,
                            st, h->ds, h->d_small.p + lp.small_off, lp.small_cnt, 1,
                            lp.small_maxL, lnz.p, unz.p, h->d_ipvt.p, arena.p, h->d_fail.p);
       }
@@ -1060,7 +1085,7 @@ void run_factor(spkhip_handle* h) {
       {
       Prof pr(h, SPKHIP_K_INIT, 0.0, lp.init_elems * sizeof(T));
       if (lp.ut_cnt > 0)
-        hipLaunchKernelGGL((spk::large_utile_kernel<T, false>), dim3(lp.ut_cnt), dim3(kWG), 0, st,
+        hipLaunchKernelGGL((spk::large_utile_kernel<T, false>), dim3(lp.ut_cnt, h->nvs), dim3(kWG), 0, st,
                            h->ds, h->d_ut_front.p + lp.ut_off, h->d_ut_piece.p + lp.ut_off,
                            h->d_ut_r0.p + lp.ut_off, lnz.p, unz.p, arena.p);
       // the column is assembled in LDS when the longest one fits in 64 KB;
@@ -1068,13 +1093,13 @@ void run_factor(spkhip_handle* h) {
       const size_t clds = (size_t)lp.large_maxL * sizeof(T);
       const bool staged = clds <= 64 * 1024;
       if (lu && clds <= 16 * 1024)
-        hipLaunchKernelGGL(spk::large_init_wave_kernel<T>, dim3((lp.items_cnt + 3) / 4),
+        hipLaunchKernelGGL(spk::large_init_wave_kernel<T>, dim3((lp.items_cnt + 3) / 4, h->nvs),
                            dim3(kWG), 4 * clds, st, h->ds, h->d_it_front.p + lp.items_off,
                            h->d_it_col.p + lp.items_off, h->d_it_ea.p + lp.items_off,
                            h->d_ea_child.p, h->d_ea_t.p, lnz.p, arena.p, lp.items_cnt,
                            lp.large_maxL);
       else
-        hipLaunchKernelGGL(spk::large_init_kernel<T>, dim3(lp.items_cnt), dim3(kWG),
+        hipLaunchKernelGGL(spk::large_init_kernel<T>, dim3(lp.items_cnt, h->nvs), dim3(kWG),
                            staged ? clds : 0, st, h->ds, h->d_it_front.p + lp.items_off,
                            h->d_it_col.p + lp.items_off, h->d_it_ea.p + lp.items_off,
                            h->d_ea_child.p, h->d_ea_t.p, lnz.p, arena.p, (int)staged);
@@ -1086,7 +1111,7 @@ void run_factor(spkhip_handle* h) {
           h->cur_grid = G.diag.wg + G.bin.wg;
           Prof pr(h, SPKHIP_K_DIAG_BORDER, G.diag.flops + G.bin.flops,
                   (G.diag.elems + G.bin.elems) * sizeof(T));
-          hipLaunchKernelGGL((spk::blk_diag_border_kernel<T>), dim3(G.diag.wg + G.bin.wg),
+          hipLaunchKernelGGL((spk::blk_diag_border_kernel<T>), dim3(G.diag.wg + G.bin.wg, h->nvs),
                              dim3(kWG), 0, st, h->ds, h->d_bdesc.p + G.diag.off, G.diag.wg,
                              h->d_gdesc.p + G.bin.off, G.bin.cnt, arena.p, h->d_ipvt.p,
                              h->d_fail.p, h->lowp);
@@ -1095,15 +1120,15 @@ void run_factor(spkhip_handle* h) {
           h->cur_grid = G.diag.wg;
           Prof pr(h, SPKHIP_K_DIAG, G.diag.flops, G.diag.elems * sizeof(T));
           if (slab)
-            hipLaunchKernelGGL((spk::blk_diag_slab_kernel<T>), dim3(G.diag.wg), dim3(kWG), 0, st,
+            hipLaunchKernelGGL((spk::blk_diag_slab_kernel<T>), dim3(G.diag.wg, h->nvs), dim3(kWG), 0, st,
                                h->ds, h->d_bdesc.p + G.diag.off, arena.p, h->d_ipvt.p,
                                h->d_fail.p);
           else if (lu)
-            hipLaunchKernelGGL((spk::blk_diag_kernel<T, kNqLU>), dim3(G.diag.wg), dim3(kWG),
+            hipLaunchKernelGGL((spk::blk_diag_kernel<T, kNqLU>), dim3(G.diag.wg, h->nvs), dim3(kWG),
                                (size_t)G.diag.lds * sizeof(T), st, h->ds,
                                h->d_bdesc.p + G.diag.off, arena.p, h->d_ipvt.p, h->d_fail.p);
           else
-            hipLaunchKernelGGL((spk::blk_diag_kernel<T, kNqSPD>), dim3(G.diag.wg), dim3(kWG),
+            hipLaunchKernelGGL((spk::blk_diag_kernel<T, kNqSPD>), dim3(G.diag.wg, h->nvs), dim3(kWG),
                                (size_t)G.diag.lds * sizeof(T), st, h->ds,
                                h->d_bdesc.p + G.diag.off, arena.p, h->d_ipvt.p, h->d_fail.p);
           hipck(hipGetLastError());
@@ -1112,11 +1137,11 @@ void run_factor(spkhip_handle* h) {
           h->cur_grid = G.offd.wg;
           Prof pr(h, SPKHIP_K_OFFDIAG, G.offd.flops, G.offd.elems * sizeof(T));
           if (lu)
-            hipLaunchKernelGGL((spk::blk_offdiag_kernel<T, kNqLU>), dim3(G.offd.wg), dim3(kWG),
+            hipLaunchKernelGGL((spk::blk_offdiag_kernel<T, kNqLU>), dim3(G.offd.wg, h->nvs), dim3(kWG),
                                (size_t)G.offd.lds * sizeof(T), st, h->ds,
                                h->d_bdesc.p + G.offd.off, G.offd.cnt, arena.p, h->d_ipvt.p);
           else
-            hipLaunchKernelGGL((spk::blk_offdiag_kernel<T, kNqSPD>), dim3(G.offd.wg), dim3(kWG),
+            hipLaunchKernelGGL((spk::blk_offdiag_kernel<T, kNqSPD>), dim3(G.offd.wg, h->nvs), dim3(kWG),
                                (size_t)G.offd.lds * sizeof(T), st, h->ds,
                                h->d_bdesc.p + G.offd.off, G.offd.cnt, arena.p, h->d_ipvt.p);
           hipck(hipGetLastError());
@@ -1129,7 +1154,7 @@ void run_factor(spkhip_handle* h) {
           if (two) stream_join(h, st, st2);
           h->cur_grid = G.gemm2.wg;
           Prof pr(h, SPKHIP_K_UPDATE, G.gemm2.flops, G.gemm2.elems * sizeof(T));
-          hipLaunchKernelGGL(spk::blk_gemm_kernel<T>, dim3(G.gemm2.wg), dim3(kWG), 0, st2, h->ds,
+          hipLaunchKernelGGL(spk::blk_gemm_kernel<T>, dim3(G.gemm2.wg, h->nvs), dim3(kWG), 0, st2, h->ds,
                              h->d_gdesc.p + G.gemm2.off, G.gemm2.cnt, arena.p, h->lowp);
           hipck(hipGetLastError());
           pending = two;
@@ -1137,7 +1162,7 @@ void run_factor(spkhip_handle* h) {
         if (G.gemm.cnt > 0) {
           h->cur_grid = G.gemm.wg;
           Prof pr(h, SPKHIP_K_UPDATE, G.gemm.flops, G.gemm.elems * sizeof(T));
-          hipLaunchKernelGGL(spk::blk_gemm_kernel<T>, dim3(G.gemm.wg), dim3(kWG), 0, st, h->ds,
+          hipLaunchKernelGGL(spk::blk_gemm_kernel<T>, dim3(G.gemm.wg, h->nvs), dim3(kWG), 0, st, h->ds,
                              h->d_gdesc.p + G.gemm.off, G.gemm.cnt, arena.p, h->lowp);
           hipck(hipGetLastError());
         }
@@ -1145,11 +1170,11 @@ void run_factor(spkhip_handle* h) {
       if (pending) stream_join(h, st2, st);
       Prof pr(h, SPKHIP_K_WRITEBACK, 0.0, lp.wb_elems * sizeof(T));
       if (lp.ut_cnt > 0)
-        hipLaunchKernelGGL((spk::large_utile_kernel<T, true>), dim3(lp.ut_cnt), dim3(kWG), 0, st,
+        hipLaunchKernelGGL((spk::large_utile_kernel<T, true>), dim3(lp.ut_cnt, h->nvs), dim3(kWG), 0, st,
                            h->ds, h->d_ut_front.p + lp.ut_off, h->d_ut_piece.p + lp.ut_off,
                            h->d_ut_r0.p + lp.ut_off, lnz.p, unz.p, arena.p);
       if (lp.wb_cnt > 0)
-        hipLaunchKernelGGL(spk::large_writeback_kernel<T>, dim3(lp.wb_cnt), dim3(kWG), 0, st,
+        hipLaunchKernelGGL(spk::large_writeback_kernel<T>, dim3(lp.wb_cnt, h->nvs), dim3(kWG), 0, st,
                            h->ds, h->d_wb_front.p + lp.wb_off, h->d_wb_col.p + lp.wb_off,
                            lnz.p, unz.p, arena.p);
       hipck(hipGetLastError());
@@ -1171,17 +1196,19 @@ void run_assemble(spkhip_handle* h) {
   hipStream_t st = h->stream;
   DevBuf<T>& lnz = Bufs<T>::lnz(h);
   DevBuf<T>& unz = Bufs<T>::unz(h);
-  lnz.ensure((size_t)std::max<i64>(h->nlnz, 1));
-  unz.ensure((size_t)std::max<i64>(h->nunz, 1));
+  const size_t nl = per_sets(h, h->ds.vs_lnz), nu = per_sets(h, h->ds.vs_unz);
+  lnz.ensure(nl);
+  unz.ensure(nu);
   Prof pr(h, SPKHIP_K_ASSEMBLE, 0.0,
           (double)(h->nlnz + h->nunz) * sizeof(T) + (double)h->nnz * (16.0 + sizeof(T)));
-  hipLaunchKernelGGL(spk::fill_kernel<T>, dim3(grid_for(h->nlnz)), dim3(kWG), 0, st, lnz.p,
-                     h->nlnz, T(0));
-  hipLaunchKernelGGL(spk::fill_kernel<T>, dim3(grid_for(h->nunz)), dim3(kWG), 0, st, unz.p,
-                     h->nunz, T(0));
+  hipLaunchKernelGGL(spk::fill_kernel<T>, dim3(grid_for((i64)nl)), dim3(kWG), 0, st, lnz.p,
+                     (int64_t)nl, T(0));
+  hipLaunchKernelGGL(spk::fill_kernel<T>, dim3(grid_for((i64)nu)), dim3(kWG), 0, st, unz.p,
+                     (int64_t)nu, T(0));
   if (h->nnz > 0)
-    hipLaunchKernelGGL(spk::scatter_a_kernel<T>, dim3(grid_for(h->nnz)), dim3(kWG), 0, st,
-                       h->d_nzval.p, h->d_amap.p, h->nnz, lnz.p, unz.p);
+    hipLaunchKernelGGL(spk::scatter_a_kernel<T>, dim3(grid_for(h->nnz), h->nvs), dim3(kWG), 0,
+                       st, h->d_nzval.p, h->d_amap.p, h->nnz, lnz.p, unz.p, h->ds.vs_lnz,
+                       h->ds.vs_unz);
   hipck(hipGetLastError());
 }
 
@@ -1210,17 +1237,17 @@ int small_fronts_per_wg(int maxL, int qb = 1, int maxq = 1) {
 template <typename T>
 void size_solve_data(spkhip_handle* h) {
   if (h->pdesc.empty()) return;
-  Bufs<T>::prep(h).ensure((size_t)h->prep_units * 4);
-  h->d_prepi.ensure(h->pdesc.size() * spk::kPrepInts);
-  if (std::is_same<T, float>::value && inv_solves()) h->d_inv32.ensure((size_t)h->inv_elems);
+  Bufs<T>::prep(h).ensure(per_sets(h, h->ds.vs_prep));
+  h->d_prepi.ensure(per_sets(h, h->ds.vs_prepi));
+  if (std::is_same<T, float>::value && inv_solves()) h->d_inv32.ensure(per_sets(h, h->ds.vs_inv));
 }
 template <typename T>
 bool solve_data_sized(spkhip_handle* h) {
   return h->pdesc.empty() ||
-         (Bufs<T>::prep(h).n >= (size_t)h->prep_units * 4 &&
-          h->d_prepi.n >= h->pdesc.size() * spk::kPrepInts &&
+         (Bufs<T>::prep(h).n >= per_sets(h, h->ds.vs_prep) &&
+          h->d_prepi.n >= per_sets(h, h->ds.vs_prepi) &&
           !(std::is_same<T, float>::value && inv_solves() &&
-            h->d_inv32.n < (size_t)h->inv_elems));
+            h->d_inv32.n < per_sets(h, h->ds.vs_inv)));
 }
 
 // The per-block solve data of the current factor (once per factorisation),
@@ -1232,7 +1259,7 @@ void solve_prep(spkhip_handle* h, bool inv) {
   if (!h->prepped) {
     size_solve_data<T>(h);
     Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
-    hipLaunchKernelGGL(spk::solve_prep_kernel<T>, dim3((unsigned)h->pdesc.size()), dim3(kWG), 0,
+    hipLaunchKernelGGL(spk::solve_prep_kernel<T>, dim3((unsigned)h->pdesc.size(), h->nvs), dim3(kWG), 0,
                        h->stream, h->ds, h->d_pdesc.p, Bufs<T>::lnz(h).p, Bufs<T>::unz(h).p,
                        h->d_ipvt.p, prep.p, h->d_prepi.p);
     hipck(hipGetLastError());
@@ -1242,9 +1269,10 @@ void solve_prep(spkhip_handle* h, bool inv) {
     if (inv && !h->inv_prepped) {
       size_solve_data<T>(h);
       Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
-      hipLaunchKernelGGL(spk::solve_inv_prep_kernel, dim3((unsigned)h->pdesc.size()), dim3(kWG),
+      hipLaunchKernelGGL(spk::solve_inv_prep_kernel, dim3((unsigned)h->pdesc.size(), h->nvs), dim3(kWG),
                          (size_t)spk::kInvPrepLds, h->stream, h->d_pdesc.p, prep.p,
-                         h->d_prepi.p, h->d_inv32.p);
+                         h->d_prepi.p, h->d_inv32.p, h->ds.vs_prep, h->ds.vs_prepi,
+                         h->ds.vs_inv);
       hipck(hipGetLastError());
       h->inv_prepped = true;
     }
@@ -1271,7 +1299,7 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
         const int qb = small_rhs_block(nrhs);
         const int fpw = small_fronts_per_wg<T, MAXW>(lp.small_maxL, qb, lp.small_maxq);
         hipLaunchKernelGGL((spk::fwd_small_kernel<T, MAXW>),
-                           dim3((lp.small_cnt + fpw - 1) / fpw), dim3(kWG),
+                           dim3((lp.small_cnt + fpw - 1) / fpw, h->nvs), dim3(kWG),
                            (size_t)fpw * (lp.small_maxq + qb) * lp.small_maxL * sizeof(T), st,
                            h->ds, h->d_small.p + lp.small_off, lp.small_cnt, fpw, qb, lnz,
                            h->d_ipvt.p, rhs, ldr, nrhs, vecp, lp.small_maxL, lp.small_maxq);
@@ -1281,7 +1309,7 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
       {
         Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
         hipLaunchKernelGGL(spk::fwd_assemble_items_kernel<T>,
-                           dim3((lp.items_cnt + kWG - 1) / kWG), dim3(kWG), 0, st, h->ds,
+                           dim3((lp.items_cnt + kWG - 1) / kWG, h->nvs), dim3(kWG), 0, st, h->ds,
                            h->d_it_front.p + lp.items_off, h->d_it_col.p + lp.items_off,
                            h->d_it_ea.p + lp.items_off, h->d_ea_child.p, h->d_ea_t.p, rhs, ldr,
                            nrhs, vecp, lp.items_cnt);
@@ -1292,7 +1320,7 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
           for (size_t k = 0; k < lp.ifw.size(); ++k) {
             const double el = lp.fsteps[k].elems;
             Prof pr(h, SPKHIP_K_FWD_GEMV, 2.0 * el * nrhs, el * sizeof(T));
-            hipLaunchKernelGGL(spk::fwd_inv_kernel, dim3(lp.ifw[k].second), dim3(kWG), 0, st,
+            hipLaunchKernelGGL(spk::fwd_inv_kernel, dim3(lp.ifw[k].second, h->nvs), dim3(kWG), 0, st,
                                h->ds, h->d_iitems.p + lp.ifw[k].first, h->d_inv32.p, lnz, rhs,
                                ldr, nrhs, vecp);
             hipck(hipGetLastError());
@@ -1303,7 +1331,7 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
       for (const LevelPlan::SolveStep& S : lp.fsteps) {
         {
           Prof pr(h, SPKHIP_K_FWD, 0.0, 0.0);
-          hipLaunchKernelGGL((spk::fwd_chain_kernel<T>), dim3(S.chain_cnt), dim3(kWG),
+          hipLaunchKernelGGL((spk::fwd_chain_kernel<T>), dim3(S.chain_cnt, h->nvs), dim3(kWG),
                              (size_t)spk::kPanelElems * sizeof(T), st, h->ds,
                              h->d_sdesc.p + S.chain_off, prep.p, h->d_prepi.p, rhs, ldr, nrhs,
                              vecp);
@@ -1311,7 +1339,7 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
         }
         if (S.gemv_cnt > 0) {
           Prof pr(h, SPKHIP_K_FWD_GEMV, 2.0 * S.elems * nrhs, S.elems * sizeof(T));
-          hipLaunchKernelGGL(spk::fwd_bgemv_kernel<T>, dim3(S.gemv_cnt), dim3(kWG), 0, st, h->ds,
+          hipLaunchKernelGGL(spk::fwd_bgemv_kernel<T>, dim3(S.gemv_cnt, h->nvs), dim3(kWG), 0, st, h->ds,
                              h->d_sdesc.p + S.gemv_off, lnz, nrhs, vecp);
           hipck(hipGetLastError());
         }
@@ -1325,7 +1353,7 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
       if (lp.large_cnt > 0) {
         Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);
         hipLaunchKernelGGL(spk::bwd_gather_items_kernel<T>,
-                           dim3((lp.items_cnt + kWG - 1) / kWG), dim3(kWG), 0, st, h->ds,
+                           dim3((lp.items_cnt + kWG - 1) / kWG, h->nvs), dim3(kWG), 0, st, h->ds,
                            h->d_it_front.p + lp.items_off, h->d_it_col.p + lp.items_off, rhs,
                            ldr, nrhs, vecp, lp.items_cnt);
         hipck(hipGetLastError());
@@ -1337,13 +1365,13 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
             if (lp.ibw[k].second == 0) continue;
             const double el = lp.bsteps[k].elems;
             Prof pr(h, SPKHIP_K_BWD_GEMV, 2.0 * el * nrhs, el * sizeof(T));
-            hipLaunchKernelGGL(spk::bwd_inv_kernel, dim3(lp.ibw[k].second), dim3(kWG), 0, st,
+            hipLaunchKernelGGL(spk::bwd_inv_kernel, dim3(lp.ibw[k].second, h->nvs), dim3(kWG), 0, st,
                                h->ds, h->d_iitems.p + lp.ibw[k].first, h->d_inv32.p, unz, rhs,
                                ldr, nrhs, vecp);
             hipck(hipGetLastError());
           }
           Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);
-          hipLaunchKernelGGL(spk::bwd_inv_kernel, dim3(lp.ifin.second), dim3(kWG), 0, st, h->ds,
+          hipLaunchKernelGGL(spk::bwd_inv_kernel, dim3(lp.ifin.second, h->nvs), dim3(kWG), 0, st, h->ds,
                              h->d_iitems.p + lp.ifin.first, h->d_inv32.p, unz, rhs, ldr, nrhs,
                              vecp);
           hipck(hipGetLastError());
@@ -1354,12 +1382,12 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
         if (done) break;
         if (S.gemv_cnt > 0) {
           Prof pr(h, SPKHIP_K_BWD_GEMV, 2.0 * S.elems * nrhs, S.elems * sizeof(T));
-          hipLaunchKernelGGL(spk::bwd_bgemv_kernel<T>, dim3(S.gemv_cnt), dim3(kWG), 0, st, h->ds,
+          hipLaunchKernelGGL(spk::bwd_bgemv_kernel<T>, dim3(S.gemv_cnt, h->nvs), dim3(kWG), 0, st, h->ds,
                              h->d_sdesc.p + S.gemv_off, unz, nrhs, vecp);
           hipck(hipGetLastError());
         }
         Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);
-        hipLaunchKernelGGL((spk::bwd_chain_kernel<T>), dim3(S.chain_cnt), dim3(kWG),
+        hipLaunchKernelGGL((spk::bwd_chain_kernel<T>), dim3(S.chain_cnt, h->nvs), dim3(kWG),
                            (size_t)(spk::kPanelElems + 128) * sizeof(T), st, h->ds,
                            h->d_sdesc.p + S.chain_off, prep.p, rhs, ldr, nrhs, vecp);
         hipck(hipGetLastError());
@@ -1369,7 +1397,7 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
         const int qb = small_rhs_block(nrhs);
         const int fpw = small_fronts_per_wg<T, MAXW>(lp.small_maxL, qb, lp.small_maxq);
         hipLaunchKernelGGL((spk::bwd_small_kernel<T, MAXW>),
-                           dim3((lp.small_cnt + fpw - 1) / fpw), dim3(kWG),
+                           dim3((lp.small_cnt + fpw - 1) / fpw, h->nvs), dim3(kWG),
                            (size_t)fpw * (lp.small_maxq + qb) * lp.small_maxL * sizeof(T), st,
                            h->ds, h->d_small.p + lp.small_off, lp.small_cnt, fpw, qb, lnz, unz,
                            rhs, ldr, nrhs, lp.small_maxL, lp.small_maxq);
@@ -1397,7 +1425,7 @@ void solve_levels(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bw
   const char* ge = std::getenv("SPKHIP_SOLVE_GRAPH");
   const bool use_graph = !(ge && ge[0] == '0');
   DevBuf<T>& vec = Bufs<T>::vec(h);
-  vec.ensure((size_t)std::max<i64>(h->vec_elems * nrhs, 1));
+  vec.ensure(per_sets(h, h->ds.vs_vec) * nrhs);
   if (!use_graph || h->prof || !(fwd && bwd)) {
     solve_levels_direct<T>(h, rhs, ldr, nrhs, fwd, bwd, vec.p);
     return;
@@ -1410,7 +1438,7 @@ void solve_levels(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool bw
       (const void*)rhs, (const void*)(intptr_t)ldr, (const void*)(intptr_t)nrhs,
       (const void*)(intptr_t)sizeof(T), (const void*)(intptr_t)h->sym_gen, vec.p,
       Bufs<T>::prep(h).p, h->d_prepi.p, Bufs<T>::lnz(h).p, Bufs<T>::unz(h).p, h->d_ipvt.p,
-      (const void*)(intptr_t)inv_solve<T>(nrhs), h->d_inv32.p};
+      (const void*)(intptr_t)inv_solve<T>(nrhs), h->d_inv32.p, (const void*)(intptr_t)h->nvs};
   if (!h->g_exec || h->g_key != key) {
     if (h->g_seen != key) {
       h->g_seen = key;
@@ -1438,25 +1466,27 @@ template <typename T>
 void solve_device(spkhip_handle* h, double* d_b, i64 ldb, int nrhs) {
   hipStream_t st = h->stream;
   DevBuf<T>& rhs = Bufs<T>::rhs(h);
-  rhs.ensure((size_t)h->n * nrhs);
-  hipLaunchKernelGGL((spk::permute_kernel<double, T>), dim3(grid_for(h->n)), dim3(kWG), 0, st,
-                     d_b, ldb, rhs.p, h->n, h->d_perm.p, h->n, nrhs);
-  solve_levels<T>(h, rhs.p, h->n, nrhs, true, true);
-  hipLaunchKernelGGL((spk::permute_kernel<T, double>), dim3(grid_for(h->n)), dim3(kWG), 0, st,
-                     rhs.p, h->n, d_b, ldb, h->d_invp.p, h->n, nrhs);
+  const i64 n = h->n, ns = n * nrhs;  // ns: one set's permuted right-hand sides
+  rhs.ensure((size_t)ns * h->nvs);
+  hipLaunchKernelGGL((spk::permute_kernel<double, T>), dim3(grid_for(n), h->nvs), dim3(kWG), 0,
+                     st, d_b, ldb, rhs.p, n, h->d_perm.p, n, nrhs, n, ns);
+  solve_levels<T>(h, rhs.p, n, nrhs, true, true);
+  hipLaunchKernelGGL((spk::permute_kernel<T, double>), dim3(grid_for(n), h->nvs), dim3(kWG), 0,
+                     st, rhs.p, n, d_b, ldb, h->d_invp.p, n, nrhs, ns, n);
   hipck(hipGetLastError());
 }
 
-// The same solve with the permuted right-hand sides (n x nrhs) and the
-// vector arena in the given device memory, launched directly (no graph).
+// The same solve with the permuted right-hand sides (n x nrhs per value set)
+// and the vector arena in the given device memory, launched directly (no graph).
 template <typename T>
 void solve_device_in(spkhip_handle* h, double* d_b, i64 ldb, int nrhs, T* rhs, T* vecp) {
   hipStream_t st = h->stream;
-  hipLaunchKernelGGL((spk::permute_kernel<double, T>), dim3(grid_for(h->n)), dim3(kWG), 0, st,
-                     d_b, ldb, rhs, h->n, h->d_perm.p, h->n, nrhs);
-  solve_levels_direct<T>(h, rhs, h->n, nrhs, true, true, vecp);
-  hipLaunchKernelGGL((spk::permute_kernel<T, double>), dim3(grid_for(h->n)), dim3(kWG), 0, st,
-                     rhs, h->n, d_b, ldb, h->d_invp.p, h->n, nrhs);
+  const i64 n = h->n, ns = n * nrhs;
+  hipLaunchKernelGGL((spk::permute_kernel<double, T>), dim3(grid_for(n), h->nvs), dim3(kWG), 0,
+                     st, d_b, ldb, rhs, n, h->d_perm.p, n, nrhs, n, ns);
+  solve_levels_direct<T>(h, rhs, n, nrhs, true, true, vecp);
+  hipLaunchKernelGGL((spk::permute_kernel<T, double>), dim3(grid_for(n), h->nvs), dim3(kWG), 0,
+                     st, rhs, n, d_b, ldb, h->d_invp.p, n, nrhs, ns, n);
   hipck(hipGetLastError());
 }
 
@@ -1481,15 +1511,16 @@ void factor_run(spkhip_handle* h) {
   const char* ge = std::getenv("SPKHIP_FACTOR_GRAPH");
   const bool use_graph = !(ge && ge[0] == '0') && !h->prof;
   // every buffer sized before any capture
-  Bufs<T>::lnz(h).ensure((size_t)std::max<i64>(h->nlnz, 1));
-  Bufs<T>::unz(h).ensure((size_t)std::max<i64>(h->nunz, 1));
-  Bufs<T>::arena(h).ensure((size_t)std::max<i64>(h->arena_elems, 1));
-  h->d_ipvt.ensure((size_t)h->n);
+  Bufs<T>::lnz(h).ensure(per_sets(h, h->ds.vs_lnz));
+  Bufs<T>::unz(h).ensure(per_sets(h, h->ds.vs_unz));
+  Bufs<T>::arena(h).ensure(per_sets(h, h->ds.vs_arena));
+  h->d_ipvt.ensure((size_t)h->n * h->nvs);
   h->d_fail.ensure(1);
   const std::vector<const void*> key = {
       (const void*)(intptr_t)sizeof(T), (const void*)(intptr_t)h->lowp,
       (const void*)(intptr_t)h->sym_gen, Bufs<T>::lnz(h).p, Bufs<T>::unz(h).p,
-      Bufs<T>::arena(h).p, h->d_ipvt.p, h->d_fail.p, h->d_nzval.p, h->d_amap.p};
+      Bufs<T>::arena(h).p, h->d_ipvt.p, h->d_fail.p, h->d_nzval.p, h->d_amap.p,
+      (const void*)(intptr_t)h->nvs};
   if (!use_graph) {
     factor_launches<T>(h);
   } else if (h->fg_exec && h->fg_key == key) {
@@ -1584,6 +1615,7 @@ void replan(spkhip_handle* h) {
   d.fcb = h->d_fcb.p;
   d.fcbld = h->d_fcbld.p;
   d.fvec = h->d_fvec.p;
+  set_vs_strides(h);
   sync(h);
   h->sym_gen += 1;
   h->factored = false;
@@ -1713,6 +1745,7 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   d.ucol = h->d_ucol.p;
   d.pivot = (kind == SPKHIP_KIND_LU) ? 1 : 0;
   d.sym = (kind == SPKHIP_KIND_SPD) ? 1 : 0;
+  h->nvs = 1;  // a new pattern starts with one value set
   replan(h);
   h->have_sym = true;
   h->have_csc = false;
@@ -1720,65 +1753,56 @@ void upload_symbolic(spkhip_handle* h, int64_t kind, i64 n, i64 nsuper, const i6
   h->t_upload = ms_since(t0);
 }
 
+// One value set's device copy (stride-apart) to the host, converted to T.
+template <typename T, typename S>
+void download_one(spkhip_handle* h, const S* src, T* dst, i64 len, DevBuf<T>& tmp) {
+  hipStream_t st = h->stream;
+  if (len <= 0) return;
+  if constexpr (std::is_same<T, S>::value) {
+    hipck(hipMemcpyAsync(dst, src, len * sizeof(T), hipMemcpyDeviceToHost, st));
+  } else {
+    tmp.ensure((size_t)len);
+    hipLaunchKernelGGL((spk::convert_kernel<S, T>), dim3(grid_for(len)), dim3(kWG), 0, st, src,
+                       tmp.p, len);
+    hipck(hipMemcpyAsync(dst, tmp.p, len * sizeof(T), hipMemcpyDeviceToHost, st));
+  }
+  sync(h);
+}
+
+// The factor in the reference layout: set v's lnz at v * nlnz of the host
+// array, its unz at v * nunz and its pivots at v * n.
 template <typename T>
 int64_t download_impl(spkhip_handle* h, T* lnz, T* unz, int64_t* ipvt) {
   hipStream_t st = h->stream;
   const i64 nl = h->nlnz, nu = h->nunz;
-  if (h->precision == SPKHIP_FP64) {
-    if (std::is_same<T, double>::value) {
-      if (lnz) hipck(hipMemcpyAsync(lnz, h->d_lnz64.p, nl * sizeof(T), hipMemcpyDeviceToHost, st));
-      if (unz && h->kind == SPKHIP_KIND_LU)
-        hipck(hipMemcpyAsync(unz, h->d_unz64.p, nu * sizeof(T), hipMemcpyDeviceToHost, st));
+  DevBuf<T> tmp;
+  const bool lu = h->kind == SPKHIP_KIND_LU;
+  for (i64 v = 0; v < h->nvs; ++v) {
+    if (h->precision == SPKHIP_FP64) {
+      if (lnz) download_one<T, double>(h, h->d_lnz64.p + v * h->ds.vs_lnz, lnz + v * nl, nl, tmp);
+      if (unz && lu)
+        download_one<T, double>(h, h->d_unz64.p + v * h->ds.vs_unz, unz + v * nu, nu, tmp);
     } else {
-      DevBuf<float> tmp;
-      tmp.alloc(std::max(nl, nu));
-      if (lnz) {
-        hipLaunchKernelGGL((spk::convert_kernel<double, float>), dim3(grid_for(nl)), dim3(kWG), 0,
-                           st, h->d_lnz64.p, tmp.p, nl);
-        hipck(hipMemcpyAsync(lnz, tmp.p, nl * sizeof(float), hipMemcpyDeviceToHost, st));
-        sync(h);
-      }
-      if (unz && h->kind == SPKHIP_KIND_LU) {
-        hipLaunchKernelGGL((spk::convert_kernel<double, float>), dim3(grid_for(nu)), dim3(kWG), 0,
-                           st, h->d_unz64.p, tmp.p, nu);
-        hipck(hipMemcpyAsync(unz, tmp.p, nu * sizeof(float), hipMemcpyDeviceToHost, st));
-      }
-      sync(h);
-    }
-  } else {
-    if (std::is_same<T, float>::value) {
-      if (lnz) hipck(hipMemcpyAsync(lnz, h->d_lnz32.p, nl * sizeof(T), hipMemcpyDeviceToHost, st));
-      if (unz && h->kind == SPKHIP_KIND_LU)
-        hipck(hipMemcpyAsync(unz, h->d_unz32.p, nu * sizeof(T), hipMemcpyDeviceToHost, st));
-    } else {
-      DevBuf<double> tmp;
-      tmp.alloc(std::max(nl, nu));
-      if (lnz) {
-        hipLaunchKernelGGL((spk::convert_kernel<float, double>), dim3(grid_for(nl)), dim3(kWG), 0,
-                           st, h->d_lnz32.p, tmp.p, nl);
-        hipck(hipMemcpyAsync(lnz, tmp.p, nl * sizeof(double), hipMemcpyDeviceToHost, st));
-        sync(h);
-      }
-      if (unz && h->kind == SPKHIP_KIND_LU) {
-        hipLaunchKernelGGL((spk::convert_kernel<float, double>), dim3(grid_for(nu)), dim3(kWG), 0,
-                           st, h->d_unz32.p, tmp.p, nu);
-        hipck(hipMemcpyAsync(unz, tmp.p, nu * sizeof(double), hipMemcpyDeviceToHost, st));
-      }
-      sync(h);
+      if (lnz) download_one<T, float>(h, h->d_lnz32.p + v * h->ds.vs_lnz, lnz + v * nl, nl, tmp);
+      if (unz && lu)
+        download_one<T, float>(h, h->d_unz32.p + v * h->ds.vs_unz, unz + v * nu, nu, tmp);
     }
   }
-  if (ipvt) hipck(hipMemcpyAsync(ipvt, h->d_ipvt.p, h->n * sizeof(i64), hipMemcpyDeviceToHost, st));
+  if (ipvt)
+    hipck(hipMemcpyAsync(ipvt, h->d_ipvt.p, h->n * h->nvs * sizeof(i64), hipMemcpyDeviceToHost,
+                         st));
   sync(h);
   if (h->kind == SPKHIP_KIND_SPD && lnz) {
     // LDL^T layout: D on the diagonal, unit L below, zeros above
     // (SpkLDLtFactor.jl, rectangular panels of SpkSparseSpdBase.jl:359-375).
-    for (i64 J = 0; J < h->nsuper; ++J) {
-      i64 fj = h->xsuper[J] - 1, nj = h->xsuper[J + 1] - h->xsuper[J];
-      i64 jl = h->xlindx[J + 1] - h->xlindx[J];
-      T* p = lnz + (h->xlnz[fj] - 1);
-      for (i64 c = 0; c < nj; ++c)
-        for (i64 r = 0; r < c; ++r) p[r + c * jl] = T(0);
-    }
+    for (i64 v = 0; v < h->nvs; ++v)
+      for (i64 J = 0; J < h->nsuper; ++J) {
+        i64 fj = h->xsuper[J] - 1, nj = h->xsuper[J + 1] - h->xsuper[J];
+        i64 jl = h->xlindx[J + 1] - h->xlindx[J];
+        T* p = lnz + v * nl + (h->xlnz[fj] - 1);
+        for (i64 c = 0; c < nj; ++c)
+          for (i64 r = 0; r < c; ++r) p[r + c * jl] = T(0);
+      }
   }
   return 0;
 }
@@ -1978,22 +2002,25 @@ int64_t spkhip_symbolic_upload(spkhip_t h, int64_t kind, int64_t n, int64_t nsup
 }  // extern "C"
 
 namespace {
+// value set blockIdx.y: its nnz values in both orders nnz further
 __global__ void csr_values_kernel(const double* __restrict__ nz, const int64_t* __restrict__ c2r,
                                   double* __restrict__ csr, int64_t nnz) {
+  nz += (int64_t)blockIdx.y * nnz;
+  csr += (int64_t)blockIdx.y * nnz;
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < nnz) csr[c2r[k]] = nz[k];
 }
 
 int64_t inmatrix_common(spkhip_t h) {
   hipStream_t st = h->stream;
-  h->d_csrval.ensure((size_t)std::max<i64>(h->nnz, 1));
-  h->d_rowabs.ensure((size_t)h->n);
+  h->d_csrval.ensure(per_sets(h, h->nnz));
+  h->d_rowabs.ensure((size_t)h->n * h->nvs);
   h->d_norms.ensure(1);
   if (h->nnz > 0)
-    hipLaunchKernelGGL(csr_values_kernel, dim3(grid_for(h->nnz)), dim3(kWG), 0, st,
+    hipLaunchKernelGGL(csr_values_kernel, dim3(grid_for(h->nnz), h->nvs), dim3(kWG), 0, st,
                        h->d_nzval.p, h->d_csc2csr.p, h->d_csrval.p, h->nnz);
-  hipLaunchKernelGGL(spk::rowabs_kernel, dim3(grid_for(h->n)), dim3(kWG), 0, st, h->d_rowptr.p,
-                     h->d_csrval.p, h->d_rowabs.p, h->n);
+  hipLaunchKernelGGL(spk::rowabs_kernel, dim3(grid_for(h->n), h->nvs), dim3(kWG), 0, st,
+                     h->d_rowptr.p, h->d_csrval.p, h->d_rowabs.p, h->n, h->nnz);
   hipck(hipGetLastError());
   sync(h);
   h->have_values = true;
@@ -2040,7 +2067,7 @@ int64_t spkhip_inmatrix(spkhip_t h, const double* nzval) {
   if (h->pattern_error) return -1;
   return wrap([&]() -> int64_t {
     auto t0 = Clock::now();
-    h->d_nzval.upload(nzval, (size_t)h->nnz, h->stream);
+    h->d_nzval.upload(nzval, (size_t)h->nnz * h->nvs, h->stream);
     int64_t rc = inmatrix_common(h);
     h->t_inmatrix = ms_since(t0);
     return rc;
@@ -2053,10 +2080,10 @@ int64_t spkhip_inmatrix_device(spkhip_t h, const double* d_nzval) {
   if (h->pattern_error) return -1;
   return wrap([&]() -> int64_t {
     auto t0 = Clock::now();
-    h->d_nzval.ensure((size_t)std::max<i64>(h->nnz, 1));
+    h->d_nzval.ensure(per_sets(h, h->nnz));
     wait_caller(h);
     if (h->nnz)
-      hipck(hipMemcpyAsync(h->d_nzval.p, d_nzval, h->nnz * sizeof(double),
+      hipck(hipMemcpyAsync(h->d_nzval.p, d_nzval, h->nnz * h->nvs * sizeof(double),
                            hipMemcpyDeviceToDevice, h->stream));
     int64_t rc = inmatrix_common(h);
     h->t_inmatrix = ms_since(t0);
@@ -2104,7 +2131,7 @@ int64_t spkhip_factor(spkhip_t h, int64_t precision) {
 int64_t spkhip_solve_device(spkhip_t h, int64_t nrhs, double* d_b, int64_t ldb) {
   if (guard(h)) return -4;
   if (!h->factored) return -3;
-  if (nrhs < 0 || ldb < h->n) return -4;
+  if (nrhs < 0 || ldb < h->n * h->nvs) return -4;
   if (nrhs == 0) return 0;
   return wrap([&]() -> int64_t {
     wait_caller(h);
@@ -2126,15 +2153,16 @@ int64_t spkhip_solve_device(spkhip_t h, int64_t nrhs, double* d_b, int64_t ldb) 
 int64_t spkhip_solve_device_noalloc(spkhip_t h, int64_t nrhs, double* d_b, int64_t ldb) {
   if (guard(h)) return -4;
   if (!h->factored) return -3;
-  if (nrhs < 0 || ldb < h->n || !d_b) return -4;
+  if (nrhs < 0 || ldb < h->n * h->nvs || !d_b) return -4;
   if (nrhs == 0) return 0;
   if (!h->scratch) return -4;
   const bool f64 = h->precision == SPKHIP_FP64;
   const size_t es = f64 ? sizeof(double) : sizeof(float);
-  const size_t per_rhs = ((size_t)h->n + (size_t)std::max<i64>(h->vec_elems, 1)) * es;
+  // every value set's permuted right-hand sides, then every set's vectors
+  const size_t per_rhs = ((size_t)h->n + (size_t)h->ds.vs_vec) * h->nvs * es;
   if ((size_t)nrhs > kScratchBytes / per_rhs) return -2;
-  const size_t rhs_bytes = ((size_t)h->n * nrhs * es + 255) & ~size_t(255);
-  const size_t vec_bytes = (size_t)std::max<i64>(h->vec_elems, 1) * nrhs * es;
+  const size_t rhs_bytes = ((size_t)h->n * nrhs * h->nvs * es + 255) & ~size_t(255);
+  const size_t vec_bytes = per_sets(h, h->ds.vs_vec) * nrhs * es;
   if (rhs_bytes + vec_bytes > kScratchBytes) return -2;
   if (!(f64 ? solve_data_sized<double>(h) : solve_data_sized<float>(h))) return -2;
   return wrap([&]() -> int64_t {
@@ -2166,10 +2194,10 @@ int64_t spkhip_solve_device_noalloc(spkhip_t h, int64_t nrhs, double* d_b, int64
 int64_t spkhip_solve(spkhip_t h, int64_t nrhs, double* b, int64_t ldb) {
   if (guard(h)) return -4;
   if (!h->factored) return -3;
-  if (nrhs < 0 || ldb < h->n) return -4;
+  if (nrhs < 0 || ldb < h->n * h->nvs) return -4;
   if (nrhs == 0) return 0;
   return wrap([&]() -> int64_t {
-    const i64 n = h->n;
+    const i64 n = h->n * h->nvs;  // every value set's rows
     h->d_b.ensure((size_t)n * nrhs);
     for (i64 q = 0; q < nrhs; ++q)
       hipck(hipMemcpyAsync(h->d_b.p + q * n, b + q * ldb, n * sizeof(double),
@@ -2188,12 +2216,15 @@ int64_t spkhip_solve(spkhip_t h, int64_t nrhs, double* b, int64_t ldb) {
 
 namespace {
 // Iterative refinement on device data: X (n x nrhs, ld n) receives the
-// solution of A X = Bm (both double, original numbering).
+// solution of A X = Bm (both double, original numbering). With several
+// value sets, n counts every set's rows (set v's rows from v * n1, n1 the
+// order of one matrix) and each set's rows are segments of their own.
 template <typename T>
 int64_t refine_impl(spkhip_t h, const double* d_bm, double* d_xo, int nrhs, double tol,
                     int64_t max_iter) {
   hipStream_t st = h->stream;
-  const i64 n = h->n;
+  const i64 n1 = h->n, n = n1 * h->nvs, ns = n1 * nrhs;
+  const unsigned nvs = (unsigned)h->nvs;
   h->d_r.ensure((size_t)n * nrhs);
   h->d_norms.ensure((size_t)3 * nrhs + 1);
   DevBuf<T>& rhs = Bufs<T>::rhs(h);
@@ -2212,9 +2243,13 @@ int64_t refine_impl(spkhip_t h, const double* d_bm, double* d_xo, int nrhs, doub
   // x0 = solve(b)
   hipLaunchKernelGGL(spk::fill_kernel<double>, dim3(grid_for(n * nrhs)), dim3(kWG), 0, st, d_xo,
                      n * nrhs, 0.0);
-  // per (segment, column) norms: r, x, b, and ||A||_inf per segment
-  if (h->seg.size() < 2 || h->seg.back() != n) h->seg = {0, n};
-  const int nseg = (int)h->seg.size() - 1;
+  // per (segment, column) norms: r, x, b, and ||A||_inf per segment; the
+  // caller's segments of one matrix repeat in every value set
+  if (h->seg.size() < 2 || h->seg.back() != n1) h->seg = {0, n1};
+  std::vector<i64> segs{0};
+  for (i64 v = 0; v < h->nvs; ++v)
+    for (size_t i = 1; i < h->seg.size(); ++i) segs.push_back(v * n1 + h->seg[i]);
+  const int nseg = (int)segs.size() - 1;
   h->d_norms.ensure((size_t)3 * nseg * nrhs + nseg);
   double* nr_r = h->d_norms.p;
   double* nr_x = nr_r + (size_t)nseg * nrhs;
@@ -2223,11 +2258,11 @@ int64_t refine_impl(spkhip_t h, const double* d_bm, double* d_xo, int nrhs, doub
   std::vector<double> nrm((size_t)3 * nseg * nrhs + nseg);
   // row chunks of at most 8192 rows, none crossing a segment boundary
   // (built once per segment layout)
-  if (h->chunk_key != h->seg) {
+  if (h->chunk_key != segs) {
     std::vector<i64> clo;
     std::vector<i32> cseg;
     for (int i = 0; i < nseg; ++i)
-      for (i64 r = h->seg[i]; r < h->seg[i + 1]; r += 8192) {
+      for (i64 r = segs[i]; r < segs[i + 1]; r += 8192) {
         clo.push_back(r);
         cseg.push_back(i);
       }
@@ -2235,7 +2270,7 @@ int64_t refine_impl(spkhip_t h, const double* d_bm, double* d_xo, int nrhs, doub
     clo.push_back(n);
     h->d_clo.upload(clo, st);
     h->d_cseg.upload(cseg.empty() ? std::vector<i32>{0} : cseg, st);
-    h->chunk_key = h->seg;
+    h->chunk_key = segs;
   }
   const int nch = h->nchunk;
   auto segmax = [&](const double* v, int ncol, double* out) {
@@ -2252,19 +2287,20 @@ int64_t refine_impl(spkhip_t h, const double* d_bm, double* d_xo, int nrhs, doub
   double eta_max = 0;
   for (;;) {
     // correction d = A^-1 src in precision T; x += d
-    hipLaunchKernelGGL((spk::permute_kernel<double, T>), dim3(grid_for(n)), dim3(kWG), 0, st, src,
-                       n, rhs.p, n, h->d_perm.p, n, nrhs);
-    solve_levels<T>(h, rhs.p, n, nrhs, true, true);
-    hipLaunchKernelGGL((spk::permute_kernel<T, T>), dim3(grid_for(n)), dim3(kWG), 0, st, rhs.p, n,
-                       tbuf, n, h->d_invp.p, n, nrhs);
+    hipLaunchKernelGGL((spk::permute_kernel<double, T>), dim3(grid_for(n1), nvs), dim3(kWG), 0,
+                       st, src, n, rhs.p, n1, h->d_perm.p, n1, nrhs, n1, ns);
+    solve_levels<T>(h, rhs.p, n1, nrhs, true, true);
+    hipLaunchKernelGGL((spk::permute_kernel<T, T>), dim3(grid_for(n1), nvs), dim3(kWG), 0, st,
+                       rhs.p, n1, tbuf, n, h->d_invp.p, n1, nrhs, ns, n1);
     hipLaunchKernelGGL(spk::axpy_kernel<T>, dim3(grid_for(n)), dim3(kWG), 0, st, d_xo, n, tbuf, n,
                        n, nrhs);
     // r = b - A x (double), eta per column
     {
       Prof pr(h, SPKHIP_K_RESIDUAL, 2.0 * h->nnz * nrhs,
               (double)h->nnz * 12.0 + (double)n * 24.0 * nrhs);
-      hipLaunchKernelGGL(spk::residual_kernel, dim3(grid_for(n)), dim3(kWG), 0, st, h->d_rowptr.p,
-                         h->d_colind.p, h->d_csrval.p, d_xo, n, d_bm, n, h->d_r.p, n, n, nrhs);
+      hipLaunchKernelGGL(spk::residual_kernel, dim3(grid_for(n1), nvs), dim3(kWG), 0, st,
+                         h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, d_xo, n, d_bm, n, h->d_r.p,
+                         n, n1, nrhs, h->nnz);
     }
     segmax(h->d_r.p, nrhs, nr_r);
     segmax(d_xo, nrhs, nr_x);
@@ -2417,7 +2453,7 @@ extern "C" {
 
 int64_t spkhip_condest(spkhip_t h, double* cond1, double* ainv_norm1) {
   if (guard(h)) return -4;
-  if (h->kind != SPKHIP_KIND_SPD) return -4;
+  if (h->kind != SPKHIP_KIND_SPD || h->nvs != 1) return -4;  // one value set only
   if (!h->factored) return -3;
   return wrap([&]() -> int64_t {
     hipck(hipSetDevice(h->device));
@@ -2442,10 +2478,10 @@ int64_t spkhip_solve_refine_device(spkhip_t h, int64_t nrhs, double* d_b, int64_
                                    int64_t max_iter) {
   if (guard(h)) return -4;
   if (!h->factored) return -3;
-  if (nrhs < 0 || ldb < h->n || max_iter < 0) return -4;
+  if (nrhs < 0 || ldb < h->n * h->nvs || max_iter < 0) return -4;
   if (nrhs == 0) return 0;
   return wrap([&]() -> int64_t {
-    const i64 n = h->n;
+    const i64 n = h->n * h->nvs;  // every value set's rows
     DevBuf<double>& bm = h->d_bm;
     bm.ensure((size_t)n * nrhs);
     h->d_x.ensure((size_t)n * nrhs);
@@ -2474,10 +2510,10 @@ int64_t spkhip_solve_refine(spkhip_t h, int64_t nrhs, double* b, int64_t ldb, do
                             int64_t max_iter) {
   if (guard(h)) return -4;
   if (!h->factored) return -3;
-  if (nrhs < 0 || ldb < h->n || max_iter < 0) return -4;
+  if (nrhs < 0 || ldb < h->n * h->nvs || max_iter < 0) return -4;
   if (nrhs == 0) return 0;
   return wrap([&]() -> int64_t {
-    const i64 n = h->n;
+    const i64 n = h->n * h->nvs;  // every value set's rows
     h->d_b.ensure((size_t)n * nrhs);
     for (i64 q = 0; q < nrhs; ++q)
       hipck(hipMemcpyAsync(h->d_b.p + q * n, b + q * ldb, n * sizeof(double),
@@ -2581,6 +2617,20 @@ int64_t spkhip_set_segments(spkhip_t h, int64_t nseg, const int64_t* offsets) {
   for (int64_t i = 0; i < nseg; ++i)
     if (offsets[i + 1] < offsets[i]) return -4;
   h->seg.assign(offsets, offsets + nseg + 1);
+  return 0;
+}
+
+int64_t spkhip_set_value_sets(spkhip_t h, int64_t nvs) {
+  if (guard(h)) return -4;
+  if (!h->have_sym) return -3;
+  if (nvs < 1 || nvs > 65535) return -4;
+  if (nvs == h->nvs) return 0;
+  h->nvs = (int)nvs;
+  // the values and the factor held are those of the old number of sets
+  h->have_values = false;
+  h->factored = false;
+  h->prepped = false;
+  h->inv_prepped = false;
   return 0;
 }
 

@@ -57,6 +57,10 @@ struct DevSym {
   int pivot;              // 1: pivot inside diagonal blocks (LU); 0: none
   int sym;                // 1: U entries are read from the transposed lnz slot
                           //    (SPD input: lower triangle only)
+  // value sets: one pattern, several sets of values, set v = blockIdx.y of
+  // every launch; element strides between consecutive sets' data (ipvt: n;
+  // right-hand sides: ldr * nrhs; solve vectors: vs_vec per right-hand side)
+  int64_t vs_lnz, vs_unz, vs_arena, vs_prep, vs_prepi, vs_inv, vs_vec;
 };
 
 // Per-launch descriptor of the blocked large-front kernels (one per front

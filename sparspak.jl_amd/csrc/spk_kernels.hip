@@ -72,11 +72,16 @@ __global__ void fill_kernel(T* __restrict__ a, int64_t len, T v) {
   for (; i < len; i += stride) a[i] = v;
 }
 
+// value set blockIdx.y: its nnz values, its lnz / unz at vs_l / vs_u apart
 template <typename T>
 __global__ void scatter_a_kernel(const double* __restrict__ nzval,
                                  const int64_t* __restrict__ amap,
                                  int64_t nnz, T* __restrict__ lnz,
-                                 T* __restrict__ unz) {
+                                 T* __restrict__ unz, int64_t vs_l, int64_t vs_u) {
+  const int64_t vs = blockIdx.y;
+  nzval += vs * nnz;
+  lnz += vs * vs_l;
+  unz += vs * vs_u;
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnz) return;
   int64_t d = amap[k];
@@ -93,7 +98,11 @@ template <typename T>
 __global__ void scatter_a2_kernel(const double* __restrict__ nzval,
                                   const int64_t* __restrict__ amap2,
                                   int64_t nnz, T* __restrict__ lnz,
-                                  T* __restrict__ unz) {
+                                  T* __restrict__ unz, int64_t vs_l, int64_t vs_u) {
+  const int64_t vs = blockIdx.y;
+  nzval += vs * nnz;
+  lnz += vs * vs_l;
+  unz += vs * vs_u;
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnz) return;
   int64_t d = amap2[k];
@@ -183,6 +192,11 @@ small_front_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw,
                    T* __restrict__ lnz, T* __restrict__ unz,
                    int64_t* __restrict__ ipvt, T* __restrict__ arena,
                    unsigned long long* __restrict__ first_fail) {
+  const int64_t vs = blockIdx.y;  // value set
+  lnz += vs * s.vs_lnz;
+  unz += vs * s.vs_unz;
+  ipvt += vs * s.n;
+  arena += vs * s.vs_arena;
   extern __shared__ unsigned char smem_raw[];
   __shared__ int s_piv_all[kWG / G][kSmallLmax];  // a piece of a small front has <= L columns
   __shared__ int s_zero_all[kWG / G];
@@ -306,6 +320,10 @@ __global__ void __launch_bounds__(kWG)
 large_utile_kernel(DevSym s, const int32_t* __restrict__ it_front,
                    const int32_t* __restrict__ it_piece, const int32_t* __restrict__ it_r0,
                    T* __restrict__ lnz, T* __restrict__ unz, T* __restrict__ arena) {
+  const int64_t vs = blockIdx.y;  // value set
+  lnz += vs * s.vs_lnz;
+  unz += vs * s.vs_unz;
+  arena += vs * s.vs_arena;
   __shared__ T tile[64 * 97];
   const int f = it_front[blockIdx.x];
   const int p = it_piece[blockIdx.x];
@@ -362,6 +380,9 @@ large_init_kernel(DevSym s, const int32_t* __restrict__ it_front,
                   const int32_t* __restrict__ ea_child,
                   const int32_t* __restrict__ ea_t, const T* __restrict__ lnz,
                   T* __restrict__ arena, int staged) {
+  const int64_t vs = blockIdx.y;  // value set
+  lnz += vs * s.vs_lnz;
+  arena += vs * s.vs_arena;
   extern __shared__ unsigned char smem_raw[];
   const int item = blockIdx.x;
   const int f = it_front[item];
@@ -480,6 +501,9 @@ large_init_wave_kernel(DevSym s, const int32_t* __restrict__ it_front,
                        const int32_t* __restrict__ ea_child,
                        const int32_t* __restrict__ ea_t, const T* __restrict__ lnz,
                        T* __restrict__ arena, int nitems, int maxL) {
+  const int64_t vs = blockIdx.y;  // value set
+  lnz += vs * s.vs_lnz;
+  arena += vs * s.vs_arena;
   extern __shared__ unsigned char smem_raw[];
   const int lane = threadIdx.x % 64, wave = wave_id();
   const int item = blockIdx.x * 4 + wave;
@@ -543,6 +567,10 @@ large_writeback_kernel(DevSym s, const int32_t* __restrict__ it_front,
                        const int32_t* __restrict__ it_col,
                        T* __restrict__ lnz, T* __restrict__ unz,
                        const T* __restrict__ arena) {
+  const int64_t vs = blockIdx.y;  // value set
+  lnz += vs * s.vs_lnz;
+  unz += vs * s.vs_unz;
+  arena += vs * s.vs_arena;
   const int item = blockIdx.x;
   const int f = it_front[item];
   const int j = it_col[item];
@@ -1023,6 +1051,9 @@ template <typename T, int NQ>
 __global__ void __launch_bounds__(kWG)
 blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ arena,
                 int64_t* __restrict__ ipvt, unsigned long long* __restrict__ first_fail) {
+  const int64_t vs = blockIdx.y;  // value set
+  arena += vs * s.vs_arena;
+  ipvt += vs * s.n;
   constexpr int LD = kRLD;
   extern __shared__ unsigned char smem_raw[];
   __shared__ int piv[NQ];
@@ -1353,6 +1384,9 @@ template <typename T>
 __global__ void __launch_bounds__(kWG)
 blk_diag_slab_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ arena,
                      int64_t* __restrict__ ipvt, unsigned long long* __restrict__ first_fail) {
+  const int64_t vs = blockIdx.y;  // value set
+  arena += vs * s.vs_arena;
+  ipvt += vs * s.n;
   __shared__ __attribute__((aligned(16))) unsigned char lds[kSlabLds<T>];
   blk_diag_slab_body<T>(s, desc[blockIdx.x], arena, ipvt, first_fail, lds);
 }
@@ -1365,6 +1399,9 @@ template <typename T, int NQ>
 __global__ void __launch_bounds__(kWG)
 blk_offdiag_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
                    T* __restrict__ arena, const int64_t* __restrict__ ipvt) {
+  const int64_t vs = blockIdx.y;  // value set
+  arena += vs * s.vs_arena;
+  ipvt += vs * s.n;
   extern __shared__ unsigned char smem_raw[];
   __shared__ int spiv[NQ];
   T* S = reinterpret_cast<T*>(smem_raw);
@@ -1809,6 +1846,8 @@ template <typename T>
 __global__ void __launch_bounds__(kWG, 2)
 blk_gemm_kernel(DevSym s, const GemmDesc* __restrict__ desc, int ndesc,
                 T* __restrict__ arena, int lowp) {
+  const int64_t vs = blockIdx.y;  // value set
+  arena += vs * s.vs_arena;
   __shared__ __attribute__((aligned(16))) unsigned char lds_raw[kGemmLds<T>];
   blk_gemm_body<T>(s, desc, ndesc, arena, lowp, blockIdx.x, lds_raw);
 }
@@ -1824,6 +1863,9 @@ blk_diag_border_kernel(DevSym s, const BlkDesc* __restrict__ ddesc, int ndiag,
                        const GemmDesc* __restrict__ gdesc, int ngemm, T* __restrict__ arena,
                        int64_t* __restrict__ ipvt, unsigned long long* __restrict__ first_fail,
                        int lowp) {
+  const int64_t vs = blockIdx.y;  // value set
+  arena += vs * s.vs_arena;
+  ipvt += vs * s.n;
   constexpr int kL = kSlabLds<T> > kGemmLds<T> ? kSlabLds<T> : kGemmLds<T>;
   __shared__ __attribute__((aligned(16))) unsigned char lds[kL];
   if ((int)blockIdx.x < ndiag)
@@ -1933,6 +1975,11 @@ fwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw, i
                  const T* __restrict__ lnz, const int64_t* __restrict__ ipvt,
                  T* __restrict__ rhs, int64_t ldr, int nrhs, T* __restrict__ vec, int maxL,
                  int maxq) {
+  const int64_t vs = blockIdx.y;  // value set
+  lnz += vs * s.vs_lnz;
+  ipvt += vs * s.n;
+  rhs += vs * ldr * nrhs;
+  vec += vs * s.vs_vec * nrhs;
   extern __shared__ unsigned char smem_raw[];
   __shared__ int spiv_all[4][MAXW];
   const int lane = threadIdx.x % 64, wave = wave_id();
@@ -2015,6 +2062,10 @@ __global__ void __launch_bounds__(kWG)
 bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw, int qb,
                  const T* __restrict__ lnz, const T* __restrict__ unz, T* __restrict__ rhs,
                  int64_t ldr, int nrhs, int maxL, int maxq) {
+  const int64_t vs = blockIdx.y;  // value set
+  lnz += vs * s.vs_lnz;
+  unz += vs * s.vs_unz;
+  rhs += vs * ldr * nrhs;
   extern __shared__ unsigned char smem_raw[];
   const int lane = threadIdx.x % 64, wave = wave_id();
   const int item = blockIdx.x * fpw + wave;
@@ -2089,6 +2140,9 @@ fwd_assemble_items_kernel(DevSym s, const int32_t* __restrict__ it_front,
                           const int32_t* __restrict__ ea_child,
                           const int32_t* __restrict__ ea_t, const T* __restrict__ rhs,
                           int64_t ldr, int nrhs, T* __restrict__ vec, int nitems) {
+  const int64_t vs = blockIdx.y;  // value set
+  rhs += vs * ldr * nrhs;
+  vec += vs * s.vs_vec * nrhs;
   const int item = blockIdx.x * kWG + threadIdx.x;
   if (item >= nitems) return;
   const int f = it_front[item], r = it_col[item];
@@ -2113,6 +2167,9 @@ __global__ void __launch_bounds__(kWG)
 bwd_gather_items_kernel(DevSym s, const int32_t* __restrict__ it_front,
                         const int32_t* __restrict__ it_col, const T* __restrict__ rhs,
                         int64_t ldr, int nrhs, T* __restrict__ vec, int nitems) {
+  const int64_t vs = blockIdx.y;  // value set
+  rhs += vs * ldr * nrhs;
+  vec += vs * s.vs_vec * nrhs;
   const int item = blockIdx.x * kWG + threadIdx.x;
   if (item >= nitems) return;
   const int f = it_front[item], j = it_col[item];
@@ -2165,6 +2222,12 @@ __global__ void __launch_bounds__(kWG)
 solve_prep_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ lnz,
                   const T* __restrict__ unz, const int64_t* __restrict__ ipvt,
                   T* __restrict__ prep, int32_t* __restrict__ prepi) {
+  const int64_t vs = blockIdx.y;  // value set
+  lnz += vs * s.vs_lnz;
+  unz += vs * s.vs_unz;
+  ipvt += vs * s.n;
+  prep += vs * s.vs_prep;
+  prepi += vs * s.vs_prepi;
   __shared__ int spiv[128];
   __shared__ int sperm[128];
   __shared__ int pstart[128];
@@ -2405,6 +2468,11 @@ __global__ void __launch_bounds__(kWG)
 fwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ prep,
                  const int32_t* __restrict__ prepi, T* __restrict__ rhs, int64_t ldr, int nrhs,
                  T* __restrict__ vec) {
+  const int64_t vs = blockIdx.y;  // value set
+  prep += vs * s.vs_prep;
+  prepi += vs * s.vs_prepi;
+  rhs += vs * ldr * nrhs;
+  vec += vs * s.vs_vec * nrhs;
   extern __shared__ __attribute__((aligned(32))) unsigned char smem_a32[];
   fwd_chain_body<T>(s, desc[blockIdx.x], prep, prepi, rhs, ldr, nrhs, vec, smem_a32);
 }
@@ -2458,6 +2526,9 @@ template <typename T>
 __global__ void __launch_bounds__(kWG)
 fwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ lnz,
                  int nrhs, T* __restrict__ vec) {
+  const int64_t vs = blockIdx.y;  // value set
+  lnz += vs * s.vs_lnz;
+  vec += vs * s.vs_vec * nrhs;
   fwd_bgemv_body<T>(s, desc[blockIdx.x], lnz, nrhs, vec);
 }
 
@@ -2528,6 +2599,9 @@ template <typename T>
 __global__ void __launch_bounds__(kWG)
 bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ unz,
                  int nrhs, T* __restrict__ vec) {
+  const int64_t vs = blockIdx.y;  // value set
+  unz += vs * s.vs_unz;
+  vec += vs * s.vs_vec * nrhs;
   bwd_bgemv_body<T>(s, desc[blockIdx.x], unz, nrhs, vec);
 }
 
@@ -2686,6 +2760,10 @@ template <typename T>
 __global__ void __launch_bounds__(kWG)
 bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ prep,
                  T* __restrict__ rhs, int64_t ldr, int nrhs, T* __restrict__ vec) {
+  const int64_t vs = blockIdx.y;  // value set
+  prep += vs * s.vs_prep;
+  rhs += vs * ldr * nrhs;
+  vec += vs * s.vs_vec * nrhs;
   extern __shared__ __attribute__((aligned(32))) unsigned char smem_a32[];
   bwd_chain_body<T>(s, desc[blockIdx.x], prep, rhs, ldr, nrhs, vec, smem_a32);
 }
@@ -2707,7 +2785,12 @@ bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
 constexpr int kInvPrepLds = (128 * 128 + kPanelElems) * 4;  // float
 __global__ void __launch_bounds__(kWG)
 solve_inv_prep_kernel(const SolveDesc* __restrict__ desc, const float* __restrict__ prep,
-                      const int32_t* __restrict__ prepi, float* __restrict__ inv) {
+                      const int32_t* __restrict__ prepi, float* __restrict__ inv,
+                      int64_t vs_prep, int64_t vs_prepi, int64_t vs_inv) {
+  const int64_t vs = blockIdx.y;  // value set
+  prep += vs * vs_prep;
+  prepi += vs * vs_prepi;
+  inv += vs * vs_inv;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_inv[];
   float* X = reinterpret_cast<float*>(smem_inv);  // X[r * 128 + j]
   float* P = X + 128 * 128;                       // the panels of one factor
@@ -2804,6 +2887,11 @@ __global__ void __launch_bounds__(kWG)
 fwd_inv_kernel(DevSym s, const InvItem* __restrict__ items, const float* __restrict__ inv,
                const float* __restrict__ lnz, float* __restrict__ rhs, int64_t ldr, int nrhs,
                float* __restrict__ vec) {
+  const int64_t vs = blockIdx.y;  // value set
+  inv += vs * s.vs_inv;
+  lnz += vs * s.vs_lnz;
+  rhs += vs * ldr * nrhs;
+  vec += vs * s.vs_vec * nrhs;
   __shared__ float yb[128];
   __shared__ float zp[2][128];
   __shared__ float zs[128];
@@ -2864,6 +2952,11 @@ __global__ void __launch_bounds__(kWG)
 bwd_inv_kernel(DevSym s, const InvItem* __restrict__ items, const float* __restrict__ inv,
                const float* __restrict__ unz, float* __restrict__ rhs, int64_t ldr, int nrhs,
                float* __restrict__ vec) {
+  const int64_t vs = blockIdx.y;  // value set
+  inv += vs * s.vs_inv;
+  unz += vs * s.vs_unz;
+  rhs += vs * ldr * nrhs;
+  vec += vs * s.vs_vec * nrhs;
   constexpr int QB = 4;   // right-hand sides sharing one read of the row
   constexpr int CH = 16;  // columns per thread and pass
   __shared__ float yb[128];
@@ -2954,25 +3047,35 @@ bwd_inv_kernel(DevSym s, const InvItem* __restrict__ items, const float* __restr
 // pb[i] = b[perm[i]-1] (rhs = b[rperm], SpkSparseBase.jl:407) and
 // x[j] = pb[invp[j]-1] (solution = rhs[rinvp], :413).
 // ---------------------------------------------------------------------------
+// value set blockIdx.y: in and out start ivs and ovs elements further
 template <typename TI, typename TO>
 __global__ void permute_kernel(const TI* __restrict__ in, int64_t ldi,
                                TO* __restrict__ out, int64_t ldo,
                                const int64_t* __restrict__ idx, int64_t n,
-                               int nrhs) {
+                               int nrhs, int64_t ivs, int64_t ovs) {
+  in += (int64_t)blockIdx.y * ivs;
+  out += (int64_t)blockIdx.y * ovs;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int64_t src = idx[i] - 1;
   for (int q = 0; q < nrhs; ++q) out[i + q * ldo] = (TO)in[src + q * ldi];
 }
 
-// r = b - A x with A in CSR (original numbering), one row per thread.
+// r = b - A x with A in CSR (original numbering), one row per thread. Value
+// set blockIdx.y: its values nnz further, its rows of x, b and r n further
+// (the sets' systems stacked as one block-diagonal system).
 __global__ void residual_kernel(const int64_t* __restrict__ rowptr,
                                 const int32_t* __restrict__ colind,
                                 const double* __restrict__ val,
                                 const double* __restrict__ x, int64_t ldx,
                                 const double* __restrict__ b, int64_t ldb,
                                 double* __restrict__ r, int64_t ldr, int64_t n,
-                                int nrhs) {
+                                int nrhs, int64_t nnz) {
+  const int64_t vs = blockIdx.y;
+  val += vs * nnz;
+  x += vs * n;
+  b += vs * n;
+  r += vs * n;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int q = 0; q < nrhs; ++q) {
@@ -2990,10 +3093,13 @@ __device__ inline double abs_nan_inf(double x) {
   return (a != a) ? __builtin_huge_val() : a;
 }
 
-// Row sums |A| for ||A||_inf (a NaN entry makes its row sum +inf).
+// Row sums |A| for ||A||_inf (a NaN entry makes its row sum +inf); value set
+// blockIdx.y as in residual_kernel.
 __global__ void rowabs_kernel(const int64_t* __restrict__ rowptr,
                               const double* __restrict__ val,
-                              double* __restrict__ out, int64_t n) {
+                              double* __restrict__ out, int64_t n, int64_t nnz) {
+  val += (int64_t)blockIdx.y * nnz;
+  out += (int64_t)blockIdx.y * n;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double acc = 0.0;

@@ -88,6 +88,7 @@ _SIGS = {
     "spkhip_solve_refine_device": (i64, [vp, i64, vp, i64, ctypes.c_double,
                                          i64]),
     "spkhip_set_segments": (i64, [vp, i64, i64p]),
+    "spkhip_set_value_sets": (i64, [vp, i64]),
     "spkhip_condest": (i64, [vp, dp, dp]),
     "spkhip_set_caller_stream": (i64, [vp, vp]),
     "spkhip_download": (i64, [vp, dp, dp, i64p]),

@@ -6,10 +6,15 @@ per-system symbolic factors are concatenated into one block-diagonal symbolic
 factor: offsets are added to every index and pointer array. The device then
 sees one forest with one root per system and factors every system's fronts
 of the same elimination-tree level in the same launches — thousands of
-uneven systems, or many value sets sharing one pattern (consecutive systems
-with an identical pattern share one host symbolic factor), without a launch
-per system. The refinement certifies the backward error per system. Results per system are identical to factoring each system
-alone with the same kernels (no arithmetic crosses systems).
+uneven systems without a launch per system. Consecutive systems with an
+identical pattern share one host symbolic factor.
+
+When every system has the same pattern (many value sets of one matrix), the
+device holds that one symbolic factor and its plan once, and every launch
+covers all value sets (spkhip_set_value_sets): the schedule, index maps and
+solve plan are not replicated per system. The refinement certifies the
+backward error per system. Results per system are identical to factoring
+each system alone with the same kernels (no arithmetic crosses systems).
 """
 import ctypes
 
@@ -28,9 +33,12 @@ class BatchedSolver:
     matrices: list of scipy sparse matrices (square). Systems whose pattern
     is identical to the previous one reuse its symbolic factor.
     kind: "lu" (SparseSolver) or "spd" (SparseSpdSolver).
+    value_sets: None (default) takes the value-set mode when there are
+    several systems and all share one pattern; False always concatenates the
+    symbolic factors; True requires one shared pattern.
     """
 
-    def __init__(self, matrices, kind="lu", device=0):
+    def __init__(self, matrices, kind="lu", device=0, value_sets=None):
         self.kind = kind
         self.device = device
         self.systems = []
@@ -46,6 +54,10 @@ class BatchedSolver:
                 sym.symbolicfactor()
             self.systems.append((n, cp, rv, nz, sym))
             prev = (n, cp, rv, sym)
+        one = all(s[4] is self.systems[0][4] for s in self.systems)
+        if value_sets and not one:
+            raise ValueError("value sets need one pattern shared by every system")
+        self.value_sets = len(self.systems) > 1 and one if value_sets is None else bool(value_sets)
         self._concat()
         self._dev = None
         self._uploaded = False
@@ -104,6 +116,20 @@ class BatchedSolver:
             self._dev = _Device(self.device)
         L = self._dev.L
         kind = 0 if self.kind == "lu" else 1
+        if self.value_sets:
+            n, cp, rv, _, s = self.systems[0]
+            xunz = s.xunz if s.xunz is not None else np.zeros(n + 1, dtype=np.int64)
+            rc = L.spkhip_symbolic_upload(self._dev.h, kind, n, s.nsuper, p64(s.xsuper),
+                                          p64(s.snode), p64(s.xlindx), p64(s.lindx),
+                                          p64(s.xlnz), p64(xunz), p64(s.order.rperm),
+                                          p64(s.order.rinvp), p64(cp), p64(rv))
+            if rc == -1:
+                raise RuntimeError("No space for matrix element.")
+            check(rc, "spkhip_symbolic_upload")
+            check(L.spkhip_set_value_sets(self._dev.h, len(self.systems)),
+                  "spkhip_set_value_sets")
+            self._uploaded = True
+            return
         rc = L.spkhip_symbolic_upload(self._dev.h, kind, self.n, self.nsuper, p64(self.xsuper),
                                       p64(self.snode), p64(self.xlindx), p64(self.lindx),
                                       p64(self.xlnz), p64(self.xunz), p64(self.perm),
