@@ -356,9 +356,9 @@ int64_t spkhip_profile_get(spkhip_t h, int64_t kclass, spkhip_kstat_t* out);
  *                   rows are factored whole in LDS, larger ones block by
  *                   block on the matrix cores: the small-versus-MFMA split
  *                   (0..64, default 64)
- *   block_cols      widest block of consecutive pieces of an LU front, the
+ *   block_cols      widest block of consecutive pieces of a front, the
  *                   tile width of the diagonal, off-diagonal and update
- *                   kernels (32..128, default 64; SPD blocks are one piece)
+ *                   kernels (32..128, default 64; SPD blocks at most 96)
  *   upd_group       blocks per wide trailing update, i.e. its depth in
  *                   blocks (1..64, default 8)
  *   small_wave_min  small fronts in a level from which one wave (instead of

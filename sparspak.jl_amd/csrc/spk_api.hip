@@ -635,7 +635,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
       }
     lp.wb_cnt = (int)h->wb_front.size() - lp.wb_off;
     // blocks of consecutive pieces, at most kGW columns each (a piece wider
-    // than that forms its own block); SPD keeps one piece per block
+    // than that forms its own block)
     struct Blk {
       i32 p0, np, c0, c1;
     };
@@ -659,9 +659,11 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
     std::vector<std::vector<Blk>> fblocks, sblocks;
     size_t maxg = 0, maxgs = 0;
     const bool lub = h->kind == SPKHIP_KIND_LU;
+    // (SPD factor blocks at most 96 columns: the off-diagonal kernel's strip
+    // of 96-column pieces then fits its LDS in double)
     for (i32 f : large) {
-      fblocks.push_back(partition(f, lub ? h->block_cols : 0));
-      sblocks.push_back(partition(f, lub ? kGW : 0));
+      fblocks.push_back(partition(f, lub ? h->block_cols : std::min(h->block_cols, 96)));
+      sblocks.push_back(partition(f, kGW));
       maxg = std::max(maxg, fblocks.back().size());
       maxgs = std::max(maxgs, sblocks.back().size());
     }
@@ -726,12 +728,33 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
       auto rect = [&](i32 f, i32 k0, i32 k1, i32 r0, i32 r1, i32 j0, i32 j1, int dest) {
         if (r1 <= r0 || j1 <= j0 || k1 <= k0) return;
         LevelPlan::Launch& Lc = dest == 1 ? G.gemm2 : dest == 2 ? carryL : G.gemm;
+        auto& out = dest == 1 ? g2desc : dest == 2 ? carry : gdesc;
+        const int big = h->fW[f] > 256 ? 1 : 0;
+        const double kk = k1 - k0;
+        if (!lub) {
+          // symmetric (SPD) fronts stay symmetric: of A22 -= L21 U12 only
+          // the tiles on or below the diagonal are computed, one descriptor
+          // per 128-column tile column with rows from that column's start;
+          // the entries below the column's own square also write their
+          // transposes. A
+          // rectangle above the diagonal is the transpose of one below it in
+          // the same launch (the plan's rectangles pair up that way).
+          if (r1 <= j0) return;
+          if (r0 != j0) throw ApiError{-4};
+          for (i32 cj = j0; cj < j1; cj += 128) {
+            const i32 ce = std::min(cj + 128, j1), ntr = (r1 - cj + 127) / 128;
+            out.push_back(spk::GemmDesc{f, k0, k1, cj, r1 - cj, cj, ce - cj, Lc.wg, ntr, big, 1});
+            Lc.wg += ntr;
+            const double m = r1 - cj, nn = ce - cj;
+            Lc.flops += 2.0 * m * nn * kk;
+            Lc.elems += 2.0 * m * nn + (m + nn) * kk + (m - nn) * nn;  // + the transposes
+          }
+          return;
+        }
         const i32 ntr = (r1 - r0 + 127) / 128, ntc = (j1 - j0 + 127) / 128;
-        (dest == 1 ? g2desc : dest == 2 ? carry : gdesc)
-            .push_back(spk::GemmDesc{f, k0, k1, r0, r1 - r0, j0, j1 - j0, Lc.wg, ntr,
-                                     h->fW[f] > 256 ? 1 : 0});
+        out.push_back(spk::GemmDesc{f, k0, k1, r0, r1 - r0, j0, j1 - j0, Lc.wg, ntr, big, 0});
         Lc.wg += ntr * ntc;
-        const double m = r1 - r0, nn = j1 - j0, kk = k1 - k0;
+        const double m = r1 - r0, nn = j1 - j0;
         Lc.flops += 2.0 * m * nn * kk;
         Lc.elems += 2.0 * m * nn + (m + nn) * kk;
       };
@@ -2297,10 +2320,22 @@ int64_t refine_impl(spkhip_t h, const double* d_bm, double* d_xo, int nrhs, doub
     // r = b - A x (double), eta per column
     {
       Prof pr(h, SPKHIP_K_RESIDUAL, 2.0 * h->nnz * nrhs,
-              (double)h->nnz * 12.0 + (double)n * 24.0 * nrhs);
-      hipLaunchKernelGGL(spk::residual_kernel, dim3(grid_for(n1), nvs), dim3(kWG), 0, st,
-                         h->d_rowptr.p, h->d_colind.p, h->d_csrval.p, d_xo, n, d_bm, n, h->d_r.p,
-                         n, n1, nrhs, h->nnz);
+              (double)h->nnz * 12.0 + (double)n1 * 24.0 * nrhs);
+      // lanes per row by the mean row length (long rows: coalesced reads)
+      const i64 avg = h->nnz / std::max<i64>(n1, 1);
+      auto res = [&](auto kern, int g) {
+        hipLaunchKernelGGL(kern, dim3(grid_for(n1 * g), nvs), dim3(kWG), 0, st, h->d_rowptr.p,
+                           h->d_colind.p, h->d_csrval.p, d_xo, n, d_bm, n, h->d_r.p, n, n1, nrhs,
+                           h->nnz);
+      };
+      if (avg >= 48)
+        res(spk::residual_kernel<16>, 16);
+      else if (avg >= 20)
+        res(spk::residual_kernel<8>, 8);
+      else if (avg >= 10)
+        res(spk::residual_kernel<4>, 4);
+      else
+        res(spk::residual_kernel<1>, 1);
     }
     segmax(h->d_r.p, nrhs, nr_r);
     segmax(d_xo, nrhs, nr_x);
@@ -2720,7 +2755,6 @@ int64_t spkhip_autotune(spkhip_t h, int64_t precision, int64_t reps, spkhip_tuni
                                           {8, 4, 16, 2},
                                           {16384, 4096, 65536}};
     for (int knob = 0; knob < 4; ++knob) {
-      if (knob == 1 && h->kind != SPKHIP_KIND_LU) continue;  // SPD blocks are one piece
       for (int64_t v : cand[knob]) {
         spkhip_tuning_t t = best;
         int64_t* field = knob == 0   ? &t.small_lmax

@@ -88,6 +88,9 @@ struct GemmDesc {
   int32_t wg_begin;  // prefix sum of workgroups over the launch
   int32_t ntr;       // row tiles
   int32_t big;       // the front has more than 256 own columns
+  int32_t mirror;    // symmetric front: entries whose row lies past their
+                     // tile's last column also write their transposes
+                     // (no tile above the diagonal is computed)
 };
 
 // Integers of a block's solve data: the composed row permutation.

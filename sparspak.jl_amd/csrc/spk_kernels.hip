@@ -1681,6 +1681,10 @@ __device__ __attribute__((always_inline)) void blk_gemm_body(
   const int bw = d.k1 - d.k0;
   const int rend = d.r0 + d.nr, cend = d.j0 + d.nc;
   const int r0 = d.r0 + (b % d.ntr) * TM, j0 = d.j0 + (b / d.ntr) * TM;
+  // symmetric fronts: an entry whose row lies past the tile's last column
+  // also stands for its transpose, which no tile computes (a tile column's
+  // own square, its first tile, is computed whole)
+  const int mend = d.mirror ? min(j0 + TM, cend) : (1 << 30);
   const int wr = (wave % 2) * 64, wc = (wave / 2) * 64;
   const int l16 = lane % 16, lk = lane / 16;
   // this thread's share of a chunk: A element (ai, ak + 2 t), B element
@@ -1835,8 +1839,11 @@ __device__ __attribute__((always_inline)) void blk_gemm_body(
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int col = j0 + wc + c * 16 + Mfma16<T>::row(lk, v);
-          if (r < rend && col < cend)
-            F[r + (int64_t)col * L] = cv[ag][c][v] - acc[a0 + ag][c][v];
+          if (r < rend && col < cend) {
+            const T nv = cv[ag][c][v] - acc[a0 + ag][c][v];
+            F[r + (int64_t)col * L] = nv;
+            if (r >= mend) F[col + (int64_t)r * L] = nv;
+          }
         }
     }
   }
@@ -3061,33 +3068,36 @@ __global__ void permute_kernel(const TI* __restrict__ in, int64_t ldi,
   for (int q = 0; q < nrhs; ++q) out[i + q * ldo] = (TO)in[src + q * ldi];
 }
 
-// r = b - A x with A in CSR (original numbering), one row per thread. Value
-// set blockIdx.y: its values nnz further, its rows of x, b and r n further
-// (the sets' systems stacked as one block-diagonal system).
-__global__ void residual_kernel(const int64_t* __restrict__ rowptr,
-                                const int32_t* __restrict__ colind,
-                                const double* __restrict__ val,
-                                const double* __restrict__ x, int64_t ldx,
-                                const double* __restrict__ b, int64_t ldb,
-                                double* __restrict__ r, int64_t ldr, int64_t n,
-                                int nrhs, int64_t nnz) {
+// r = b - A x with A in CSR (original numbering): G consecutive lanes per
+// row, lane g summing the row's entries g, g + G, ... in order, the G
+// partial sums then added pairwise (G = 1: one row per thread, the entries in
+// order). Value set blockIdx.y: its values nnz further, its rows of x, b and
+// r n further (the sets' systems stacked as one block-diagonal system).
+template <int G>
+__global__ void __launch_bounds__(kWG)
+residual_kernel(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
+                const double* __restrict__ val, const double* __restrict__ x, int64_t ldx,
+                const double* __restrict__ b, int64_t ldb, double* __restrict__ r, int64_t ldr,
+                int64_t n, int nrhs, int64_t nnz) {
   const int64_t vs = blockIdx.y;
   val += vs * nnz;
   x += vs * n;
   b += vs * n;
   r += vs * n;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int g = (int)(threadIdx.x % G);
+  if (i >= n) return;  // a row's G lanes leave together
+  const int64_t k0 = rowptr[i], k1 = rowptr[i + 1];
   for (int q = 0; q < nrhs; ++q) {
+    const double* xq = x + (int64_t)q * ldx;
     double acc = 0.0;
-    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
-      acc = fma(val[k], x[colind[k] + q * ldx], acc);
-    r[i + q * ldr] = b[i + q * ldb] - acc;
+    for (int64_t k = k0 + g; k < k1; k += G) acc = fma(val[k], xq[colind[k]], acc);
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
+    if (g == 0) r[i + q * ldr] = b[i + q * ldb] - acc;
   }
 }
 
-// |x| for a max-norm that must not lose a NaN: fmax drops a NaN operand, so
-// a NaN counts as +inf (the norm then reports the vector as non-finite).
 __device__ inline double abs_nan_inf(double x) {
   const double a = fabs(x);
   return (a != a) ? __builtin_huge_val() : a;

@@ -16,7 +16,7 @@ from sparspak_hip import (Problem, SparseSolver, SparseSpdSolver, factor, findor
                           inmatrix, solve, sparspaklu, sparspaklu_refactor, symbolicfactor,
                           triangularsolve, backslash)
 from sparspak_hip import _lib
-from sparspak_hip.gallery import convdiff3d, laplace2d, uniform
+from sparspak_hip.gallery import convdiff3d, elasticity3d, laplace2d, uniform
 
 pytestmark = pytest.mark.gpu
 
@@ -174,8 +174,9 @@ def test_gpu_fixture_solves(fixtures):
         assert np.linalg.norm(q.x - xd) / np.linalg.norm(xd) < 1e-14, case["source"]
 
 
-@pytest.mark.parametrize("A", [laplace2d(25), convdiff3d(8, velocity=(0, 0, 0))],
-                         ids=["laplace2d_25", "laplace3d_8"])
+@pytest.mark.parametrize("A", [laplace2d(25), convdiff3d(8, velocity=(0, 0, 0)),
+                               convdiff3d(18, velocity=(0, 0, 0)), elasticity3d(8)],
+                         ids=["laplace2d_25", "laplace3d_8", "laplace3d_18", "elasticity3d_8"])
 def test_gpu_spd_solver(A):
     _say("case spd")
     s = _prepared(A, cls=SparseSpdSolver)
@@ -309,3 +310,39 @@ def test_gpu_refactor_same_pattern():
     assert backward_error(A2, x2, b) < 1e-15
     with pytest.raises(RuntimeError, match="pattern"):
         sparspaklu_refactor(lu, laplace2d(5), allow_pattern_change=False)
+
+
+def _update_flops(s):
+    L, h = s.slvr._dev.L, s.slvr._dev.h
+    _lib.check(L.spkhip_profile_enable(h, 1), "profile")
+    _lib.check(L.spkhip_factor(h, 0), "factor")
+    k = _lib.KStat()
+    _lib.check(L.spkhip_profile_get(h, _lib.KCLASSES.index("large_update"), ctypes.byref(k)),
+               "profile_get")
+    _lib.check(L.spkhip_profile_enable(h, 0), "profile")
+    return k.flops
+
+
+def test_gpu_spd_update_is_half_of_lu():
+    # the symmetric factor's trailing updates compute the tiles on or below
+    # the diagonal only (SpkLDLtFactor.jl:144-146, 188, 205 update L alone):
+    # their flops stay near F_Chol = sum of squared column counts, about half
+    # the LU form's F_LU on the same matrix; the refined solve still meets
+    # the double bar
+    _say("case spd update flops: elasticity3d 14")
+    A = elasticity3d(14)
+    n = A.shape[0]
+    lu = _prepared(A, cls=SparseSolver)
+    spd = _prepared(A, cls=SparseSpdSolver)
+    f_upd = _update_flops(spd)
+    st_lu, st_spd = _lib.Stats(), _lib.Stats()
+    lu.slvr._dev.L.spkhip_stats(lu.slvr._dev.h, ctypes.byref(st_lu))
+    spd.slvr._dev.L.spkhip_stats(spd.slvr._dev.h, ctypes.byref(st_spd))
+    _say(f"  SPD update flops {f_upd:.3e}, F_Chol {st_spd.factor_flops:.3e}, "
+         f"F_LU {st_lu.factor_flops:.3e}")
+    assert f_upd < 1.15 * st_spd.factor_flops
+    assert f_upd < 0.6 * st_lu.factor_flops
+    b = uniform(9, n, -1, 1)
+    x = b.copy()
+    triangularsolve(spd, x)
+    assert backward_error(A, x, b) < 1e-15
