@@ -1236,7 +1236,7 @@ void run_assemble(spkhip_handle* h) {
 }
 
 // Right-hand sides that share one staging of a small front's panels.
-inline int small_rhs_block(int nrhs) { return std::max(1, std::min(nrhs, 4)); }
+inline int small_rhs_block(int nrhs) { return std::max(1, std::min(nrhs, spk::kSmallRhsBlock)); }
 
 // Small fronts per workgroup in the solves (one wave each): four when their
 // staging (a panel of at most maxq columns and qb solve vectors) fits in

@@ -1965,6 +1965,9 @@ __device__ void wave_upper_solve(T* yb, const T* D, int ld, int nq) {
   }
 }
 
+// Most right-hand sides the small-front solves take per staging of a panel.
+constexpr int kSmallRhsBlock = 16;
+
 // Small fronts, one wave per front (fpw <= 4 fronts per workgroup, as many
 // as the LDS holds), everything in LDS; lane loops stride 64 and a wave
 // barrier orders the steps.
@@ -2002,24 +2005,24 @@ fwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw, i
   const int64_t fc0 = s.fc0[f];
   const int sp0 = s.fsp0[f], np = s.fnp[f];
   // right-hand sides in blocks of qb (y of the block's right-hand side qq
-  // at yW + qq maxL), each piece's panel staged once per block; every
-  // right-hand side receives the same operations in the same order
+  // at Y + qq maxL), each piece's panel staged once per block; the lanes
+  // take (right-hand side, row) pairs, and every right-hand side receives
+  // the same operations in the same order as when solved alone
   for (int q0 = 0; q0 < nrhs; q0 += qb) {
     const int nq_b = min(qb, nrhs - q0);
-    for (int qq = 0; qq < nq_b; ++qq) {
-      T* yW = Y + qq * maxL;
-      const T* x = rhs + (int64_t)(q0 + qq) * ldr;
-      for (int r = lane; r < L; r += 64) yW[r] = (r < W) ? x[fc0 + r] : T(0);
+    for (int idx = lane; idx < nq_b * L; idx += 64) {
+      const int qq = idx / L, r = idx - qq * L;
+      Y[qq * maxL + r] = (r < W) ? rhs[(int64_t)(q0 + qq) * ldr + fc0 + r] : T(0);
     }
     wave_lds_sync();
     for (int ci = s.fchp[f]; ci < s.fchp[f + 1]; ++ci) {
       const int c = s.fch[ci];
-      const int Lc = s.fL[c], Wc = s.fW[c];
+      const int Lc = s.fL[c], Wc = s.fW[c], Rc = Lc - Wc;
       const int32_t* rel = s.relmap + s.frel[c];
-      for (int qq = 0; qq < nq_b; ++qq) {
-        T* yW = Y + qq * maxL;
-        const T* uc = vec + s.fvec[c] * nrhs + (int64_t)(q0 + qq) * Lc + Wc;
-        for (int t = lane; t < Lc - Wc; t += 64) yW[rel[t]] += uc[t];
+      const T* uc = vec + s.fvec[c] * nrhs + (int64_t)q0 * Lc + Wc;
+      for (int idx = lane; idx < nq_b * Rc; idx += 64) {
+        const int qq = idx / Rc, t = idx - qq * Rc;
+        Y[qq * maxL + rel[t]] += uc[(int64_t)qq * Lc + t];
       }
       wave_lds_sync();
     }
@@ -2036,25 +2039,51 @@ fwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw, i
       }
       for (int i = lane; i < nq; i += 64) spiv[i] = (int)ipvt[pc0 + i] - 1;
       wave_lds_sync();
-      for (int qq = 0; qq < nq_b; ++qq) {
-        T* yW = Y + qq * maxL;
-        wave_lower_solve<T>(yW + cq, D, maxL, spiv, nq);
-        wave_lds_sync();
-        for (int r = cq + nq + lane; r < L; r += 64) {
-          const T* dr = D + (r - cq);
-          T acc = T(0);
-          for (int kk = 0; kk < nq; ++kk) acc += dr[kk * maxL] * yW[cq + kk];
-          yW[r] -= acc;
+      // the piece's interchanges (glaswp!), one lane per right-hand side
+      if (lane < nq_b) {
+        T* yb = Y + lane * maxL + cq;
+        for (int k = 0; k < nq; ++k) {
+          const int ip = spiv[k];
+          if (ip != k) {
+            const T t = yb[k];
+            yb[k] = yb[ip];
+            yb[ip] = t;
+          }
+        }
+      }
+      wave_lds_sync();
+      // unit-lower block solve (gtrsm! 'l','l','n','u'): step k updates the
+      // entries below k of every right-hand side
+      for (int k = 0; k + 1 < nq; ++k) {
+        const int m = nq - 1 - k;
+        for (int idx = lane; idx < nq_b * m; idx += 64) {
+          const int qq = idx / m, i = k + 1 + (idx - qq * m);
+          T* yb = Y + qq * maxL + cq;
+          const T yk = yb[k];
+          if (yk != T(0)) yb[i] -= yk * D[i + k * maxL];
         }
         wave_lds_sync();
       }
+      // the rows below the block: dot products with the staged panel
+      const int nrb = L - cq - nq;
+      for (int idx = lane; idx < nq_b * nrb; idx += 64) {
+        const int qq = idx / nrb, r = cq + nq + (idx - qq * nrb);
+        T* yW = Y + qq * maxL;
+        const T* dr = D + (r - cq);
+        T acc = T(0);
+        for (int kk = 0; kk < nq; ++kk) acc += dr[kk * maxL] * yW[cq + kk];
+        yW[r] -= acc;
+      }
+      wave_lds_sync();
     }
-    for (int qq = 0; qq < nq_b; ++qq) {
-      const T* yW = Y + qq * maxL;
-      T* yg = vec + s.fvec[f] * nrhs + (int64_t)(q0 + qq) * L;
-      T* x = rhs + (int64_t)(q0 + qq) * ldr;
-      for (int r = W + lane; r < L; r += 64) yg[r] = yW[r];
-      for (int r = lane; r < W; r += 64) x[fc0 + r] = yW[r];
+    T* yg = vec + s.fvec[f] * nrhs + (int64_t)q0 * L;
+    for (int idx = lane; idx < nq_b * L; idx += 64) {
+      const int qq = idx / L, r = idx - qq * L;
+      const T v = Y[qq * maxL + r];
+      if (r < W)
+        rhs[(int64_t)(q0 + qq) * ldr + fc0 + r] = v;
+      else
+        yg[(int64_t)qq * L + r] = v;
     }
     wave_lds_sync();
   }
@@ -2074,6 +2103,7 @@ bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw, i
   unz += vs * s.vs_unz;
   rhs += vs * ldr * nrhs;
   extern __shared__ unsigned char smem_raw[];
+  __shared__ T syk_all[4][kSmallRhsBlock];  // step k's entry of each right-hand side
   const int lane = threadIdx.x % 64, wave = wave_id();
   const int item = blockIdx.x * fpw + wave;
   if (wave >= fpw || item >= cnt) return;
@@ -2087,13 +2117,15 @@ bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw, i
   const int sp0 = s.fsp0[f], np = s.fnp[f];
   const int32_t* I = s.lindx + (s.xlindx[sp0] - 1);
   // right-hand sides in blocks of qb (y of the block's right-hand side qq
-  // at yW + qq maxL), each piece's U rows staged once per block
+  // at Y + qq maxL), each piece's U rows staged once per block; lanes take
+  // (right-hand side, row) pairs, every right-hand side receiving the same
+  // operations in the same order as when solved alone
+  T* syk = syk_all[wave];
   for (int q0 = 0; q0 < nrhs; q0 += qb) {
     const int nq_b = min(qb, nrhs - q0);
-    for (int qq = 0; qq < nq_b; ++qq) {
-      T* yW = Y + qq * maxL;
-      const T* x = rhs + (int64_t)(q0 + qq) * ldr;
-      for (int r = lane; r < L; r += 64) yW[r] = x[I[r] - 1];
+    for (int idx = lane; idx < nq_b * L; idx += 64) {
+      const int qq = idx / L, r = idx - qq * L;
+      Y[qq * maxL + r] = rhs[(int64_t)(q0 + qq) * ldr + I[r] - 1];
     }
     wave_lds_sync();
     for (int p = sp0 + np - 1; p >= sp0; --p) {
@@ -2111,23 +2143,47 @@ bwd_small_kernel(DevSym s, const int32_t* __restrict__ list, int cnt, int fpw, i
         D[c + j * maxq] = (j < nq) ? lp[c + (int64_t)j * jl] : unz[s.ucol[pc0 + c] + cq + j];
       }
       wave_lds_sync();
-      for (int qq = 0; qq < nq_b; ++qq) {
-        T* yW = Y + qq * maxL;
-        for (int c = 0; c < nq; ++c) {
+      // U12 products: 16 lanes per (right-hand side, row) pair, lane g
+      // summing the columns g, g + 16, ... in order, then a fixed tree over
+      // the 16 partial sums
+      {
+        const int grp = lane / 16, gl = lane % 16;
+        for (int pr = grp; pr < nq_b * nq; pr += 4) {
+          const int qq = pr / nq, c = pr - qq * nq;
+          T* yW = Y + qq * maxL;
           T acc = T(0);
-          for (int j = cq + nq + lane; j < L; j += 64) acc += D[c + (j - cq) * maxq] * yW[j];
-          for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-          if (lane == 0) yW[cq + c] -= acc;
+          for (int j = cq + nq + gl; j < L; j += 16) acc += D[c + (j - cq) * maxq] * yW[j];
+          acc += __shfl_xor(acc, 8, 16);
+          acc += __shfl_xor(acc, 4, 16);
+          acc += __shfl_xor(acc, 2, 16);
+          acc += __shfl_xor(acc, 1, 16);
+          if (gl == 0) yW[cq + c] -= acc;
         }
+      }
+      wave_lds_sync();
+      // upper block solve (gtrsm! 'l','u','n','n'): step k divides entry k
+      // of every right-hand side by U(k, k) and updates the entries above it
+      for (int k = nq - 1; k >= 0; --k) {
+        if (lane < nq_b) syk[lane] = Y[lane * maxL + cq + k];
         wave_lds_sync();
-        wave_upper_solve<T>(yW + cq, D, maxq, nq);
+        for (int idx = lane; idx < nq_b * (k + 1); idx += 64) {
+          const int qq = idx / (k + 1), i = idx - qq * (k + 1);
+          T yk = syk[qq];
+          if (yk != T(0)) {
+            yk /= D[k + k * maxq];
+            T* yb = Y + qq * maxL + cq;
+            if (i == k)
+              yb[k] = yk;
+            else
+              yb[i] -= yk * D[i + k * maxq];
+          }
+        }
         wave_lds_sync();
       }
     }
-    for (int qq = 0; qq < nq_b; ++qq) {
-      const T* yW = Y + qq * maxL;
-      T* x = rhs + (int64_t)(q0 + qq) * ldr;
-      for (int c = lane; c < W; c += 64) x[fc0 + c] = yW[c];
+    for (int idx = lane; idx < nq_b * W; idx += 64) {
+      const int qq = idx / W, c = idx - qq * W;
+      rhs[(int64_t)(q0 + qq) * ldr + fc0 + c] = Y[qq * maxL + c];
     }
     wave_lds_sync();
   }
