@@ -1405,8 +1405,12 @@ void solve_levels_w(spkhip_handle* h, T* rhs, i64 ldr, int nrhs, bool fwd, bool 
         if (done) break;
         if (S.gemv_cnt > 0) {
           Prof pr(h, SPKHIP_K_BWD_GEMV, 2.0 * S.elems * nrhs, S.elems * sizeof(T));
-          hipLaunchKernelGGL(spk::bwd_bgemv_kernel<T>, dim3(S.gemv_cnt, h->nvs), dim3(kWG), 0, st, h->ds,
-                             h->d_sdesc.p + S.gemv_off, unz, nrhs, vecp);
+          // a workgroup per row, or four rows per workgroup when the launch
+          // would otherwise hold more than 16384 workgroups (same results)
+          const int rows = (i64)S.gemv_cnt * h->nvs > 16384 ? 4 : 1;
+          hipLaunchKernelGGL(spk::bwd_bgemv_kernel<T>, dim3((S.gemv_cnt + rows - 1) / rows, h->nvs),
+                             dim3(kWG), 0, st, h->ds, h->d_sdesc.p + S.gemv_off, S.gemv_cnt, rows,
+                             unz, nrhs, vecp);
           hipck(hipGetLastError());
         }
         Prof pr(h, SPKHIP_K_BWD, 0.0, 0.0);

@@ -2595,8 +2595,8 @@ fwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
   fwd_bgemv_body<T>(s, desc[blockIdx.x], lnz, nrhs, vec);
 }
 
-// Backward: t[c] = U[c, c1:L] x[c1:L] for row c = c0 + r0 of the block, one
-// workgroup per row, stored in y[c]. x[c1:L] is read from the front's vector:
+// Backward: t[c] = U[c, c1:L] x[c1:L] for row c = c0 + r0 of the block (a
+// workgroup takes one or a few rows in turn), stored in y[c]. x[c1:L] is read from the front's vector:
 // its update rows from bwd_gather_items_kernel, its own columns as the backward
 // chains of the later blocks left them. Every thread takes columns c1 + tid
 // + 256 m, sixteen loads in flight; per-thread sums are added over the wave,
@@ -2605,7 +2605,7 @@ template <typename T>
 __device__ __attribute__((always_inline)) void bwd_bgemv_body(
     const DevSym& s, const SolveDesc& d, const T* __restrict__ unz, int nrhs,
     T* __restrict__ vec) {
-  constexpr int QB = 8;   // right-hand sides sharing one read of the row
+  constexpr int QB = 16;  // right-hand sides sharing one read of the row
   constexpr int CH = 16;  // columns per thread and pass
   __shared__ T red[QB][4];
   const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
@@ -2658,14 +2658,100 @@ __device__ __attribute__((always_inline)) void bwd_bgemv_body(
   }
 }
 
+// The same for R rows of one block of one front at once: every load of x
+// serves the R rows. Each (row, right-hand side) sum takes bwd_bgemv_body's
+// operations in its order (a thread's columns c1 + tid + 256 m in ascending
+// m, the wave tree, the four waves in order), so the results are the same.
+template <typename T, int R>
+__device__ __attribute__((always_inline)) void bwd_bgemv_rows_body(
+    const SolveDesc* __restrict__ dr, const T* __restrict__ unz, int nrhs, T* __restrict__ vec) {
+  constexpr int QB = 8;  // right-hand sides per pass
+  constexpr int CH = 8;  // columns per thread and chunk
+  __shared__ T red[R][QB][4];
+  const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
+  const SolveDesc& d = dr[0];
+  const int L = d.L, c1 = d.c1;
+  T* const yv = vec + d.fvec * nrhs;
+  const T* u[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) u[i] = unz + dr[i].ucol;
+  for (int q0 = 0; q0 < nrhs; q0 += QB) {
+    const int nq = min(QB, nrhs - q0);
+    T acc[R][QB];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int qq = 0; qq < QB; ++qq) acc[i][qq] = T(0);
+    for (int j0 = c1 + tid; j0 < L; j0 += CH * kWG) {
+      T uv[R][CH];
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int t = 0; t < CH; ++t) {
+          const int j = j0 + kWG * t;
+          uv[i][t] = (j < L) ? u[i][j] : T(0);
+        }
+#pragma unroll
+      for (int qq = 0; qq < QB; ++qq) {
+        if (qq < nq) {
+          const T* yg = yv + (int64_t)(q0 + qq) * L;
+          T xv[CH];
+#pragma unroll
+          for (int t = 0; t < CH; ++t) {
+            const int j = j0 + kWG * t;
+            xv[t] = (j < L) ? yg[j] : T(0);
+          }
+#pragma unroll
+          for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int t = 0; t < CH; ++t) acc[i][qq] += uv[i][t] * xv[t];
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int qq = 0; qq < QB; ++qq) {
+        if (qq < nq) {
+          T a = acc[i][qq];
+          for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+          if (lane == 0) red[i][qq][wave] = a;
+        }
+      }
+    __syncthreads();
+    if (tid < R * nq) {
+      const int i = tid / nq, qq = tid - i * nq;
+      yv[(int64_t)(q0 + qq) * L + dr[i].c0 + dr[i].r0] =
+          ((red[i][qq][0] + red[i][qq][1]) + red[i][qq][2]) + red[i][qq][3];
+    }
+    __syncthreads();
+  }
+}
+
+// rows: consecutive row descriptors per workgroup (1 or 4); four rows of one
+// block of one front share their loads of x, any other group is taken one
+// row after another (the results do not depend on the grouping)
 template <typename T>
 __global__ void __launch_bounds__(kWG)
-bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restrict__ unz,
-                 int nrhs, T* __restrict__ vec) {
+bwd_bgemv_kernel(DevSym s, const SolveDesc* __restrict__ desc, int ndesc, int rows,
+                 const T* __restrict__ unz, int nrhs, T* __restrict__ vec) {
   const int64_t vs = blockIdx.y;  // value set
   unz += vs * s.vs_unz;
   vec += vs * s.vs_vec * nrhs;
-  bwd_bgemv_body<T>(s, desc[blockIdx.x], unz, nrhs, vec);
+  const int base = (int)blockIdx.x * rows;
+  if (rows == 4 && base + 3 < ndesc) {
+    const SolveDesc* dr = desc + base;
+    if (dr[1].front == dr[0].front && dr[2].front == dr[0].front && dr[3].front == dr[0].front &&
+        dr[3].c0 == dr[0].c0) {
+      bwd_bgemv_rows_body<T, 4>(dr, unz, nrhs, vec);
+      return;
+    }
+  }
+  for (int i = 0; i < rows; ++i) {
+    const int di = base + i;
+    if (di >= ndesc) break;
+    bwd_bgemv_body<T>(s, desc[di], unz, nrhs, vec);
+  }
 }
 
 // b / u from r = 1 / u (correctly rounded): q0 = b r, then one correction
