@@ -29,7 +29,19 @@ n = 206,763, z = 0 face clamped as identity rows, body load N(0, 1)), in
 fp64, fp32_bf16x3 and fp32_bf16 (key "elasticity" of the JSON line, with the
 best mixed path's factor+solve speed-up over fp64). Prints one JSON line.
 
+--batched adds the two batched configurations (key "batched"): 2 000
+independent 2D 5-point Laplacians (sides uniform in 8..96, splitmix64 seed 4;
+a diagonal shift U[0.1, 1] each, seed 40; SPD, 4 right-hand sides each) in
+fp64 and fp32 + refinement, and 256 value sets of one 32^3 convection-
+diffusion pattern (log-uniform diffusivity in [1e-2, 1e2] per cell, seeds
+5000 + k; 16 right-hand sides each) in fp32 and fp32_bf16x3 + refinement.
+Each path is autotuned, then timed over the same steps (values and
+right-hand sides resident in HBM; factor, then the solve and refinement of
+every right-hand side of every system). Not in the default run (the systems
+take about a minute to generate on the host).
+
     python bench.py [--steps K] [--warmup W] [--grid 64] [--cpu-grid 36] [--elastic 40]
+                    [--batched]
 """
 import argparse
 import ctypes
@@ -83,6 +95,8 @@ def main():
     ap.add_argument("--tuning", default=None,
                     help="small_lmax,block_cols,upd_group,small_wave_min for every path "
                          "instead of autotuning (e.g. to profile the plan a bench run chose)")
+    ap.add_argument("--batched", action="store_true",
+                    help="also run the two batched configurations (key \"batched\")")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (three times), the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -187,6 +201,8 @@ def main():
         roof["traffic"] = tr
         roof["traffic_source"] = "profiles/" + TRAFFIC_FILE
 
+    batched = run_batched(args) if args.batched else None
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_grid)
@@ -213,8 +229,92 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "elasticity": elastic,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
+
+
+def run_batched(args):
+    """The batched configurations through BatchedSolver (one device factor
+    for all systems): per path autotune, warmup, then the timed steps
+    (factor; solve of every right-hand side, refined for the float paths),
+    times from the handle's HIP events, each system's backward error
+    checked on the last step."""
+    import torch
+    from sparspak_hip.batch import BatchedSolver
+    from sparspak_hip.gallery import convdiff3d_varcoef, laplace2d, splitmix64, uniform
+
+    out = {}
+    configs = []
+    nsys = 2000
+    sides = 8 + (splitmix64(4, nsys) % np.uint64(89)).astype(int)
+    shifts = uniform(40, nsys, 0.1, 1.0)
+    configs.append(("laplace2d_uneven_2000_spd", "spd", lambda: [
+        laplace2d(int(sd), shift=float(sh)) for sd, sh in zip(sides, shifts)], 4, ["fp64", "fp32"],
+        "2 000 2D 5-point Laplacians, sides U{8..96}, shift U[0.1,1], SPD, 4 rhs each"))
+    g = 32
+    configs.append(("convdiff3d_32_value_sets_256", "lu", lambda: [
+        convdiff3d_varcoef(g, 10.0 ** uniform(5000 + k, g ** 3, -2.0, 2.0)) for k in range(256)],
+        16, ["fp32", "fp32_bf16x3"],
+        "256 value sets of one 32^3 convection-diffusion pattern, kappa log-U[1e-2,1e2], "
+        "velocity (1,0,0), 16 rhs each"))
+    for key, kind, gen, nrhs, paths, desc in configs:
+        t0 = time.time()
+        mats = gen()
+        bs = BatchedSolver(mats, kind=kind)
+        bs.set_values()
+        n = bs.n
+        rhs = np.stack([uniform(900 + k, n, -1.0, 1.0) for k in range(nrhs)], axis=1)
+        d_b = torch.from_numpy(np.ascontiguousarray(rhs.T)).cuda()  # column-major n x nrhs
+        d_x = torch.empty_like(d_b)
+        log(f"batched {key}: n={n} systems={len(mats)} value_sets={bs.value_sets} "
+            f"({time.time() - t0:.1f}s)")
+        res = {"workload": desc, "n": n, "systems": len(mats), "value_sets": bool(bs.value_sets),
+               "nrhs_per_system": nrhs, "F": bs.flops(), "paths": {}}
+        for path in paths:
+            refine = path != "fp64"
+            tuning = bs.autotune(path)
+
+            def step():
+                bs.factor(path)
+                d_x.copy_(d_b)
+                bs.solve_device(d_x, nrhs, refine=refine)
+                st = bs.stats()
+                return st.t_factor_ms, (st.t_refine_ms if refine else st.t_solve_ms), st
+
+            for _ in range(args.warmup):
+                step()
+            torch.cuda.synchronize()
+            tf, ts = [], []
+            for _ in range(args.steps):
+                f_ms, s_ms, st = step()
+                tf.append(f_ms)
+                ts.append(s_ms)
+            torch.cuda.synchronize()
+            x = d_x.cpu().numpy().T
+            worst = 0.0
+            off = 0
+            for A in mats:
+                m = A.shape[0]
+                xs, bsys = x[off:off + m], rhs[off:off + m]
+                r = bsys - A @ xs
+                an = abs(A).sum(axis=1).max()
+                eta = np.max(np.abs(r), axis=0) / (an * np.max(np.abs(xs), axis=0)
+                                                   + np.max(np.abs(bsys), axis=0))
+                worst = max(worst, float(np.max(eta)))
+                off += m
+            f_ms, s_ms = float(np.median(tf)), float(np.median(ts))
+            res["paths"][path] = {"factor_s": f_ms / 1e3, "solve_refine_s": s_ms / 1e3,
+                                  "factor_solve_s": (f_ms + s_ms) / 1e3,
+                                  "factor_tflops": bs.flops() / (f_ms / 1e3) / 1e12,
+                                  "refine_iters": int(st.refine_iters),
+                                  "worst_backward_error": worst, "tuning": tuning}
+            log(f"batched {key} {path}: factor {f_ms:.1f} ms "
+                f"({bs.flops() / f_ms / 1e9:.2f} TF/s), solve{'+ir' if refine else ''} "
+                f"{s_ms:.1f} ms, iters {st.refine_iters}, worst eta {worst:.2e}")
+        out[key] = res
+        del bs
+    return out
 
 
 def run_workload(A, b, xtrue, paths, args):
