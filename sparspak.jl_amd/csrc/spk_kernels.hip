@@ -1565,9 +1565,13 @@ __device__ void gemm_bf16_chunks(const float* __restrict__ F, int L, int k0, int
   const int wr = (wave % 2) * 64, wc = (wave / 2) * 64;
   const int l16 = lane % 16, lk = lane / 16;
   // A: rows 4 ag .. 4 ag + 3 of the tile, k = 4 aq + u; B: columns
-  // bc + 32 u, k = 4 bq .. 4 bq + 3
-  const int ag = tid % 32, aq = tid / 32;
-  const int bq = tid % 8, bc = tid / 8;
+  // bc + 32 u, k = 4 bq .. 4 bq + 3. The 32 lanes of a half-wave store rows
+  // 4 apart (A: ag = tid / 8; B: bc = 4 ((tid / 8) % 4) + ...), which at 80
+  // bytes per LDS row start 16 banks apart: their 8-byte stores hit 64
+  // distinct banks (rows 1 apart would collide every fourth row, rows 16
+  // apart every row)
+  const int ag = tid / 8, aq = tid % 8;
+  const int bq = tid % 8, bc = 4 * ((tid / 8) % 4) + (tid / 32) % 4 + 16 * (tid / 128);
   const int arow = r0 + 4 * ag;
   const int anv = rend - arow;  // valid rows of this thread's A groups
   f4u ra[4], rb[4];
