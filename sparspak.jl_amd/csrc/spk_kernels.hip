@@ -2958,42 +2958,47 @@ solve_inv_prep_kernel(const SolveDesc* __restrict__ desc, const float* __restric
   const int32_t* pi = prepi + (int64_t)d.r0 * kPrepInts;
   float* ML = inv + d.inv;
   float* MU = ML + (int64_t)bwp * bwp;
+  // Column j of each inverse is the solve of e_j: thread (j, h) = (tid %
+  // 128, tid / 128) takes column j's rows of parity h, and all threads take
+  // the eliminated column c in step, so that the panel entries are read as
+  // broadcasts. Column j's entries receive the terms of c = j, j + 1, ...
+  // (ML) or c = j, j - 1, ... (MU) in that order; the steps with c before j
+  // (ML) or after it (MU) meet a zero entry of column j and change nothing.
+  const int j = tid & 127, h = tid >> 7;
+  const bool cj = j < bw;
   for (int i = tid; i < bw; i += kWG) iperm[pi[i]] = i;
   for (int i = tid; i < npan; i += kWG) P[i] = lo[i];
-  for (int i = tid; i < 128 * 128; i += kWG) X[i] = 0.0f;
+  for (int i = tid; i < 128 * 128; i += kWG) X[i] = (i / 128 == i % 128) ? 1.0f : 0.0f;
   __syncthreads();
-  if (tid < bw) {
-    const int j = tid;
-    X[j * 128 + j] = 1.0f;
-    for (int c = j; c < bw; ++c) {
+  for (int c = 0; c + 1 < bw; ++c) {
+    if (cj && c >= j) {
       const float xc = X[c * 128 + j];
       const float* Pc = P + lo_panel(c / 4, ng) + (c % 4) - 4 * 4 * (c / 4);
-      for (int r = c + 1; r < bw; ++r) X[r * 128 + j] -= Pc[4 * r] * xc;
+      for (int r = c + 1 + h; r < bw; r += 2) X[r * 128 + j] -= Pc[4 * r] * xc;
     }
+    __syncthreads();
   }
-  __syncthreads();
   for (int idx = tid; idx < bwp * bwp; idx += kWG) {
     const int i = idx / bwp, jj = idx % bwp;
     ML[idx] = (i < bw && jj < bw) ? X[i * 128 + iperm[jj]] : 0.0f;
   }
   __syncthreads();
   for (int i = tid; i < npan; i += kWG) P[i] = up[i];
-  for (int i = tid; i < 128 * 128; i += kWG) X[i] = 0.0f;
+  for (int i = tid; i < 128 * 128; i += kWG) X[i] = (i / 128 == i % 128) ? 1.0f : 0.0f;
   __syncthreads();
-  if (tid < bw) {
-    const int j = tid;
-    X[j * 128 + j] = 1.0f;
-    for (int c = j; c >= 0; --c) {
+  for (int c = bw - 1; c >= 0; --c) {
+    if (cj && c <= j) {
       const float xc = X[c * 128 + j] * rinv[c];
-      X[c * 128 + j] = xc;
+      if (h == 0) MU[(int64_t)c * bwp + j] = xc;  // row c of column j is final
       const float* Pc = P + up_panel(c / 4) + (c % 4);
-      for (int r = 0; r < c; ++r) X[r * 128 + j] -= Pc[4 * r] * xc;
+      for (int r = h; r < c; r += 2) X[r * 128 + j] -= Pc[4 * r] * xc;
     }
+    __syncthreads();
   }
-  __syncthreads();
+  // the rest of MU: zeros below the diagonal and in the padding
   for (int idx = tid; idx < bwp * bwp; idx += kWG) {
     const int i = idx / bwp, jj = idx % bwp;
-    MU[idx] = (i < bw && jj < bw) ? X[i * 128 + jj] : 0.0f;
+    if (!(i < bw && jj < bw && i <= jj)) MU[idx] = 0.0f;
   }
 }
 
