@@ -2931,11 +2931,12 @@ bwd_chain_kernel(DevSym s, const SolveDesc* __restrict__ desc, const T* __restri
 // ---------------------------------------------------------------------------
 
 // ML and MU of one block (one workgroup per block), each bwp x bwp
-// row-major and zero outside bw x bw, from the block's panels: thread j
-// forms column j by substitution on e_j in LDS (X[r][j]), the lower inverse
-// first (its columns then permuted by the block's interchanges), then the
-// upper one.
-constexpr int kInvPrepLds = (128 * 128 + kPanelElems) * 4;  // float
+// row-major and zero outside bw x bw, from the block's panels (read from
+// global memory, every lane the same entry): column j is formed by
+// substitution on e_j in LDS (X[r][j]), the lower inverse first (its columns
+// then permuted by the block's interchanges), then the upper one. 64 KB of
+// LDS: two workgroups per CU.
+constexpr int kInvPrepLds = 128 * 128 * 4;  // float
 __global__ void __launch_bounds__(kWG)
 solve_inv_prep_kernel(const SolveDesc* __restrict__ desc, const float* __restrict__ prep,
                       const int32_t* __restrict__ prepi, float* __restrict__ inv,
@@ -2946,7 +2947,6 @@ solve_inv_prep_kernel(const SolveDesc* __restrict__ desc, const float* __restric
   inv += vs * vs_inv;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_inv[];
   float* X = reinterpret_cast<float*>(smem_inv);  // X[r * 128 + j]
-  float* P = X + 128 * 128;                       // the panels of one factor
   __shared__ int iperm[128];
   const SolveDesc d = desc[blockIdx.x];
   const int bw = d.c1 - d.c0, ng = (bw + 3) / 4, bwp = 4 * ng;
@@ -2967,13 +2967,12 @@ solve_inv_prep_kernel(const SolveDesc* __restrict__ desc, const float* __restric
   const int j = tid & 127, h = tid >> 7;
   const bool cj = j < bw;
   for (int i = tid; i < bw; i += kWG) iperm[pi[i]] = i;
-  for (int i = tid; i < npan; i += kWG) P[i] = lo[i];
   for (int i = tid; i < 128 * 128; i += kWG) X[i] = (i / 128 == i % 128) ? 1.0f : 0.0f;
   __syncthreads();
   for (int c = 0; c + 1 < bw; ++c) {
     if (cj && c >= j) {
       const float xc = X[c * 128 + j];
-      const float* Pc = P + lo_panel(c / 4, ng) + (c % 4) - 4 * 4 * (c / 4);
+      const float* Pc = lo + lo_panel(c / 4, ng) + (c % 4) - 4 * 4 * (c / 4);
       for (int r = c + 1 + h; r < bw; r += 2) X[r * 128 + j] -= Pc[4 * r] * xc;
     }
     __syncthreads();
@@ -2983,14 +2982,13 @@ solve_inv_prep_kernel(const SolveDesc* __restrict__ desc, const float* __restric
     ML[idx] = (i < bw && jj < bw) ? X[i * 128 + iperm[jj]] : 0.0f;
   }
   __syncthreads();
-  for (int i = tid; i < npan; i += kWG) P[i] = up[i];
   for (int i = tid; i < 128 * 128; i += kWG) X[i] = (i / 128 == i % 128) ? 1.0f : 0.0f;
   __syncthreads();
   for (int c = bw - 1; c >= 0; --c) {
     if (cj && c <= j) {
       const float xc = X[c * 128 + j] * rinv[c];
       if (h == 0) MU[(int64_t)c * bwp + j] = xc;  // row c of column j is final
-      const float* Pc = P + up_panel(c / 4) + (c % 4);
+      const float* Pc = up + up_panel(c / 4) + (c % 4);
       for (int r = h; r < c; r += 2) X[r * 128 + j] -= Pc[4 * r] * xc;
     }
     __syncthreads();
