@@ -72,7 +72,10 @@ __global__ void fill_kernel(T* __restrict__ a, int64_t len, T v) {
   for (; i < len; i += stride) a[i] = v;
 }
 
-// value set blockIdx.y: its nnz values, its lnz / unz at vs_l / vs_u apart
+// A's values added into their slots of the zeroed lnz / unz (amap[k] >= 0:
+// lnz slot, <= -2: unz slot -(amap[k] + 2), -1: not stored; distinct entries
+// of a CSC pattern map to distinct slots). Value set blockIdx.y: its nnz
+// values, its lnz / unz at vs_l / vs_u apart.
 template <typename T>
 __global__ void scatter_a_kernel(const double* __restrict__ nzval,
                                  const int64_t* __restrict__ amap,
@@ -85,27 +88,6 @@ __global__ void scatter_a_kernel(const double* __restrict__ nzval,
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnz) return;
   int64_t d = amap[k];
-  T v = (T)nzval[k];
-  if (d >= 0)
-    lnz[d] += v;
-  else if (d <= -2)
-    unz[-d - 2] += v;
-}
-
-// SPD input: every stored lower entry is mirrored into the U slot of its
-// transpose so that the LU front kernels see the full symmetric matrix.
-template <typename T>
-__global__ void scatter_a2_kernel(const double* __restrict__ nzval,
-                                  const int64_t* __restrict__ amap2,
-                                  int64_t nnz, T* __restrict__ lnz,
-                                  T* __restrict__ unz, int64_t vs_l, int64_t vs_u) {
-  const int64_t vs = blockIdx.y;
-  nzval += vs * nnz;
-  lnz += vs * vs_l;
-  unz += vs * vs_u;
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nnz) return;
-  int64_t d = amap2[k];
   T v = (T)nzval[k];
   if (d >= 0)
     lnz[d] += v;
