@@ -163,6 +163,22 @@ function condest(h::Handle)
     return c[], a[]
 end
 
+# Value sets: nvs matrices of the uploaded pattern (same ordering and
+# symbolic factor, other values) factored and solved in the same launches,
+# the batched form of sparspaklu!'s refactorisation (SparseCSCInterface.jl:
+# 247-268). After it, inmatrix! takes the sets' matrices (their nzval
+# concatenated) and solve! the sets' right-hand sides stacked (n * nvs rows).
+setvaluesets!(h::Handle, nvs::Integer) = check(
+    ccall((:spkhip_set_value_sets, libspk), Int64, (Ptr{Cvoid}, Int64), h.ptr, nvs),
+    "spkhip_set_value_sets")
+
+function inmatrix!(h::Handle, ms::AbstractVector)
+    nz = reduce(vcat, [m.nzval for m in ms])
+    GC.@preserve nz check(
+        ccall((:spkhip_inmatrix, libspk), Int64, (Ptr{Cvoid}, Ptr{Float64}), h.ptr, nz),
+        "spkhip_inmatrix")
+end
+
 factor!(h::Handle; precision=SPKHIP_FP64) = check(
     ccall((:spkhip_factor, libspk), Int64, (Ptr{Cvoid}, Int64), h.ptr, precision),
     "An empty problem. No matrix.")   # the message _factor! raises (SpkSparseBase.jl:387)
