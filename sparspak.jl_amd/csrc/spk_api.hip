@@ -97,6 +97,7 @@ struct LevelPlan {
     // (which must wait for the previous gemm2)
     Launch gemm2;
     bool w1 = false;
+    bool wide = false;  // a block of more than 64 columns: the LDS diagonal form
     // the previous step's update within its block group, launched with this
     // step's diagonal regions (blk_diag_border_kernel)
     Launch bin;
@@ -514,14 +515,16 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
                 std::vector<i32>& ut_piece, std::vector<i32>& ut_r0) {
   const i32 nf = h->nfront;
   {
-    // LU blocks of at most 64 columns: the diagonal region in registers
-    // (SPKHIP_DIAG_SLAB=0 keeps the LDS form), and each in-group update
-    // beside the next block's regions (SPKHIP_MERGE_BORDER=0: after them)
+    // blocks of at most 64 columns: the diagonal region in registers
+    // (SPKHIP_DIAG_SLAB=0 keeps the LDS form; an SPD step with a wider
+    // block, a piece of more than 64 columns, takes the LDS form), and, LU
+    // only, each in-group update beside the next block's regions
+    // (SPKHIP_MERGE_BORDER=0: after them)
     const char* ds = std::getenv("SPKHIP_DIAG_SLAB");
     const char* mb = std::getenv("SPKHIP_MERGE_BORDER");
-    h->slab_plan =
-        h->kind == SPKHIP_KIND_LU && h->block_cols <= 64 && !(ds && ds[0] == '0');
-    h->merge_plan = h->slab_plan && !(mb && mb[0] == '0');
+    h->slab_plan = h->block_cols <= 64 && !(ds && ds[0] == '0');
+    h->merge_plan =
+        h->slab_plan && h->kind == SPKHIP_KIND_LU && !(mb && mb[0] == '0');
   }
   h->pdesc.clear();
   h->iitems.clear();
@@ -684,6 +687,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         G.diag.wg += 1;
         const i32 bw = B.c1 - B.c0;
         G.diag.lds = std::max(G.diag.lds, bw * spk::kRLD);
+        if (bw > 64) G.wide = true;
         for (i32 p = B.p0; p < B.p0 + B.np; ++p) {
           const double q = (double)(h->xsuper[p + 1] - h->xsuper[p]);
           const double u = (double)(B.c1 - (h->xsuper[p + 1] - 1 - h->fc0[f]));
@@ -1142,7 +1146,7 @@ void run_factor(spkhip_handle* h) {
         } else {
           h->cur_grid = G.diag.wg;
           Prof pr(h, SPKHIP_K_DIAG, G.diag.flops, G.diag.elems * sizeof(T));
-          if (slab)
+          if (slab && !G.wide)
             hipLaunchKernelGGL((spk::blk_diag_slab_kernel<T>), dim3(G.diag.wg, h->nvs), dim3(kWG), 0, st,
                                h->ds, h->d_bdesc.p + G.diag.off, arena.p, h->d_ipvt.p,
                                h->d_fail.p);

@@ -1124,10 +1124,11 @@ blk_diag_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__ aren
   store_tile<T, 1, LD>(Fr, L, bw, bw, R);
 }
 
-// Diagonal region of one LU block of at most 64 columns per workgroup, held
+// Diagonal region of one block of at most 64 columns per workgroup, held
 // in registers: wave w keeps columns [16 w, 16 w + 16) of the region, one
 // row per lane. The region is factored right-looking, one column at a time,
-// with the reference's pivoting confined to each piece's rows
+// with the reference's pivoting confined to each piece's rows (LU; an SPD
+// front takes the row at position k, no interchanges)
 // (SpkLUFactor.jl:230-240, GenericBlasLapackFragments.jl:56-100). Step k,
 // a column of piece p, takes as pivot the row of p with the largest |a|
 // among those whose position is k or later (ties: the smallest position; no
@@ -1309,7 +1310,8 @@ __device__ __attribute__((always_inline)) void blk_diag_slab_body(
       __hip_atomic_store(&sdone, k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (k == plo) piece_begin(pk, k);
     const T v = a[0];
-    const int bl = pivot_lane(v, pos, pr == pk, k);
+    // without pivoting (SPD) the pivot row is the one at position k: lane k
+    const int bl = s.pivot ? pivot_lane(v, pos, pr == pk, k) : k;
     const int kp = __builtin_amdgcn_readlane(pos, bl);
     const T pv = rdlane(v, bl);
     T u[SW];
