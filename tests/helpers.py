@@ -101,3 +101,19 @@ def backward_error(A, x, b):
     r = b - A @ x
     an = abs(A).sum(axis=1).max()
     return np.max(np.abs(r), axis=0) / (an * np.max(np.abs(x), axis=0) + np.max(np.abs(b), axis=0))
+
+
+def spd_meaningful(sym):
+    """Mask of the LDL^T panel entries the factor defines: the diagonal
+    block's diagonal and lower triangle and the rows below it. The strict
+    upper triangle of a diagonal block is never read by _ldltsolve!; the
+    reference leaves partial update sums there, this library zeros."""
+    m = np.ones(sym.nlnz, dtype=bool)
+    for J in range(sym.nsuper):
+        f = sym.xsuper[J] - 1
+        w = sym.xsuper[J + 1] - sym.xsuper[J]
+        jl = sym.xlindx[J + 1] - sym.xlindx[J]
+        base = sym.xlnz[f] - 1
+        for c in range(w):
+            m[base + c * jl: base + c * jl + c] = False
+    return m

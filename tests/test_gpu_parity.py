@@ -11,7 +11,7 @@ import pytest
 import scipy.sparse as sp
 
 import oracle.oracle as orc
-from helpers import backward_error, csc1, random_pattern
+from helpers import backward_error, csc1, random_pattern, spd_meaningful
 from sparspak_hip import (Problem, SparseSolver, SparseSpdSolver, factor, findorder,
                           inmatrix, solve, sparspaklu, sparspaklu_refactor, symbolicfactor,
                           triangularsolve, backslash)
@@ -42,22 +42,6 @@ def _oracle_factor(s, A, dtype=np.float64):
 
 def _relmax(a, b):
     return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
-
-
-def _spd_meaningful(sym):
-    """Mask of the LDL^T panel entries the factor defines: the diagonal
-    block's diagonal and lower triangle and the rows below it. The strict
-    upper triangle of a diagonal block is never read by _ldltsolve!; the
-    reference leaves partial update sums there, this library zeros."""
-    m = np.ones(sym.nlnz, dtype=bool)
-    for J in range(sym.nsuper):
-        f = sym.xsuper[J] - 1
-        w = sym.xsuper[J + 1] - sym.xsuper[J]
-        jl = sym.xlindx[J + 1] - sym.xlindx[J]
-        base = sym.xlnz[f] - 1
-        for c in range(w):
-            m[base + c * jl: base + c * jl + c] = False
-    return m
 
 
 CASES = {
@@ -184,7 +168,7 @@ def test_gpu_spd_solver(A):
     n, cp, rv, nz = csc1(A)
     ol, _, _ = orc.inmatrix(s.slvr.sym, cp, rv, nz)
     orc.ldltfactor(s.slvr.sym, ol)
-    m = _spd_meaningful(s.slvr.sym)
+    m = spd_meaningful(s.slvr.sym)
     assert _relmax(lnz[m], ol[m]) < 1e-12
     assert np.all(lnz[~m] == 0.0)
     b = uniform(7, n, -1, 1)
@@ -241,7 +225,7 @@ def test_gpu_stateless_ldlt():
                                  P(sym.lindx), P(sym.xlnz), _lib.pd(g))
     assert rc == 0
     orc.ldltfactor(sym, lnz)
-    m = _spd_meaningful(sym)
+    m = spd_meaningful(sym)
     assert _relmax(g[m], lnz[m]) < 1e-12
     b = uniform(11, n, -1, 1)
     r1 = b[sym.order.rperm - 1].copy()
@@ -270,7 +254,7 @@ def test_gpu_stateless_ldlt_f32():
                                  P(sym.lindx), P(sym.xlnz), _lib.pf(g))
     assert rc == 0
     orc.ldltfactor(sym, lnz)
-    m = _spd_meaningful(sym)
+    m = spd_meaningful(sym)
     assert _relmax(g[m].astype(np.float64), lnz[m]) < 2e-5
     b = uniform(12, n, -1, 1)
     r2 = b[sym.order.rperm - 1].copy()

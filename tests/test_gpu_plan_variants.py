@@ -1,0 +1,90 @@
+"""The factor's plan variants against the oracle: the diagonal regions in
+registers or in LDS (SPKHIP_DIAG_SLAB), each in-group update merged with the
+next block's diagonal launch or launched after it (SPKHIP_MERGE_BORDER), for
+LU with pivoting confined to the supernodes (SpkLUFactor.jl:230-240) and for
+SPD (no interchanges, symmetric updates). Every variant gives the oracle's
+pivots and its entries within 1e-12 (double). The switches are read when the
+plan is built, so each variant uploads its own symbolic factor."""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle.oracle as orc
+from helpers import backward_error, csc1, random_pattern, spd_meaningful
+from sparspak_hip import (SparseSolver, SparseSpdSolver, factor, findorder, inmatrix,
+                          symbolicfactor, triangularsolve)
+from sparspak_hip.gallery import convdiff3d, elasticity3d, uniform
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = {"slab+merge": ("1", "1"), "slab": ("1", "0"), "lds": ("0", "0")}
+
+
+def _factor_with(A, cls, slab, merge):
+    old = {k: os.environ.get(k) for k in ("SPKHIP_DIAG_SLAB", "SPKHIP_MERGE_BORDER")}
+    os.environ["SPKHIP_DIAG_SLAB"] = slab
+    os.environ["SPKHIP_MERGE_BORDER"] = merge
+    try:
+        s = cls(A)
+        findorder(s)
+        symbolicfactor(s)
+        inmatrix(s)
+        factor(s, "fp64")
+        return s
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def _weak_lu(n, seed):
+    # an unsymmetric pattern with a weak diagonal: the pivot search matters
+    return sp.csc_matrix(random_pattern(n, 0.05, seed, False, False) + sp.diags(np.full(n, 0.3)))
+
+
+LU_CASES = {"convdiff3d_20": lambda: convdiff3d(20), "weakdiag_400": lambda: _weak_lu(400, 11)}
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+@pytest.mark.parametrize("name", list(LU_CASES))
+def test_lu_plan_variants_match_oracle(name, variant):
+    print(f"case plan variant LU {name} {variant}", flush=True)
+    A = sp.csc_matrix(LU_CASES[name]())
+    s = _factor_with(A, SparseSolver, *VARIANTS[variant])
+    lnz, unz, ipiv = s.slvr.download()
+    n, cp, rv, nz = csc1(A)
+    ol, ou, op, rc, _ = orc.factor_csc(s.slvr.sym, cp, rv, nz)
+    assert rc == 0
+    assert np.array_equal(ipiv, op), "pivots differ from the oracle"
+    scale = max(np.max(np.abs(ol)), np.max(np.abs(ou)))
+    assert np.max(np.abs(lnz - ol)) <= 1e-12 * scale
+    assert np.max(np.abs(unz - ou)) <= 1e-12 * scale
+    # the weak diagonal grows the factor: the oracle's own unrefined solve
+    # reaches about 7e-12 there, so the bar is relative to it
+    b = uniform(5, n, -1.0, 1.0)
+    eta_o = backward_error(A, orc.solve(s.slvr.sym, ol, ou, op, b.copy()), b)
+    x = b.copy()
+    triangularsolve(s, x)
+    assert backward_error(A, x, b) <= max(4.0 * eta_o, 1e-15)
+
+
+@pytest.mark.parametrize("variant", ["slab", "lds"])
+def test_spd_plan_variants_match_oracle(variant):
+    print(f"case plan variant SPD elasticity3d 8 {variant}", flush=True)
+    A = sp.csc_matrix(elasticity3d(8))
+    s = _factor_with(A, SparseSpdSolver, *VARIANTS[variant])
+    lnz = s.slvr.download()[0]
+    n, cp, rv, nz = csc1(A)
+    ol, _, _ = orc.inmatrix(s.slvr.sym, cp, rv, nz)
+    orc.ldltfactor(s.slvr.sym, ol)
+    m = spd_meaningful(s.slvr.sym)
+    assert np.max(np.abs(lnz[m] - ol[m])) <= 1e-12 * np.max(np.abs(ol[m]))
+    assert np.all(lnz[~m] == 0.0)
+    b = uniform(6, n, -1.0, 1.0)
+    x = b.copy()
+    triangularsolve(s, x)
+    assert backward_error(A, x, b) < 1e-14
