@@ -660,6 +660,14 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
     // solve data is built from lnz/unz, so its blocks need not be the
     // factor's, and fewer blocks mean fewer dependent solve steps)
     std::vector<std::vector<Blk>> fblocks, sblocks;
+    // the front's fields and the piece widths in a block's descriptor
+    auto front_fields = [&](BlkDesc& D, i32 f, const Blk& B) {
+      D.L = h->fL[f];
+      D.fc0 = h->fc0[f];
+      D.foff = h->foff[f];
+      for (i32 q = 0; q < B.np; ++q)
+        D.pw[q] = (i32)(h->xsuper[B.p0 + q + 1] - h->xsuper[B.p0 + q]);
+    };
     size_t maxg = 0, maxgs = 0;
     const bool lub = h->kind == SPKHIP_KIND_LU;
     // (SPD factor blocks at most 96 columns: the off-diagonal kernel's strip
@@ -684,6 +692,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         const i32 f = large[fi];
         bdesc.push_back(
             BlkDesc{f, B.p0, B.np, B.c0, B.c1, G.diag.wg, 0, 0, fold[fi].first, fold[fi].second});
+        front_fields(bdesc.back(), f, B);
         G.diag.wg += 1;
         const i32 bw = B.c1 - B.c0;
         G.diag.lds = std::max(G.diag.lds, bw * spk::kRLD);
@@ -712,12 +721,14 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         if (t <= 0) continue;
         const i32 nt = (t + 63) / 64;
         bdesc.push_back(BlkDesc{f, B.p0, B.np, B.c0, B.c1, G.offd.wg, nt, 0});
+        front_fields(bdesc.back(), f, B);
         G.offd.wg += 2 * nt;
         const i32 bw = B.c1 - B.c0;
         // strip of the widest extent o + nq, NQ elements per row
         const i32 nqmax = (h->kind == SPKHIP_KIND_LU) ? kNqLU : kNqSPD;
         const i32 strip = (i32)(h->xsuper[B.p0 + B.np] - 1 - h->fc0[f] - B.c0) * nqmax;
         G.offd.lds = std::max(G.offd.lds, 64 * spk::kRLD + strip);
+        if (bw <= 64) G.offd.lds = std::max(G.offd.lds, 2 * 64 * spk::kRW);  // tile + region
         G.offd.flops += 2.0 * (double)bw * bw * t;
         G.offd.elems += 4.0 * (double)bw * t;
       }

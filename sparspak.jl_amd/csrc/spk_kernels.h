@@ -75,6 +75,13 @@ struct BlkDesc {
   int32_t pad;
   int32_t fk0, fk1;  // diagonal kernel: previous block's columns folded in
                      // first (fk1 > fk0), its update running alongside
+  // the front's fields and the widths of the block's pieces (at most eight),
+  // so that a kernel reads no symbolic array before its tiles
+  int32_t L;         // the front's rows
+  int32_t pad2;
+  int64_t fc0;       // the front's first column
+  int64_t foff;      // the front's offset in the arena
+  int32_t pw[8];     // widths of pieces 0 .. np-1
 };
 
 // Per-launch descriptor of the trailing-update kernel: one rectangle of one

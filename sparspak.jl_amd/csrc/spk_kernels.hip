@@ -695,6 +695,9 @@ __device__ inline int wave_min(int v) {
 // columns wide); a compile-time stride turns every column offset into an
 // immediate LDS offset.
 constexpr int kRLD = 129;
+// Row stride of a block's diagonal region (at most 64 columns) held row by
+// row in LDS: lanes storing consecutive rows fall in different banks.
+constexpr int kRW = 65;
 // Most columns of one piece (the SPD limit 90, rounded up).
 constexpr int kMaxNq = 96;
 
@@ -1375,6 +1378,137 @@ blk_diag_slab_kernel(DevSym s, const BlkDesc* __restrict__ desc, T* __restrict__
   blk_diag_slab_body<T>(s, desc[blockIdx.x], arena, ipvt, first_fail, lds);
 }
 
+// Off-diagonal parts of a block of at most 64 columns. The tile, the
+// block's whole diagonal region and (columns) the block's interchanges are
+// read in one batch of loads, with the front and the piece widths (pw) taken
+// from the descriptor; the pieces then follow each other in LDS with the
+// operations, in the order, of the general path below (same results bit for
+// bit). Dynamic LDS: 2 * 64 * kRW elements.
+template <typename T>
+__device__ __attribute__((always_inline)) inline void
+offdiag64_body(const BlkDesc& d, const int32_t* __restrict__ pw, int b, T* __restrict__ arena,
+               const int64_t* __restrict__ ipvt, T* __restrict__ S, int* __restrict__ spiv) {
+  const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
+  const int L = d.L;
+  const int c0 = d.c0, bw = d.c1 - d.c0;
+  T* F = arena + d.foff;
+  const T* Fd = F + c0 + (int64_t)c0 * L;  // the diagonal region
+  if (b < d.nl) {
+    // rows [r0, r0 + nr): X(i, k) at X[i + 64 k]; U(k, j) of the region at
+    // R[k kRW + j]
+    const int r0 = d.c1 + b * 64, nr = min(64, L - r0);
+    T* X = S;
+    T* R = S + 64 * kRW;
+    T* Fx = F + r0 + (int64_t)c0 * L;
+    T xv[16], rv[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int c = wave + 4 * u;
+      xv[u] = (lane < nr && c < bw) ? Fx[lane + (int64_t)c * L] : T(0);
+      rv[u] = (lane < bw && c < bw) ? Fd[lane + (int64_t)c * L] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int c = wave + 4 * u;
+      X[lane + c * 64] = xv[u];
+      R[lane * kRW + c] = rv[u];
+    }
+    __syncthreads();
+    int o = 0;
+    for (int pi = 0; pi < d.np; ++pi) {
+      const int nq = pw[pi];
+      if (o > 0) {  // earlier pieces of the block: X_q -= X_<q U_<q,q
+        for (int cb = wave * 8; cb < nq; cb += 32) {
+          T acc[8];
+#pragma unroll
+          for (int jj = 0; jj < 8; ++jj) acc[jj] = T(0);
+          const int cs = min(cb, nq - 8 >= 0 ? nq - 8 : 0);  // shifted start
+          const T* xr = X + lane;
+          const T* ur = R + o + cs;
+          for (int k = 0; k < o; ++k) {
+            const T xk = xr[k * 64];
+#pragma unroll
+            for (int jj = 0; jj < 8; ++jj) acc[jj] += xk * ur[k * kRW + jj];
+          }
+#pragma unroll
+          for (int jj = 0; jj < 8; ++jj) {
+            const int j = cs + jj;
+            if (j >= cb && j < nq) X[lane + (o + j) * 64] -= acc[jj];
+          }
+        }
+        __syncthreads();
+      }
+      if (wave == 0)  // X_q U_qq^-1, one row per lane
+        blk_trsv<T, 64, kRW, 1, false>(X + lane + o * 64, R + o * kRW + o, nullptr, nq);
+      __syncthreads();
+      o += nq;
+    }
+    store_tile<T, 1, 64>(Fx, L, nr, bw, X);
+  } else {
+    // columns [j0, j0 + nc): A(r, c) at A[r + c kRW]; L(r, k) of the region
+    // at R[r + k kRW]
+    const int j0 = d.c1 + (b - d.nl) * 64, nc = min(64, L - j0);
+    T* A = S;
+    T* R = S + 64 * kRW;
+    T* Fa = F + c0 + (int64_t)j0 * L;
+    const int pv = (tid < bw) ? (int)ipvt[d.fc0 + c0 + tid] - 1 : 0;
+    T av[16], rv[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int c = wave + 4 * u;
+      av[u] = (lane < bw && c < nc) ? Fa[lane + (int64_t)c * L] : T(0);
+      rv[u] = (lane < bw && c < bw) ? Fd[lane + (int64_t)c * L] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int c = wave + 4 * u;
+      A[lane + c * kRW] = av[u];
+      R[lane + c * kRW] = rv[u];
+    }
+    if (tid < bw) spiv[tid] = pv;  // piece-local positions
+    __syncthreads();
+    int o = 0;
+    for (int pi = 0; pi < d.np; ++pi) {
+      const int nq = pw[pi];
+      if (o > 0) {  // earlier pieces: A_q -= L_q,<q A_<q (rows not yet swapped)
+        T* ac = A + lane * kRW;
+        for (int ib = wave * 8; ib < nq; ib += 32) {
+          const int is = min(ib, nq - 8 >= 0 ? nq - 8 : 0);
+          T acc[8];
+#pragma unroll
+          for (int ii = 0; ii < 8; ++ii) acc[ii] = T(0);
+          for (int k = 0; k < o; ++k) {
+            const T a = ac[k];
+#pragma unroll
+            for (int ii = 0; ii < 8; ++ii) acc[ii] += R[o + is + ii + k * kRW] * a;
+          }
+#pragma unroll
+          for (int ii = 0; ii < 8; ++ii) {
+            const int i = is + ii;
+            if (i >= ib && i < nq) ac[o + i] -= acc[ii];
+          }
+        }
+        __syncthreads();
+      }
+      if (wave == 0) {  // interchanges, then L_qq^-1, one column per lane
+        T* a = A + lane * kRW + o;
+        for (int k = 0; k < nq; ++k) {
+          const int kp = spiv[o + k];
+          if (kp != k) {
+            const T t = a[k];
+            a[k] = a[kp];
+            a[kp] = t;
+          }
+        }
+        blk_trsv<T, 1, kRW, 1, true>(a, R + o + o * kRW, nullptr, nq);
+      }
+      __syncthreads();
+      o += nq;
+    }
+    store_tile<T, 1, kRW>(Fa, L, bw, nc, A);
+  }
+}
+
 // Off-diagonal parts of one block after blk_diag_kernel (see the section
 // comment); NQ bounds the piece width. Workgroups b < d.nl take 64 rows
 // below the block, the others 64 columns right of it. Dynamic LDS:
@@ -1387,12 +1521,16 @@ blk_offdiag_kernel(DevSym s, const BlkDesc* __restrict__ desc, int ndesc,
   arena += vs * s.vs_arena;
   ipvt += vs * s.n;
   extern __shared__ unsigned char smem_raw[];
-  __shared__ int spiv[NQ];
+  __shared__ int spiv[NQ > 64 ? NQ : 64];
   T* S = reinterpret_cast<T*>(smem_raw);
   const int tid = threadIdx.x, lane = tid % 64, wave = wave_id();
   const int di = find_blk(desc, ndesc, blockIdx.x);
   const BlkDesc d = desc[di];
   const int b = blockIdx.x - d.wg_begin;
+  if (d.c1 - d.c0 <= 64) {
+    offdiag64_body<T>(d, desc[di].pw, b, arena, ipvt, S, spiv);
+    return;
+  }
   const int f = d.front;
   const int L = s.fL[f];
   const int64_t fc0 = s.fc0[f];
