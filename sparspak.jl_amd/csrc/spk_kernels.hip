@@ -1181,17 +1181,34 @@ __device__ __attribute__((always_inline)) void blk_diag_slab_body(
   __shared__ int spiv[64];                // step k: pivot position, 1-based in its piece
   __shared__ int sdone;                   // steps published
   const int lane = threadIdx.x % 64, w = wave_id();
-  const int f = d.front, L = s.fL[f];
-  const int64_t fc0 = s.fc0[f];
+  const int L = d.L;
+  const int64_t fc0 = d.fc0;
   const int c0 = d.c0, bw = d.c1 - d.c0, np = d.np;
-  T* R = arena + s.foff[f] + c0 + (int64_t)c0 * L;
-  if ((int)threadIdx.x <= np) {
-    int o = bw;
-    if ((int)threadIdx.x < np) {
-      int64_t pc0, nq;
-      piece_cols(s, d.piece + threadIdx.x, pc0, nq);
-      o = (int)(pc0 - fc0) - c0;
+  const T* const Fb = arena + d.foff;
+  T* R = arena + d.foff + c0 + (int64_t)c0 * L;
+  const int cb = SW * w;
+  // the front and the piece widths come from the descriptor, so the region,
+  // the previous block's panels and the widths are read in one batch
+  // a[i] holds column c + i, c = cb while the earlier waves' steps are
+  // applied, then the column of the current step: the slab shifts left by
+  // one column after each of this wave's steps, so that the pivot column is
+  // always a[0] and the loop body indexes registers statically
+  T a[SW];
+#pragma unroll
+  for (int i = 0; i < SW; ++i)
+    a[i] = (lane < bw && cb + i < bw) ? R[lane + (int64_t)(cb + i) * L] : T(0);
+  const int fkw = d.fk1 - d.fk0;
+  if (fkw > 0) {
+    // L(c0 + r, fk0 + k) at sout[k][r], U(fk0 + k, c0 + c) at sl[k][c]
+    for (int idx = threadIdx.x; idx < 64 * fkw; idx += kWG) {
+      const int r = idx % 64, k = idx / 64;
+      sout[k][r] = (r < bw) ? Fb[c0 + r + (int64_t)(d.fk0 + k) * L] : T(0);
+      sl[k][r] = (r < bw) ? Fb[d.fk0 + k + (int64_t)(c0 + r) * L] : T(0);
     }
+  }
+  if ((int)threadIdx.x <= np) {
+    int o = 0;  // spo[np] = bw: the pieces fill the block
+    for (int q = 0; q < (int)threadIdx.x; ++q) o += d.pw[q];
     spo[threadIdx.x] = o;
   }
   if (threadIdx.x == 0) sdone = 0;
@@ -1206,25 +1223,7 @@ __device__ __attribute__((always_inline)) void blk_diag_slab_body(
   const int pr = lane < bw ? scp[lane] : np;
   const int rlo = pr < np ? spo[pr] : 0;
   int pos = lane;
-  const int cb = SW * w;
-  // a[i] holds column c + i, c = cb while the earlier waves' steps are
-  // applied, then the column of the current step: the slab shifts left by
-  // one column after each of this wave's steps, so that the pivot column is
-  // always a[0] and the loop body indexes registers statically
-  T a[SW];
-#pragma unroll
-  for (int i = 0; i < SW; ++i)
-    a[i] = (lane < bw && cb + i < bw) ? R[lane + (int64_t)(cb + i) * L] : T(0);
-  const int fkw = d.fk1 - d.fk0;
   if (fkw > 0) {
-    // L(c0 + r, fk0 + k) at sout[k][r], U(fk0 + k, c0 + c) at sl[k][c]
-    const T* Fb = arena + s.foff[f];
-    for (int idx = threadIdx.x; idx < 64 * fkw; idx += kWG) {
-      const int r = idx % 64, k = idx / 64;
-      sout[k][r] = (r < bw) ? Fb[c0 + r + (int64_t)(d.fk0 + k) * L] : T(0);
-      sl[k][r] = (r < bw) ? Fb[d.fk0 + k + (int64_t)(c0 + r) * L] : T(0);
-    }
-    __syncthreads();
     T acc[SW];
 #pragma unroll
     for (int i = 0; i < SW; ++i) acc[i] = T(0);
