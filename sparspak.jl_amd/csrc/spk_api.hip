@@ -850,8 +850,19 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
           const i32 t = h->fL[large[fi]] - B.c1;
           const spk::SolveDesc& Db = bdata[fi][gi];
           auto fitem = [&](i32 r0, i32 write) {
-            return spk::InvItem{Db.fc0, Db.fvec, Db.inv, 0, Db.L, Db.W, B.c0, B.c1, r0, -1, 0,
-                                write};
+            spk::InvItem it{Db.fc0, Db.fvec, Db.inv, 0, Db.L, Db.W, B.c0, B.c1, r0, -1, 0, write};
+            // the block's pieces: first column and its lnz base (as lcol)
+            for (i32 q = 0; q < 8; ++q) {
+              it.pst[q] = 1 << 30;
+              it.pbs[q] = 0;
+            }
+            for (i32 q = 0; q < B.np; ++q) {
+              const i64 pc0 = h->xsuper[B.p0 + q] - 1;
+              const i64 cq = pc0 - h->fc0[large[fi]];
+              it.pst[q] = (i32)(cq - B.c0);
+              it.pbs[q] = h->xlnz[pc0] - 1 - cq;
+            }
+            return it;
           };
           if (t <= 0) h->iitems.push_back(fitem(-1, 1));
           for (i32 r0 = 0; r0 < t; r0 += 64) {

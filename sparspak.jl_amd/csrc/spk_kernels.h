@@ -130,9 +130,15 @@ struct SolveDesc {
 // rc1. write: it stores the entries it formed (forward: the right-hand
 // side; backward: the front's vector); fin: backward, it stores them in the
 // right-hand side and copies the front's other columns [x1, W) there.
+// Forward items also carry the block's pieces: piece q starts at block
+// column pst[q] (pst[np..7] past the block), its first column's lnz offset
+// in front coordinates is pbs[q] (as DevSym::lcol), and its columns follow
+// each other at the stride L - (x0 + pst[q]), its row count.
 struct InvItem {
   int64_t fc0, fvec, inv, ucol;
   int32_t L, W, x0, x1, r0, c, rc1, flags;  // flags: 1 write, 2 fin
+  int32_t pst[8];
+  int64_t pbs[8];
 };
 
 }  // namespace spk

@@ -3122,14 +3122,17 @@ solve_inv_prep_kernel(const SolveDesc* __restrict__ desc, const float* __restric
 }
 
 // Row i of a block's inverse, columns 64h .. 64h+63, as 16 four-vectors
-// (zero outside the block).
+// (zero outside the block). The row's entries outside columns [jlo, jhi)
+// are zeros of the inverse: four-vectors wholly outside are not read.
 __device__ __attribute__((always_inline)) inline void load_inv_row(
-    const float* __restrict__ M, int bw, int bwp, int i, int h, f32x4 (&mv)[16]) {
+    const float* __restrict__ M, int bw, int bwp, int i, int h, int jlo, int jhi,
+    f32x4 (&mv)[16]) {
   const f32x4* R = reinterpret_cast<const f32x4*>(M + (int64_t)i * bwp);
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const int j = 64 * h + 4 * k;
-    mv[k] = (i < bw && j < bwp) ? R[j / 4] : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    mv[k] = (i < bw && j < bwp && j < jhi && j + 4 > jlo) ? R[j / 4]
+                                                          : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   }
 }
 
@@ -3173,26 +3176,52 @@ fwd_inv_kernel(DevSym s, const InvItem* __restrict__ items, const float* __restr
   __shared__ float zp[2][128];
   __shared__ float zs[128];
   __shared__ float red[4][64];
-  __shared__ int64_t lc[128];
   const int tid = threadIdx.x, lane = tid % 64;
   const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
-  const InvItem it = items[blockIdx.x];
+  const InvItem* const ip = items + blockIdx.x;
+  const InvItem it = *ip;
   const int L = it.L, c0 = it.x0, bw = it.x1 - it.x0, bwp = 4 * ((bw + 3) / 4);
+  // row i of ML = L^-1 P is zero from the end of its piece on (the
+  // interchanges stay inside the pieces)
+  int pend = bw;
+#pragma unroll
+  for (int q = 1; q < 8; ++q) {
+    const int ps = ip->pst[q];
+    if (ps > (tid & 127) && ps < pend) pend = ps;
+  }
   f32x4 mv[16];
-  load_inv_row(inv + it.inv, bw, bwp, tid & 127, tid >> 7, mv);
+  load_inv_row(inv + it.inv, bw, bwp, tid & 127, tid >> 7, 0, pend, mv);
   const bool tile = it.r0 >= 0;
   const int r = it.x1 + it.r0 + lane;
   const int cb = bw * wave / 4, ce = bw * (wave + 1) / 4;
-  if (tile && tid < bw) lc[tid] = s.lcol[it.fc0 + c0 + tid];
+  // the tile's L entries, in flight with the inverse: block column c's lnz
+  // offset from the item's piece table (a piece's columns follow each other
+  // at its row count; the column index is uniform in the wave)
+  float lv[32];
+  {
+    int q = 0;
+    while (q < 7 && ip->pst[q + 1] <= cb) ++q;
+    int64_t str = L - c0 - ip->pst[q];
+    int64_t lcc = ip->pbs[q] + (int64_t)(cb - ip->pst[q]) * str;
+    int nxt = q < 7 ? ip->pst[q + 1] : (1 << 30);
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+      const int c = cb + j;
+      if (c == nxt) {
+        ++q;
+        str = L - c0 - ip->pst[q];
+        lcc = ip->pbs[q];
+        nxt = q < 7 ? ip->pst[q + 1] : (1 << 30);
+      }
+      lv[j] = (tile && r < L && c < ce) ? lnz[lcc + r] : 0.0f;
+      lcc += str;
+    }
+  }
   float* const yf = vec + it.fvec * nrhs;
-  // the first right-hand side's y, in flight with the column offsets
+  // the first right-hand side's y
   if (tid < 128) yb[tid] = (tid < bw) ? yf[c0 + tid] : 0.0f;
   float yr = (tile && wave == 0 && r < L) ? yf[r] : 0.0f;
   __syncthreads();
-  float lv[32];
-#pragma unroll
-  for (int j = 0; j < 32; ++j)
-    lv[j] = (tile && r < L && cb + j < ce) ? lnz[lc[cb + j] + r] : 0.0f;
   for (int q = 0; q < nrhs; ++q) {
     float* yg = yf + (int64_t)q * L;
     if (q > 0) {
@@ -3248,7 +3277,9 @@ bwd_inv_kernel(DevSym s, const InvItem* __restrict__ items, const float* __restr
   const bool hasx = xbw > 0;
   const int bwp = 4 * ((xbw + 3) / 4);
   f32x4 mv[16];
-  if (hasx) load_inv_row(inv + it.inv + (int64_t)bwp * bwp, xbw, bwp, tid & 127, tid >> 7, mv);
+  if (hasx)  // MU is upper triangular
+    load_inv_row(inv + it.inv + (int64_t)bwp * bwp, xbw, bwp, tid & 127, tid >> 7, tid & 127,
+                 xbw, mv);
   float* const yf = vec + it.fvec * nrhs;
   for (int q0 = 0; q0 < nrhs; q0 += QB) {
     const int nq = min(QB, nrhs - q0);

@@ -7,6 +7,7 @@ import pytest
 import scipy.sparse as sp
 
 from helpers import brute_symbolic, csc1, random_pattern, symbolic
+from sparspak_hip.gallery import convdiff3d, elasticity3d, laplace2d
 from sparspak_hip.problem import Problem
 from sparspak_hip.symbolic import Graph, SymbolicFactor
 
@@ -220,3 +221,41 @@ def test_findorderperm_identity():
     s.findorderperm(np.arange(1, n + 1))
     s.symbolicfactor()
     _check_against_brute(A, s)
+
+
+def _fronts(s):
+    """Chains of consecutive supernodes whose index lists nest (piece k+1's
+    list is piece k's without its own columns): the device's fronts."""
+    xs, xl, li = np.asarray(s.xsuper), np.asarray(s.xlindx), np.asarray(s.lindx)
+    nj = lambda J: xs[J + 1] - xs[J]
+    jl = lambda J: xl[J + 1] - xl[J]
+    fronts = []
+    for J in range(s.nsuper):
+        if (J > 0 and jl(J - 1) > nj(J - 1) and jl(J - 1) - nj(J - 1) == jl(J)
+                and li[xl[J - 1] - 1 + nj(J - 1)] == xs[J]):
+            fronts[-1].append(J)
+        else:
+            fronts.append([J])
+    return fronts
+
+
+@pytest.mark.parametrize("kind,gen", [
+    ("lu", lambda: random_pattern(300, 0.03, 4, False, False)),
+    ("lu", lambda: convdiff3d(14)),
+    ("spd", lambda: elasticity3d(5)),
+    ("spd", lambda: laplace2d(30)),
+])
+def test_front_piece_column_stride(kind, gen):
+    # the solve kernels address a front's factor columns from each piece's
+    # first column: inside a front with index list length L, every column of
+    # a piece starting at front column cq holds L - cq entries of lnz, and the
+    # columns are consecutive in lnz
+    s = symbolic(gen(), kind=kind)
+    xs, xl, xlnz = np.asarray(s.xsuper), np.asarray(s.xlindx), np.asarray(s.xlnz)
+    for fr in _fronts(s):
+        L = xl[fr[0] + 1] - xl[fr[0]]
+        fc0 = xs[fr[0]] - 1
+        for p in fr:
+            cq = xs[p] - 1 - fc0
+            cols = np.arange(xs[p] - 1, xs[p + 1] - 1)
+            assert np.all(xlnz[cols + 1] - xlnz[cols] == L - cq)
