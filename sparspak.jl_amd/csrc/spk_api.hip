@@ -517,14 +517,13 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
   {
     // blocks of at most 64 columns: the diagonal region in registers
     // (SPKHIP_DIAG_SLAB=0 keeps the LDS form; an SPD step with a wider
-    // block, a piece of more than 64 columns, takes the LDS form), and, LU
-    // only, each in-group update beside the next block's regions
-    // (SPKHIP_MERGE_BORDER=0: after them)
+    // block, a piece of more than 64 columns, takes the LDS form), and each
+    // in-group update beside the next block's regions (SPKHIP_MERGE_BORDER=0:
+    // after them; not before an SPD step with a wider block)
     const char* ds = std::getenv("SPKHIP_DIAG_SLAB");
     const char* mb = std::getenv("SPKHIP_MERGE_BORDER");
     h->slab_plan = h->block_cols <= 64 && !(ds && ds[0] == '0');
-    h->merge_plan =
-        h->slab_plan && h->kind == SPKHIP_KIND_LU && !(mb && mb[0] == '0');
+    h->merge_plan = h->slab_plan && !(mb && mb[0] == '0');
   }
   h->pdesc.clear();
   h->iitems.clear();
@@ -683,6 +682,12 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
     std::vector<spk::GemmDesc> carry;
     LevelPlan::Launch carryL;
     std::vector<std::pair<i32, i32>> fold(large.size(), {0, 0}), fold_next(large.size(), {0, 0});
+    // steps holding a block wider than 64 columns (diagonal regions in LDS,
+    // so no merged launch into them)
+    std::vector<char> wide_step(maxg + 1, 0);
+    for (size_t fi = 0; fi < large.size(); ++fi)
+      for (size_t g = 0; g < fblocks[fi].size(); ++g)
+        if (fblocks[fi][g].c1 - fblocks[fi][g].c0 > 64) wide_step[g] = 1;
     for (size_t gi = 0; gi < maxg; ++gi) {
       LevelPlan::Group G;
       G.diag.off = (int)bdesc.size();
@@ -751,11 +756,26 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
           // the tiles on or below the diagonal are computed, one descriptor
           // per 128-column tile column with rows from that column's start;
           // the entries below the column's own square also write their
-          // transposes. A
-          // rectangle above the diagonal is the transpose of one below it in
-          // the same launch (the plan's rectangles pair up that way).
+          // transposes. A rectangle above the diagonal is the transpose of
+          // one below it in the same launch (the plan's rectangles pair up
+          // that way).
           if (r1 <= j0) return;
-          if (r0 != j0) throw ApiError{-4};
+          if (r0 < j0) throw ApiError{-4};
+          if (r0 > j0) {
+            // rows below the next block's diagonal region (columns [j0, r0),
+            // which the merged launch's diagonal kernel takes): one
+            // descriptor whose every entry lies past its columns and so
+            // also writes its transpose; the columns from r0 on follow as
+            // tile columns of their own
+            const i32 jl = std::min(r0, j1), ntr = (r1 - r0 + 127) / 128,
+                      ntc = (jl - j0 + 127) / 128;
+            out.push_back(spk::GemmDesc{f, k0, k1, r0, r1 - r0, j0, jl - j0, Lc.wg, ntr, big, 1});
+            Lc.wg += ntr * ntc;
+            const double m = r1 - r0, nn = jl - j0;
+            Lc.flops += 2.0 * m * nn * kk;
+            Lc.elems += 2.0 * m * nn + (m + nn) * kk + m * nn;  // + the transposes
+            j0 = jl;
+          }
           for (i32 cj = j0; cj < j1; cj += 128) {
             const i32 ce = std::min(cj + 128, j1), ntr = (r1 - cj + 127) / 128;
             out.push_back(spk::GemmDesc{f, k0, k1, cj, r1 - cj, cj, ce - cj, Lc.wg, ntr, big, 1});
@@ -783,7 +803,7 @@ void build_plan(spkhip_handle* h, std::vector<i32>& small, std::vector<i32>& it_
         if (gi + 1 < ge) {
           // border of the group's remaining blocks [B.c1, E) x ..., E = their end
           const i32 E = fblocks[fi][ge - 1].c1;
-          if (h->merge_plan) {
+          if (h->merge_plan && !wide_step[gi + 1]) {
             // all of it but the next block's diagonal region, which the
             // next step's diagonal kernel takes itself, beside the rest
             const i32 cn = fblocks[fi][gi + 1].c1;

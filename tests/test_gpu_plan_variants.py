@@ -2,9 +2,11 @@
 registers or in LDS (SPKHIP_DIAG_SLAB), each in-group update merged with the
 next block's diagonal launch or launched after it (SPKHIP_MERGE_BORDER), for
 LU with pivoting confined to the supernodes (SpkLUFactor.jl:230-240) and for
-SPD (no interchanges, symmetric updates). Every variant gives the oracle's
-pivots and its entries within 1e-12 (double). The switches are read when the
-plan is built, so each variant uploads its own symbolic factor."""
+SPD (no interchanges, symmetric updates; merged, the update's first tile
+column starts below the next block's region and mirrors the entries left of
+it). Every variant gives the oracle's pivots and its entries within 1e-12
+(double). The switches are read when the plan is built, so each variant
+uploads its own symbolic factor."""
 import os
 
 import numpy as np
@@ -72,7 +74,7 @@ def test_lu_plan_variants_match_oracle(name, variant):
     assert backward_error(A, x, b) <= max(4.0 * eta_o, 1e-15)
 
 
-@pytest.mark.parametrize("variant", ["slab", "lds"])
+@pytest.mark.parametrize("variant", list(VARIANTS))
 def test_spd_plan_variants_match_oracle(variant):
     print(f"case plan variant SPD elasticity3d 8 {variant}", flush=True)
     A = sp.csc_matrix(elasticity3d(8))
