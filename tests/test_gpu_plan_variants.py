@@ -74,10 +74,22 @@ def test_lu_plan_variants_match_oracle(name, variant):
     assert backward_error(A, x, b) <= max(4.0 * eta_o, 1e-15)
 
 
+def _dense_spd(n, seed):
+    # one front of three pieces, 42, 43 and 65 columns at n = 150: the third
+    # block is wider than 64 columns, so the step before it merges nothing
+    rng = np.random.default_rng(seed)
+    M = rng.uniform(-1, 1, (n, n))
+    return sp.csc_matrix((M + M.T) / 2 + n * np.eye(n))
+
+
+SPD_CASES = {"elasticity3d_8": lambda: elasticity3d(8), "dense_spd_150": lambda: _dense_spd(150, 3)}
+
+
 @pytest.mark.parametrize("variant", list(VARIANTS))
-def test_spd_plan_variants_match_oracle(variant):
-    print(f"case plan variant SPD elasticity3d 8 {variant}", flush=True)
-    A = sp.csc_matrix(elasticity3d(8))
+@pytest.mark.parametrize("name", list(SPD_CASES))
+def test_spd_plan_variants_match_oracle(name, variant):
+    print(f"case plan variant SPD {name} {variant}", flush=True)
+    A = sp.csc_matrix(SPD_CASES[name]())
     s = _factor_with(A, SparseSpdSolver, *VARIANTS[variant])
     lnz = s.slvr.download()[0]
     n, cp, rv, nz = csc1(A)
