@@ -179,6 +179,8 @@ struct spkhip_handle {
   DevBuf<i32> d_lindx, d_fL, d_fW, d_fc0, d_fsp0, d_fnp, d_fcbld, d_fchp, d_fch,
       d_relmap, d_small, d_it_front, d_it_col, d_ea_child, d_ea_t, d_solve;
   DevBuf<i64> d_foff, d_fcb, d_frel, d_fvec, d_it_ea, d_lcol, d_ucol;
+  DevBuf<spk::InitItem> d_it_init;  // the assembly's items, resolved
+  DevBuf<spk::EaItem> d_ea_init;
   DevBuf<BlkDesc> d_bdesc;
   DevBuf<spk::GemmDesc> d_gdesc;
   DevBuf<i32> d_large, d_ut_front, d_ut_piece, d_ut_r0, d_wb_front, d_wb_col;
@@ -1163,15 +1165,14 @@ void run_factor(spkhip_handle* h) {
       const bool staged = clds <= 64 * 1024;
       if (lu && clds <= 16 * 1024)
         hipLaunchKernelGGL(spk::large_init_wave_kernel<T>, dim3((lp.items_cnt + 3) / 4, h->nvs),
-                           dim3(kWG), 4 * clds, st, h->ds, h->d_it_front.p + lp.items_off,
-                           h->d_it_col.p + lp.items_off, h->d_it_ea.p + lp.items_off,
-                           h->d_ea_child.p, h->d_ea_t.p, lnz.p, arena.p, lp.items_cnt,
-                           lp.large_maxL);
+                           dim3(kWG), 4 * clds, st, h->ds, h->d_it_init.p + lp.items_off,
+                           h->d_it_ea.p + lp.items_off, h->d_ea_init.p, lnz.p, arena.p,
+                           lp.items_cnt, lp.large_maxL);
       else
         hipLaunchKernelGGL(spk::large_init_kernel<T>, dim3(lp.items_cnt, h->nvs), dim3(kWG),
-                           staged ? clds : 0, st, h->ds, h->d_it_front.p + lp.items_off,
-                           h->d_it_col.p + lp.items_off, h->d_it_ea.p + lp.items_off,
-                           h->d_ea_child.p, h->d_ea_t.p, lnz.p, arena.p, (int)staged);
+                           staged ? clds : 0, st, h->ds, h->d_it_init.p + lp.items_off,
+                           h->d_it_ea.p + lp.items_off, h->d_ea_init.p, lnz.p, arena.p,
+                           (int)staged);
       hipck(hipGetLastError());
       }
       bool pending = false;  // second-stream work not yet joined
@@ -1659,6 +1660,38 @@ void replan(spkhip_handle* h) {
   }
   h->d_ea_child.upload(ea_child, st);
   h->d_ea_t.upload(ea_t, st);
+  {
+    // the assembly's items with their front's fields, and each child entry's
+    // update column and map, resolved here so that the kernels start with
+    // one dependent read
+    std::vector<spk::InitItem> ini(it_front.size());
+    for (size_t i = 0; i < it_front.size(); ++i) {
+      const i32 f = it_front[i], j = it_col[i];
+      spk::InitItem& m = ini[i];
+      m.L = h->fL[f];
+      m.W = h->fW[f];
+      m.j = j;
+      m.fc0 = h->fc0[f];
+      m.fcol = h->foff[f] + (i64)j * m.L;
+      if (j < m.W) {
+        const i64 g = h->fc0[f] + j;
+        const i64 cq = h->xsuper[h->snode[g] - 1] - 1 - h->fc0[f];
+        m.r1 = (i32)cq;
+        m.lcb = h->xlnz[g] - 1 - cq;
+      } else {
+        m.r1 = m.W;
+        m.lcb = -1;
+      }
+    }
+    std::vector<spk::EaItem> eai(ea_child.size());
+    for (size_t e = 0; e < ea_child.size(); ++e) {
+      const i32 c = ea_child[e];
+      eai[e] = spk::EaItem{h->fcb[c] + (i64)ea_t[e] * h->fcbld[c], h->frel[c],
+                           h->fL[c] - h->fW[c], 0};
+    }
+    h->d_it_init.upload(ini, st);
+    h->d_ea_init.upload(eai, st);
+  }
   if (bdesc.empty()) bdesc.push_back(BlkDesc{});
   if (gdesc.empty()) gdesc.push_back(spk::GemmDesc{});
   h->d_bdesc.upload(bdesc, st);

@@ -100,6 +100,26 @@ struct GemmDesc {
                      // (no tile above the diagonal is computed)
 };
 
+// The large-front assembly's items, resolved on the host so that its kernels
+// read no symbolic array before the column: one per (front f, column j)...
+struct InitItem {
+  int64_t fcol;  // arena offset of column j of the front's work matrix
+  int64_t lcb;   // own column: lnz offset of front row 0 (as DevSym::lcol);
+                 // -1: an update column
+  int64_t fc0;   // the front's first column
+  int32_t L, W;  // the front's rows and own columns
+  int32_t r1;    // first row of the column's L part (its piece's first
+                 // column); W for an update column
+  int32_t j;
+};
+// ... and one per (child, update column) pair added into a column.
+struct EaItem {
+  int64_t col;  // arena offset of the child's update-matrix column
+  int64_t rel;  // offset of the child's relative-index map
+  int32_t Rc;   // the child's update rows
+  int32_t pad;
+};
+
 // Integers of a block's solve data: the composed row permutation.
 constexpr int kPrepInts = 128;
 

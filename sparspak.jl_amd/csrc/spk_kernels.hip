@@ -353,50 +353,48 @@ large_utile_kernel(DevSym s, const int32_t* __restrict__ it_front,
 // has placed the U part. With staged set the column is assembled in LDS
 // (dynamic, L elements): read once (the U part from the front, the rest from
 // lnz or zero), the children's columns added there, written once; otherwise
-// the children's entries are added in the front in place.
+// the children's entries are added in the front in place. The item and each
+// child's entry come resolved (InitItem, EaItem); the next child's entry is
+// read while the current child's column is added.
 template <typename T>
 __global__ void __launch_bounds__(kWG)
-large_init_kernel(DevSym s, const int32_t* __restrict__ it_front,
-                  const int32_t* __restrict__ it_col,
-                  const int64_t* __restrict__ it_ea,
-                  const int32_t* __restrict__ ea_child,
-                  const int32_t* __restrict__ ea_t, const T* __restrict__ lnz,
-                  T* __restrict__ arena, int staged) {
+large_init_kernel(DevSym s, const InitItem* __restrict__ items,
+                  const int64_t* __restrict__ it_ea, const EaItem* __restrict__ ea,
+                  const T* __restrict__ lnz, T* __restrict__ arena, int staged) {
   const int64_t vs = blockIdx.y;  // value set
   lnz += vs * s.vs_lnz;
   arena += vs * s.vs_arena;
   extern __shared__ unsigned char smem_raw[];
   const int item = blockIdx.x;
-  const int f = it_front[item];
-  const int j = it_col[item];
-  const int L = s.fL[f], W = s.fW[f];
-  const int64_t fc0 = s.fc0[f];
-  T* F = arena + s.foff[f] + (int64_t)j * L;
+  const InitItem m = items[item];
+  const int L = m.L, W = m.W, j = m.j;
+  T* F = arena + m.fcol;
   const int64_t e0 = it_ea[item], e1 = it_ea[item + 1];
+  EaItem en = ea[e0 < e1 ? e0 : 0];
   if (staged && !s.sym) {
     T* Fs = reinterpret_cast<T*>(smem_raw);
     // rows below r1 come from lnz (own column) or are zero (update column);
     // rows above it are the U part already in the front
-    const int r1 = (j < W) ? piece_start_of_col(s, fc0, j) : W;
-    const T* lc = (j < W) ? lnz + s.lcol[fc0 + j] : nullptr;
-    for (int r0 = threadIdx.x; r0 < L; r0 += 8 * kWG) {
-      T v[8];
+    const int r1 = m.r1;
+    const T* lc = (m.lcb >= 0) ? lnz + m.lcb : nullptr;
+    for (int r0 = threadIdx.x; r0 < L; r0 += 16 * kWG) {
+      T v[16];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < 16; ++u) {
         const int r = r0 + u * kWG;
         v[u] = (r >= L) ? T(0) : (r < r1) ? F[r] : (lc ? lc[r] : T(0));
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
+      for (int u = 0; u < 16; ++u)
         if (r0 + u * kWG < L) Fs[r0 + u * kWG] = v[u];
     }
     __syncthreads();
     for (int64_t e = e0; e < e1; ++e) {
-      const int c = ea_child[e];
-      const int t = ea_t[e];
-      const int Rc = s.fL[c] - s.fW[c];
-      const int32_t* rel = s.relmap + s.frel[c];
-      const T* col = arena + s.fcb[c] + (int64_t)t * s.fcbld[c];
+      const EaItem ec = en;
+      if (e + 1 < e1) en = ea[e + 1];
+      const int Rc = ec.Rc;
+      const int32_t* rel = s.relmap + ec.rel;
+      const T* col = arena + ec.col;
       for (int tp0 = threadIdx.x; tp0 < Rc; tp0 += 8 * kWG) {
         int ri[8];
         T cv[8];
@@ -423,12 +421,12 @@ large_init_kernel(DevSym s, const int32_t* __restrict__ it_front,
     return;
   }
   if (j < W) {
-    const int cp = piece_start_of_col(s, fc0, j);
-    const T* lc = lnz + s.lcol[fc0 + j];
+    const int cp = m.r1;
+    const T* lc = lnz + m.lcb;
     if (s.sym) {  // lower triangle only: the diagonal block's upper part is U
       for (int r = cp + threadIdx.x; r < L; r += kWG)
         if (r >= j) F[r] = lc[r];
-        else F[r] = lnz[s.lcol[fc0 + r] + j];
+        else F[r] = lnz[s.lcol[m.fc0 + r] + j];
     } else {
       // eight loads in flight per thread
       for (int r0 = cp + threadIdx.x; r0 < L; r0 += 8 * kWG) {
@@ -445,11 +443,11 @@ large_init_kernel(DevSym s, const int32_t* __restrict__ it_front,
   }
   __syncthreads();
   for (int64_t e = e0; e < e1; ++e) {
-    const int c = ea_child[e];
-    const int t = ea_t[e];
-    const int Rc = s.fL[c] - s.fW[c];
-    const int32_t* rel = s.relmap + s.frel[c];
-    const T* col = arena + s.fcb[c] + (int64_t)t * s.fcbld[c];
+    const EaItem ec = en;
+    if (e + 1 < e1) en = ea[e + 1];
+    const int Rc = ec.Rc;
+    const int32_t* rel = s.relmap + ec.rel;
+    const T* col = arena + ec.col;
     for (int tp0 = threadIdx.x; tp0 < Rc; tp0 += 4 * kWG) {
       int ri[4];
       T cv[4], fv[4];
@@ -477,12 +475,9 @@ large_init_kernel(DevSym s, const int32_t* __restrict__ it_front,
 // one after another without barriers, in the same order as there.
 template <typename T>
 __global__ void __launch_bounds__(kWG)
-large_init_wave_kernel(DevSym s, const int32_t* __restrict__ it_front,
-                       const int32_t* __restrict__ it_col,
-                       const int64_t* __restrict__ it_ea,
-                       const int32_t* __restrict__ ea_child,
-                       const int32_t* __restrict__ ea_t, const T* __restrict__ lnz,
-                       T* __restrict__ arena, int nitems, int maxL) {
+large_init_wave_kernel(DevSym s, const InitItem* __restrict__ items,
+                       const int64_t* __restrict__ it_ea, const EaItem* __restrict__ ea,
+                       const T* __restrict__ lnz, T* __restrict__ arena, int nitems, int maxL) {
   const int64_t vs = blockIdx.y;  // value set
   lnz += vs * s.vs_lnz;
   arena += vs * s.vs_arena;
@@ -491,14 +486,13 @@ large_init_wave_kernel(DevSym s, const int32_t* __restrict__ it_front,
   const int item = blockIdx.x * 4 + wave;
   if (item >= nitems) return;  // no workgroup barrier below
   T* Fs = reinterpret_cast<T*>(smem_raw) + (size_t)wave * maxL;
-  const int f = it_front[item];
-  const int j = it_col[item];
-  const int L = s.fL[f], W = s.fW[f];
-  const int64_t fc0 = s.fc0[f];
-  T* F = arena + s.foff[f] + (int64_t)j * L;
+  const InitItem m = items[item];
+  const int L = m.L;
+  T* F = arena + m.fcol;
   const int64_t e0 = it_ea[item], e1 = it_ea[item + 1];
-  const int r1 = (j < W) ? piece_start_of_col(s, fc0, j) : W;
-  const T* lc = (j < W) ? lnz + s.lcol[fc0 + j] : nullptr;
+  EaItem en = ea[e0 < e1 ? e0 : 0];
+  const int r1 = m.r1;
+  const T* lc = (m.lcb >= 0) ? lnz + m.lcb : nullptr;
   for (int r0 = lane; r0 < L; r0 += 8 * 64) {
     T v[8];
 #pragma unroll
@@ -512,11 +506,11 @@ large_init_wave_kernel(DevSym s, const int32_t* __restrict__ it_front,
   }
   wave_lds_sync();
   for (int64_t e = e0; e < e1; ++e) {
-    const int c = ea_child[e];
-    const int t = ea_t[e];
-    const int Rc = s.fL[c] - s.fW[c];
-    const int32_t* rel = s.relmap + s.frel[c];
-    const T* col = arena + s.fcb[c] + (int64_t)t * s.fcbld[c];
+    const EaItem ec = en;
+    if (e + 1 < e1) en = ea[e + 1];
+    const int Rc = ec.Rc;
+    const int32_t* rel = s.relmap + ec.rel;
+    const T* col = arena + ec.col;
     for (int tp0 = lane; tp0 < Rc; tp0 += 8 * 64) {
       int ri[8];
       T cv[8];
