@@ -1233,16 +1233,20 @@ __device__ __attribute__((always_inline)) void blk_diag_slab_body(
   }
   // first step of piece pk (current column c): the later pieces' entries
   // right of the piece are set aside and the piece's products summed from
-  // zero in their place; last step: the entries take the sum off. Called
-  // under wave-uniform branches that hold a compiler barrier, so that they
-  // stay branches instead of selects executed at every step.
+  // zero in their place; last step: the entries take the sum off. Only this
+  // wave's own columns [cb, cb + SW): past them a[] holds placeholders, and
+  // the set-aside slots there belong to the wave owning those columns, which
+  // may still be restoring an earlier piece from them. Called under
+  // wave-uniform branches that hold a compiler barrier, so that they stay
+  // branches instead of selects executed at every step.
+  const int cown = cb + SW;
   auto piece_begin = [&](int pk, int c) {
     __asm__ volatile("" ::: "memory");
     const int cn = __builtin_amdgcn_readfirstlane(spo[pk + 1]);
     const bool later = pr > pk && pr < np;
 #pragma unroll
     for (int i = 0; i < SW; ++i)
-      if (c + i >= cn && c + i < 64) {
+      if (c + i >= cn && c + i < cown) {
         ssv[c + i][lane] = a[i];
         if (later) a[i] = T(0);
       }
@@ -1253,7 +1257,7 @@ __device__ __attribute__((always_inline)) void blk_diag_slab_body(
     const bool later = pr > pk && pr < np;
 #pragma unroll
     for (int i = 0; i < SW; ++i)
-      if (c + i >= cn && c + i < 64 && later) a[i] = ssv[c + i][lane] + a[i];
+      if (c + i >= cn && c + i < cown && later) a[i] = ssv[c + i][lane] + a[i];
   };
   // the piece of the current step, [plo, phi): advanced at its boundaries
   int pk = 0;

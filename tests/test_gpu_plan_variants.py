@@ -24,16 +24,18 @@ pytestmark = pytest.mark.gpu
 VARIANTS = {"slab+merge": ("1", "1"), "slab": ("1", "0"), "lds": ("0", "0")}
 
 
-def _factor_with(A, cls, slab, merge):
+def _factor_with(A, cls, slab, merge, maxblocksize=None, precision="fp64"):
     old = {k: os.environ.get(k) for k in ("SPKHIP_DIAG_SLAB", "SPKHIP_MERGE_BORDER")}
     os.environ["SPKHIP_DIAG_SLAB"] = slab
     os.environ["SPKHIP_MERGE_BORDER"] = merge
     try:
         s = cls(A)
+        if maxblocksize is not None:  # smaller pieces: more of them per block
+            s.slvr.sym.maxblocksize = maxblocksize
         findorder(s)
         symbolicfactor(s)
         inmatrix(s)
-        factor(s, "fp64")
+        factor(s, precision)
         return s
     finally:
         for k, v in old.items():
@@ -102,3 +104,56 @@ def test_spd_plan_variants_match_oracle(name, variant):
     x = b.copy()
     triangularsolve(s, x)
     assert backward_error(A, x, b) < 1e-14
+
+
+# pieces of at most 6 or 8 columns: factor blocks of up to eight pieces (the
+# descriptors' piece tables full) and solve blocks of up to eight pieces
+MANY = {"convdiff3d_16_lu_mb8": (lambda: convdiff3d(16), SparseSolver, 8),
+        "convdiff3d_16_lu_mb6": (lambda: convdiff3d(16), SparseSolver, 6),
+        "elasticity3d_6_spd_mb6": (lambda: elasticity3d(6), SparseSpdSolver, 6)}
+
+
+@pytest.mark.parametrize("variant", ["slab+merge", "lds"])
+@pytest.mark.parametrize("name", list(MANY))
+def test_many_pieces_per_block_match_oracle(name, variant):
+    print(f"case plan variant many pieces {name} {variant}", flush=True)
+    gen, cls, mb = MANY[name]
+    A = sp.csc_matrix(gen())
+    s = _factor_with(A, cls, *VARIANTS[variant], maxblocksize=mb)
+    sym = s.slvr.sym
+    assert max(np.diff(np.asarray(sym.xsuper))) <= 1.5 * mb
+    lnz, unz, ipiv = s.slvr.download()
+    n, cp, rv, nz = csc1(A)
+    if cls is SparseSolver:
+        ol, ou, op, rc, _ = orc.factor_csc(sym, cp, rv, nz)
+        assert rc == 0
+        assert np.array_equal(ipiv, op), "pivots differ from the oracle"
+        scale = max(np.max(np.abs(ol)), np.max(np.abs(ou)))
+        assert np.max(np.abs(lnz - ol)) <= 1e-12 * scale
+        assert np.max(np.abs(unz - ou)) <= 1e-12 * scale
+    else:
+        ol, _, _ = orc.inmatrix(sym, cp, rv, nz)
+        orc.ldltfactor(sym, ol)
+        m = spd_meaningful(sym)
+        assert np.max(np.abs(lnz[m] - ol[m])) <= 1e-12 * np.max(np.abs(ol[m]))
+    # float factor: explicit-inverse solves (blocks of up to eight pieces)
+    # against substitution, and the refined solution
+    s32 = _factor_with(A, cls, *VARIANTS[variant], maxblocksize=mb, precision="fp32")
+    b = uniform(9, n, -1.0, 1.0)
+    xs = []
+    for inv in ("1", "0"):
+        old = os.environ.get("SPKHIP_SOLVE_INV")
+        os.environ["SPKHIP_SOLVE_INV"] = inv
+        try:
+            x = b.copy()
+            triangularsolve(s32, x, refine=False)
+            xs.append(x)
+        finally:
+            if old is None:
+                os.environ.pop("SPKHIP_SOLVE_INV", None)
+            else:
+                os.environ["SPKHIP_SOLVE_INV"] = old
+    assert np.max(np.abs(xs[0] - xs[1])) <= 1e-4 * np.max(np.abs(xs[1]))
+    x = b.copy()
+    triangularsolve(s32, x)
+    assert backward_error(A, x, b) <= 1e-14
