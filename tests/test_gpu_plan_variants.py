@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 VARIANTS = {"slab+merge": ("1", "1"), "slab": ("1", "0"), "lds": ("0", "0")}
 
 
-def _factor_with(A, cls, slab, merge, maxblocksize=None, precision="fp64"):
+def _factor_with(A, cls, slab, merge, maxblocksize=None, precision="fp64", tuning=None):
     old = {k: os.environ.get(k) for k in ("SPKHIP_DIAG_SLAB", "SPKHIP_MERGE_BORDER")}
     os.environ["SPKHIP_DIAG_SLAB"] = slab
     os.environ["SPKHIP_MERGE_BORDER"] = merge
@@ -35,6 +35,8 @@ def _factor_with(A, cls, slab, merge, maxblocksize=None, precision="fp64"):
         findorder(s)
         symbolicfactor(s)
         inmatrix(s)
+        if tuning:
+            s.slvr.set_tuning(**tuning)
         factor(s, precision)
         return s
     finally:
@@ -108,18 +110,24 @@ def test_spd_plan_variants_match_oracle(name, variant):
 
 # pieces of at most 6 or 8 columns: factor blocks of up to eight pieces (the
 # descriptors' piece tables full) and solve blocks of up to eight pieces
-MANY = {"convdiff3d_16_lu_mb8": (lambda: convdiff3d(16), SparseSolver, 8),
-        "convdiff3d_16_lu_mb6": (lambda: convdiff3d(16), SparseSolver, 6),
-        "elasticity3d_6_spd_mb6": (lambda: elasticity3d(6), SparseSpdSolver, 6)}
+# (blocks of 96 or 128 columns take the LDS diagonal kernel and the general
+# off-diagonal path, whatever the variant)
+MANY = {"convdiff3d_16_lu_mb8": (lambda: convdiff3d(16), SparseSolver, 8, None),
+        "convdiff3d_16_lu_mb6": (lambda: convdiff3d(16), SparseSolver, 6, None),
+        "elasticity3d_6_spd_mb6": (lambda: elasticity3d(6), SparseSpdSolver, 6, None),
+        "convdiff3d_16_lu_mb8_block128": (lambda: convdiff3d(16), SparseSolver, 8,
+                                          dict(block_cols=128)),
+        "elasticity3d_6_spd_mb6_block96": (lambda: elasticity3d(6), SparseSpdSolver, 6,
+                                           dict(block_cols=96))}
 
 
 @pytest.mark.parametrize("variant", ["slab+merge", "lds"])
 @pytest.mark.parametrize("name", list(MANY))
 def test_many_pieces_per_block_match_oracle(name, variant):
     print(f"case plan variant many pieces {name} {variant}", flush=True)
-    gen, cls, mb = MANY[name]
+    gen, cls, mb, tuning = MANY[name]
     A = sp.csc_matrix(gen())
-    s = _factor_with(A, cls, *VARIANTS[variant], maxblocksize=mb)
+    s = _factor_with(A, cls, *VARIANTS[variant], maxblocksize=mb, tuning=tuning)
     sym = s.slvr.sym
     assert max(np.diff(np.asarray(sym.xsuper))) <= 1.5 * mb
     lnz, unz, ipiv = s.slvr.download()
@@ -138,7 +146,8 @@ def test_many_pieces_per_block_match_oracle(name, variant):
         assert np.max(np.abs(lnz[m] - ol[m])) <= 1e-12 * np.max(np.abs(ol[m]))
     # float factor: explicit-inverse solves (blocks of up to eight pieces)
     # against substitution, and the refined solution
-    s32 = _factor_with(A, cls, *VARIANTS[variant], maxblocksize=mb, precision="fp32")
+    s32 = _factor_with(A, cls, *VARIANTS[variant], maxblocksize=mb, precision="fp32",
+                       tuning=tuning)
     b = uniform(9, n, -1.0, 1.0)
     xs = []
     for inv in ("1", "0"):
