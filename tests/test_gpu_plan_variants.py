@@ -114,6 +114,8 @@ def test_spd_plan_variants_match_oracle(name, variant):
 # off-diagonal path, whatever the variant)
 MANY = {"convdiff3d_16_lu_mb8": (lambda: convdiff3d(16), SparseSolver, 8, None),
         "convdiff3d_16_lu_mb6": (lambda: convdiff3d(16), SparseSolver, 6, None),
+        "convdiff3d_14_lu_mb4": (lambda: convdiff3d(14), SparseSolver, 4, None),
+        "convdiff3d_16_lu_mb12": (lambda: convdiff3d(16), SparseSolver, 12, None),
         "elasticity3d_6_spd_mb6": (lambda: elasticity3d(6), SparseSpdSolver, 6, None),
         "convdiff3d_16_lu_mb8_block128": (lambda: convdiff3d(16), SparseSolver, 8,
                                           dict(block_cols=128)),
